@@ -1,0 +1,60 @@
+/*
+ * oracle.h -- CPU restatement of the reference 2D FMM (test infrastructure).
+ *
+ * THIS IS THE CHECKER, NOT THE PRODUCT.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load liboracle.so.  The HIP engine in
+ * fmm2d_c_translation_amd/ never links or calls anything here.
+ *
+ * Every routine restates the algorithm of one reference routine and cites
+ * it (paths relative to the reference repo).  Box ids are 1-based with -1
+ * meaning "none", exactly the value convention of the reference itree, so
+ * trees and lists can be compared element by element.  Arrays indexed by a
+ * box id have a dummy slot 0.
+ */
+#ifndef FMM2D_ORACLE_H
+#define FMM2D_ORACLE_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef double _Complex cplx;
+
+typedef struct {
+    int nlevels;      /* deepest level index */
+    int nboxes;
+    int cap;          /* allocated box slots (ids 1..cap-1) */
+    int *laddr;       /* laddr[2*l], laddr[2*l+1]: first/last box of level l */
+    int *ilevel, *iparent, *nchild, *ichild; /* ichild[4*id + (j-1)] */
+    int *nnbors, *nbors;                     /* nbors[9*id + (i-1)] */
+    double *centers;                         /* centers[2*id + 0/1] */
+    double *boxsize;                         /* boxsize[l], l = 0..nlevels */
+    /* point partition of the final tree (pts_tree_sort) */
+    int *isrc, *isrcse;                      /* isrc: 1-based perm; isrcse[2*id] */
+    int *itarg, *itargse;
+} otree;
+
+typedef struct {
+    int mn1, mn2, mn3, mn4;          /* 13, 27, 20, 5 */
+    int *n1, *l1, *n2, *l2, *n3, *l3, *n4, *l4; /* n*[id], l*[mn*id + (i-1)] */
+} olists;
+
+/* tree + lists (otree.c) */
+void otree_build(const double *src, int ns, const double *targ, int nt,
+                 int idivflag, int ndiv, int nlmin, int nlmax, int ifunif,
+                 otree *t);
+void otree_free(otree *t);
+void olists_build(const otree *t, olists *L);
+void olists_free(olists *L);
+
+/* parameter selection (oparams.c) */
+int  o_lndiv(double eps, int ns, int nt);
+int  o_l2dterms(double eps);
+int  o_next235(double base);
+
+/* special functions (ospecial.c) */
+void o_hank01(cplx z, cplx *h0, cplx *h1);
+void o_h2dall(int nterms, cplx z, double rscale, cplx *hvec);
+void o_jbessel(int nterms, cplx z, double rscale, cplx *fjs);
+int  o_h2dterms(double bsize, cplx zk, double eps, int *ier);
+
+#endif
