@@ -1,0 +1,413 @@
+// abi.hip -- the C-ABI layer: gfortran-ABI drop-in entry points and the
+// device-resident variants.  Host pointers are staged through HBM; all
+// compute runs in the device engine (points.hip, tree.cpp, lap.hip).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+
+#include "../../include/fmm2d_hip.h"
+#include "engine.h"
+
+namespace fmm {
+namespace {
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+// Complex-density log kernel as two Cauchy channels per density
+// (lfmm2d_ndiv.c:139-161): charge1 = (Re c, Im c), dipstr1 = -(v1 + i v2) (Re d, Im d).
+__global__ void k_lfmm_prep(const double2* chg, const double2* dip, const double* dipvec,
+                            const int* perm, int n, int nd, double2* chg1, double2* dip1) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int s = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    if (chg1) {
+      double2 c = chg[(size_t)s * nd + d];
+      chg1[(size_t)(2 * d) * n + i] = make_double2(c.x, 0.0);
+      chg1[(size_t)(2 * d + 1) * n + i] = make_double2(c.y, 0.0);
+    }
+    if (dip1) {
+      double2 q = dip[(size_t)s * nd + d];
+      double v1 = dipvec[((size_t)s * 2 + 0) * nd + d], v2 = dipvec[((size_t)s * 2 + 1) * nd + d];
+      dip1[(size_t)(2 * d) * n + i] = make_double2(-q.x * v1, -q.x * v2);
+      dip1[(size_t)(2 * d + 1) * n + i] = make_double2(-q.y * v1, -q.y * v2);
+    }
+  }
+}
+
+// unpack (lfmm2d_ndiv.c:171-243): pot = Re ch1 + i Re ch2; grad = (Re, -Im) of
+// d/dz; hess = (Re, -Im, -Re) of d2/dz2; scattered to original order
+__global__ void k_lfmm_unpack(const double2* P, const double2* G, const double2* H,
+                              const int* perm, int n, int nd, int pg, double2* pot,
+                              double2* grad, double2* hess) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int t = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    size_t a = (size_t)(2 * d) * n + i, b = (size_t)(2 * d + 1) * n + i;
+    pot[(size_t)t * nd + d] = make_double2(P[a].x, P[b].x);
+    if (pg >= 2) {
+      grad[((size_t)t * 2 + 0) * nd + d] = make_double2(G[a].x, G[b].x);
+      grad[((size_t)t * 2 + 1) * nd + d] = make_double2(-G[a].y, -G[b].y);
+    }
+    if (pg >= 3) {
+      hess[((size_t)t * 3 + 0) * nd + d] = make_double2(H[a].x, H[b].x);
+      hess[((size_t)t * 3 + 1) * nd + d] = make_double2(-H[a].y, -H[b].y);
+      hess[((size_t)t * 3 + 2) * nd + d] = make_double2(-H[a].x, -H[b].x);
+    }
+  }
+}
+
+struct LapReq {
+  bool logk = false;  // lfmm (log kernel, complex densities) vs cfmm (Cauchy)
+  int nd = 1;
+  double eps = 1e-6;
+  int ns = 0, nt = 0;
+  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
+  int ifc = 0, ifd = 0;
+  const void *chg = nullptr, *dip = nullptr;
+  int pg = 0, pgt = 0;
+  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
+  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
+  int ndiv = 0, idivflag = 0, ifnear = 1;
+  bool device = false;
+  hipStream_t stream = nullptr;
+};
+
+template <class T>
+int upload(Scratch& S, hipStream_t st, const T* h, size_t n, const T** d) {
+  T* p = S.get<T>(n);
+  if (S.err) return S.err;
+  if (n) HIP_TRY(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+  *d = p;
+  return IER_OK;
+}
+
+int run_lap(const LapReq& r) {
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  if (r.stream) c.st = r.stream;
+  if (r.nd < 1 || r.ns < 1 || r.nt < 0 || r.pg < 0 || r.pg > 3 || r.pgt < 0 || r.pgt > 3)
+    return IER_ARGS;
+  Scratch S(c.st);
+  const int nd = r.nd, ns = r.ns, nt = r.nt;
+  const int ncomp_g = r.logk ? 2 : 1, ncomp_h = r.logk ? 3 : 1;
+  const double *src = r.src, *trg = r.trg, *dv = r.dipvec;
+  const double2 *chg = static_cast<const double2*>(r.chg), *dip = static_cast<const double2*>(r.dip);
+  if (!r.device) {
+    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &src))) return e;
+    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &trg))) return e;
+    if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
+    if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
+    if (r.logk && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
+  }
+  SortedPoints P;
+  Plan plan;
+  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt);
+  if (e) return e;
+  const int ndc = r.logk ? 2 * nd : nd;
+  LapArgs a;
+  a.nd = ndc;
+  a.eps = r.eps;
+  a.ifcharge = r.ifc;
+  a.ifdipole = r.ifd;
+  a.ifpgh = r.pg;
+  a.ifpghtarg = r.pgt;
+  a.ifnear = r.ifnear;
+  double2* cs = r.ifc ? S.get<double2>((size_t)ndc * ns) : nullptr;
+  double2* qs = r.ifd ? S.get<double2>((size_t)ndc * ns) : nullptr;
+  if (S.err) return S.err;
+  if (r.logk) {
+    k_lfmm_prep<<<nblk(ns, 256), 256, 0, c.st>>>(chg, dip, dv, P.perm_s, ns, nd, cs, qs);
+  } else {
+    if (cs) gather_channels(c.st, chg, cs, P.perm_s, ns, nd, 1);
+    if (qs) gather_channels(c.st, dip, qs, P.perm_s, ns, nd, 1);
+  }
+  a.chg = cs;
+  a.dip = qs;
+  auto zeros = [&](int flag, int need, int n) -> double2* {
+    if (flag < need) return nullptr;
+    size_t cnt = (size_t)ndc * std::max(n, 1);
+    double2* p = S.get<double2>(cnt);
+    if (p) (void)hipMemsetAsync(p, 0, sizeof(double2) * cnt, c.st);
+    return p;
+  };
+  a.pot = zeros(r.pg, 1, ns); a.grad = zeros(r.pg, 2, ns); a.hess = zeros(r.pg, 3, ns);
+  a.pott = zeros(r.pgt, 1, nt); a.gradt = zeros(r.pgt, 2, nt); a.hesst = zeros(r.pgt, 3, nt);
+  if (S.err) return S.err;
+  e = lap_run(c, S, P, plan, a);
+  if (e) return e;
+
+  // results in original order, in device memory (caller's, or staging)
+  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh; };
+  Out outs[2] = {{r.pg, ns, P.perm_s, a.pot, a.grad, a.hess, r.pot, r.grad, r.hess},
+                 {r.pgt, nt, P.perm_t, a.pott, a.gradt, a.hesst, r.pott, r.gradt, r.hesst}};
+  for (auto& o : outs) {
+    if (o.flag <= 0 || o.n == 0) continue;
+    size_t np = (size_t)nd * o.n, ng = np * ncomp_g, nh = np * ncomp_h;
+    double2 *dp = static_cast<double2*>(o.up), *dg = static_cast<double2*>(o.ug),
+            *dh = static_cast<double2*>(o.uh);
+    if (!r.device) {
+      dp = S.get<double2>(np);
+      dg = o.flag >= 2 ? S.get<double2>(ng) : nullptr;
+      dh = o.flag >= 3 ? S.get<double2>(nh) : nullptr;
+      if (S.err) return S.err;
+    }
+    if (r.logk) {
+      k_lfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag,
+                                                      dp, dg, dh);
+    } else {
+      scatter_channels(c.st, o.sp, dp, o.perm, o.n, nd, 1);
+      if (o.flag >= 2) scatter_channels(c.st, o.sg, dg, o.perm, o.n, nd, 1);
+      if (o.flag >= 3) scatter_channels(c.st, o.sh, dh, o.perm, o.n, nd, 1);
+    }
+    if (!r.device) {
+      HIP_TRY(hipMemcpyAsync(o.up, dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, sizeof(double2) * ng, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, sizeof(double2) * nh, hipMemcpyDeviceToHost, c.st));
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
+}
+
+int guarded(const LapReq& r) {
+  try {
+    return run_lap(r);
+  } catch (const std::bad_alloc&) {
+    return IER_ALLOC;
+  } catch (...) {
+    return IER_HIP;
+  }
+}
+
+void zero8(double* t) {
+  if (t)
+    for (int i = 0; i < 8; ++i) t[i] = 0.0;
+}
+
+}  // namespace
+}  // namespace fmm
+
+using fmm::LapReq;
+
+extern "C" {
+
+void cfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns,
+                  const double* sources, const fmm2d_int* ifcharge, const fmm2d_cplx* charge,
+                  const fmm2d_int* ifdipole, const fmm2d_cplx* dipstr, fmm2d_int* iper,
+                  const fmm2d_int* ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess,
+                  const fmm2d_int* nt, const double* targ, const fmm2d_int* ifpghtarg,
+                  fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg,
+                  const fmm2d_int* ndiv, const fmm2d_int* idivflag, const fmm2d_int* ifnear,
+                  double* timeinfo, fmm2d_int* ier) {
+  *iper = 0;
+  fmm::zero8(timeinfo);
+  LapReq r;
+  r.nd = *nd; r.eps = *eps; r.ns = *ns; r.nt = *nt; r.src = sources; r.trg = targ;
+  r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr;
+  r.pg = *ifpgh; r.pgt = *ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear;
+  *ier = fmm::guarded(r);
+}
+
+void cfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns, const double* sources,
+             const fmm2d_int* ifcharge, const fmm2d_cplx* charge, const fmm2d_int* ifdipole,
+             const fmm2d_cplx* dipstr, fmm2d_int* iper, const fmm2d_int* ifpgh, fmm2d_cplx* pot,
+             fmm2d_cplx* grad, fmm2d_cplx* hess, const fmm2d_int* nt, const double* targ,
+             const fmm2d_int* ifpghtarg, fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg,
+             fmm2d_cplx* hesstarg, fmm2d_int* ier) {
+  fmm2d_int ndiv = fmm::ndiv_rule(*eps, *ns, *nt), idiv = 0, ifnear = 1;
+  double ti[8];
+  cfmm2d_ndiv_(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, iper, ifpgh, pot, grad,
+               hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg, &ndiv, &idiv, &ifnear, ti,
+               ier);
+}
+
+void lfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns,
+                  const double* sources, const fmm2d_int* ifcharge, const fmm2d_cplx* charge,
+                  const fmm2d_int* ifdipole, const fmm2d_cplx* dipstr, const double* dipvec,
+                  fmm2d_int* iper, const fmm2d_int* ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+                  fmm2d_cplx* hess, const fmm2d_int* nt, const double* targ,
+                  const fmm2d_int* ifpghtarg, fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg,
+                  fmm2d_cplx* hesstarg, const fmm2d_int* ndiv, const fmm2d_int* idivflag,
+                  const fmm2d_int* ifnear, double* timeinfo, fmm2d_int* ier) {
+  *iper = 0;
+  fmm::zero8(timeinfo);
+  LapReq r;
+  r.logk = true;
+  r.nd = *nd; r.eps = *eps; r.ns = *ns; r.nt = *nt; r.src = sources; r.trg = targ;
+  r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
+  r.pg = *ifpgh; r.pgt = *ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear;
+  *ier = fmm::guarded(r);
+}
+
+void lfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns, const double* sources,
+             const fmm2d_int* ifcharge, const fmm2d_cplx* charge, const fmm2d_int* ifdipole,
+             const fmm2d_cplx* dipstr, const double* dipvec, fmm2d_int* iper,
+             const fmm2d_int* ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess,
+             const fmm2d_int* nt, const double* targ, const fmm2d_int* ifpghtarg,
+             fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_int* ier) {
+  fmm2d_int ndiv = fmm::ndiv_rule(*eps, *ns, *nt), idiv = 0, ifnear = 1;
+  double ti[8];
+  lfmm2d_ndiv_(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec, iper, ifpgh, pot,
+               grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg, &ndiv, &idiv,
+               &ifnear, ti, ier);
+}
+
+void hndiv2d_(const double* eps, const fmm2d_int* ns, const fmm2d_int* nt, const fmm2d_int*,
+              const fmm2d_int*, const fmm2d_int*, const fmm2d_int*, fmm2d_int* ndiv,
+              fmm2d_int* idivflag) {
+  *idivflag = 0;
+  *ndiv = fmm::ndiv_rule(*eps, *ns, *nt);
+}
+
+void lndiv2d_(const double* eps, const fmm2d_int* ns, const fmm2d_int* nt, const fmm2d_int*,
+              const fmm2d_int*, const fmm2d_int*, const fmm2d_int*, fmm2d_int* ndiv,
+              fmm2d_int* idivflag) {
+  *idivflag = 0;
+  *ndiv = fmm::ndiv_rule(*eps, *ns, *nt);
+}
+
+void l2dterms_(const double* eps, fmm2d_int* nterms, fmm2d_int* ier) {
+  *ier = 0;
+  *nterms = fmm::l2dterms(*eps);
+}
+
+void fmm2dhip_cfmm2d_dev(int nd, double eps, int ns, const double* sources, int ifcharge,
+                         const void* charge, int ifdipole, const void* dipstr, int ifpgh,
+                         void* pot, void* grad, void* hess, int nt, const double* targ,
+                         int ifpghtarg, void* pottarg, void* gradtarg, void* hesstarg, int ndiv,
+                         int idivflag, int ifnear, double* timeinfo, void* stream, int* ier) {
+  fmm::zero8(timeinfo);
+  LapReq r;
+  r.device = true;
+  r.stream = static_cast<hipStream_t>(stream);
+  r.nd = nd; r.eps = eps; r.ns = ns; r.nt = nt; r.src = sources; r.trg = targ;
+  r.ifc = ifcharge; r.ifd = ifdipole; r.chg = charge; r.dip = dipstr;
+  r.pg = ifpgh; r.pgt = ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear;
+  *ier = fmm::guarded(r);
+}
+
+void fmm2dhip_lfmm2d_dev(int nd, double eps, int ns, const double* sources, int ifcharge,
+                         const void* charge, int ifdipole, const void* dipstr,
+                         const double* dipvec, int ifpgh, void* pot, void* grad, void* hess,
+                         int nt, const double* targ, int ifpghtarg, void* pottarg,
+                         void* gradtarg, void* hesstarg, int ndiv, int idivflag, int ifnear,
+                         double* timeinfo, void* stream, int* ier) {
+  fmm::zero8(timeinfo);
+  LapReq r;
+  r.logk = true;
+  r.device = true;
+  r.stream = static_cast<hipStream_t>(stream);
+  r.nd = nd; r.eps = eps; r.ns = ns; r.nt = nt; r.src = sources; r.trg = targ;
+  r.ifc = ifcharge; r.ifd = ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
+  r.pg = ifpgh; r.pgt = ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear;
+  *ier = fmm::guarded(r);
+}
+
+int fmm2dhip_test_tree(const double* src, int ns, const double* targ, int nt, int idivflag,
+                       int ndiv, int* nlevels, int* nboxes, int* itree, double* centers,
+                       double* boxsize, int* isrc, int* isrcse, int* itarg, int* itargse,
+                       int* nlist, int* lists) {
+  try {
+    double xmin = src[0], xmax = src[0], ymin = src[1], ymax = src[1];
+    for (int pass = 0; pass < 2; ++pass) {
+      const double* a = pass ? targ : src;
+      int n = pass ? nt : ns;
+      for (int i = 0; i < n; ++i) {
+        xmin = std::min(xmin, a[2 * i]); xmax = std::max(xmax, a[2 * i]);
+        ymin = std::min(ymin, a[2 * i + 1]); ymax = std::max(ymax, a[2 * i + 1]);
+      }
+    }
+    double bs0 = std::max(xmax - xmin, ymax - ymin), cx = (xmin + xmax) / 2, cy = (ymin + ymax) / 2;
+    auto sorted = [&](const double* a, int n, std::vector<uint64_t>& k, std::vector<int>& idx) {
+      std::vector<uint64_t> raw(n);
+      for (int i = 0; i < n; ++i) raw[i] = fmm::point_key(a[2 * i], a[2 * i + 1], cx, cy, bs0);
+      idx.resize(n);
+      std::iota(idx.begin(), idx.end(), 0);
+      std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return raw[x] < raw[y]; });
+      k.resize(n);
+      for (int i = 0; i < n; ++i) k[i] = raw[idx[i]];
+    };
+    std::vector<uint64_t> ks, kt;
+    std::vector<int> is, it;
+    sorted(src, ns, ks, is);
+    sorted(targ, nt, kt, it);
+    fmm::Tree T;
+    int e = fmm::build_tree(ks.data(), ns, kt.data(), nt, cx, cy, bs0, idivflag, ndiv, T);
+    if (e) return e;
+    *nlevels = T.nlevels;
+    *nboxes = T.nboxes;
+    if (!itree) return 0;
+    fmm::build_lists(T);
+    const int n = T.nboxes, L = T.nlevels;
+    int k = 0;
+    auto id1 = [](int b) { return b >= 0 ? b + 1 : -1; };
+    for (int l = 0; l <= L; ++l) {
+      itree[k++] = T.lfirst[l] + 1;
+      itree[k++] = T.lfirst[l] + T.lcount[l];
+    }
+    for (int b = 0; b < n; ++b) itree[k++] = T.level[b];
+    for (int b = 0; b < n; ++b) itree[k++] = id1(T.parent[b]);
+    for (int b = 0; b < n; ++b) itree[k++] = T.nchild[b];
+    for (int b = 0; b < n; ++b)
+      for (int q = 0; q < 4; ++q) itree[k++] = id1(T.child[4 * b + q]);
+    for (int b = 0; b < n; ++b) itree[k++] = T.nnb[b];
+    for (int b = 0; b < n; ++b)
+      for (int i = 0; i < 9; ++i) itree[k++] = id1(T.nbr[9 * b + i]);
+    for (int b = 0; b < n; ++b) {
+      centers[2 * b] = T.cx[b];
+      centers[2 * b + 1] = T.cy[b];
+      isrcse[2 * b] = T.s0[b] + 1;
+      isrcse[2 * b + 1] = T.s1[b];
+      itargse[2 * b] = T.t0[b] + 1;
+      itargse[2 * b + 1] = T.t1[b];
+      nlist[4 * b + 0] = T.n1[b]; nlist[4 * b + 1] = T.n2[b];
+      nlist[4 * b + 2] = T.n3[b]; nlist[4 * b + 3] = T.n4[b];
+      int* o = lists + 65 * b;
+      for (int i = 0; i < 13; ++i) o[i] = i < T.n1[b] ? id1(T.l1[13 * b + i]) : 0;
+      for (int i = 0; i < 27; ++i) o[13 + i] = i < T.n2[b] ? id1(T.l2[27 * b + i]) : 0;
+      for (int i = 0; i < 20; ++i) o[40 + i] = i < T.n3[b] ? id1(T.l3[20 * b + i]) : 0;
+      for (int i = 0; i < 5; ++i) o[60 + i] = i < T.n4[b] ? id1(T.l4[5 * b + i]) : 0;
+    }
+    for (int l = 0; l <= L; ++l) boxsize[l] = T.bs[l];
+    // original-index order inside each leaf
+    for (int b = 0; b < n; ++b) {
+      if (T.nchild[b]) continue;
+      std::sort(is.begin() + T.s0[b], is.begin() + T.s1[b]);
+      std::sort(it.begin() + T.t0[b], it.begin() + T.t1[b]);
+    }
+    for (int i = 0; i < ns; ++i) isrc[i] = is[i] + 1;
+    for (int i = 0; i < nt; ++i) itarg[i] = it[i] + 1;
+    return 0;
+  } catch (...) {
+    return fmm::IER_ALLOC;
+  }
+}
+
+int fmm2dhip_device_info(char* name, int len) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fmm::IER_NOGPU;
+  hipDeviceProp_t p;
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || hipGetDeviceProperties(&p, d) != hipSuccess)
+    return fmm::IER_NOGPU;
+  if (name && len > 0) {
+    std::strncpy(name, p.gcnArchName, (size_t)len - 1);
+    name[len - 1] = 0;
+  }
+  return std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 0 : fmm::IER_NOGPU;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// common.h -- shared types and helpers for the MI355X 2D FMM engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+
+#define FMM_HD __host__ __device__ inline
+
+namespace fmm {
+
+// Error codes written to `ier`.  0/4/13 keep their reference meaning
+// (success / workspace allocation failure / expansion order over 10000,
+// c_translation/src/hfmm2d.c:1810-1815, h2dterms.c:62-64); codes >= 100
+// are this engine's own and are documented in include/fmm2d_hip.h.
+enum : int {
+  IER_OK = 0,
+  IER_ALLOC = 4,
+  IER_NTERMS = 13,
+  IER_HIP = 101,        // a HIP runtime call failed
+  IER_ARGS = 102,       // argument outside what the engine supports
+  IER_DEPTH = 103,      // tree would need more than kMaxDepth levels
+  IER_NOGPU = 104,      // no usable gfx950 device
+  IER_HIGHFREQ = 105,   // Helmholtz box size above 16 wavelengths (diagonal form)
+  IER_P_TOO_LARGE = 106 // expansion order beyond the kernels' register/LDS budget
+};
+
+// Keys hold 2 bits per level below the root: kMaxDepth levels in 58 bits.
+constexpr int kMaxDepth = 29;
+
+struct cd {  // complex double, 16-byte aligned, layout of double _Complex
+  double x, y;
+};
+
+FMM_HD cd mk(double a, double b) { return cd{a, b}; }
+FMM_HD cd operator+(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+FMM_HD cd operator-(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+FMM_HD cd operator-(cd a) { return cd{-a.x, -a.y}; }
+FMM_HD cd operator*(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+FMM_HD cd operator*(cd a, double s) { return cd{a.x * s, a.y * s}; }
+FMM_HD cd operator*(double s, cd a) { return cd{a.x * s, a.y * s}; }
+FMM_HD cd operator/(cd a, double s) { return cd{a.x / s, a.y / s}; }
+FMM_HD cd& operator+=(cd& a, cd b) { a.x += b.x; a.y += b.y; return a; }
+FMM_HD cd& operator-=(cd& a, cd b) { a.x -= b.x; a.y -= b.y; return a; }
+FMM_HD cd conj(cd a) { return cd{a.x, -a.y}; }
+FMM_HD double norm2(cd a) { return a.x * a.x + a.y * a.y; }
+// 1/z by Smith's scaling: no overflow for |z| near the double range ends.
+FMM_HD cd inv(cd z) {
+  if (fabs(z.x) >= fabs(z.y)) {
+    double r = z.y / z.x, d = z.x + z.y * r;
+    return cd{1.0 / d, -r / d};
+  }
+  double r = z.x / z.y, d = z.y + z.x * r;
+  return cd{r / d, -1.0 / d};
+}
+FMM_HD cd cdiv(cd a, cd b) { return a * inv(b); }
+
+// Quadrant digit of a point relative to a box centre: bit 0 set for the
+// right half (x - cx >= 0), bit 1 for the upper half (y - cy >= 0).  Child
+// j = digit + 1 in the reference numbering (-x,-y), (+x,-y), (-x,+y), (+x,+y)
+// (c_translation/src/tree_routs2d.c:77-92, pts_tree2d.c:700-745).
+FMM_HD int quadrant(double x, double y, double cx, double cy) {
+  return (x - cx < 0 ? 0 : 1) | (y - cy < 0 ? 0 : 2);
+}
+
+// Child centre from the parent centre and the child box size, the same
+// floating-point operations as the reference's tree refinement
+// (tree_routs2d.c:81-84): c + (+-1) * bs / 2.
+FMM_HD double child_coord(double c, int positive, double bs_child) {
+  return c + (positive ? 1.0 : -1.0) * bs_child / 2;
+}
+
+// Morton path key of a point: digit of level l (1-based) in bits
+// 2*(kMaxDepth-l) .. +1, obtained by walking down from the root centre with
+// the reference's centre recurrence, so the digit at every level is exactly
+// the child the reference's stable partition would put the point in.
+FMM_HD uint64_t point_key(double x, double y, double cx, double cy, double bs0) {
+  uint64_t key = 0;
+  double bs = bs0;
+  for (int l = 1; l <= kMaxDepth; ++l) {
+    int q = quadrant(x, y, cx, cy);
+    bs = bs / 2;
+    cx = child_coord(cx, q & 1, bs);
+    cy = child_coord(cy, q & 2, bs);
+    key |= (uint64_t)q << (2 * (kMaxDepth - l));
+  }
+  return key;
+}
+
+FMM_HD uint64_t box_key_lo(uint64_t prefix, int level) {
+  return prefix << (2 * (kMaxDepth - level));
+}
+
+}  // namespace fmm
+
+#define HIP_TRY(expr)                         \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) return fmm::IER_HIP; \
+  } while (0)

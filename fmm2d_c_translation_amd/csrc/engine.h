@@ -1,0 +1,112 @@
+// engine.h -- internal interface between the C-ABI layer and the device engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "tree.h"
+
+namespace fmm {
+
+// Stream-ordered device allocations released when the object dies.
+class Scratch {
+ public:
+  explicit Scratch(hipStream_t s) : st_(s) {}
+  ~Scratch() {
+    for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
+  }
+  template <class T>
+  T* get(size_t n) {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, (n ? n : 1) * sizeof(T), st_) != hipSuccess) {
+      err = IER_ALLOC;
+      return nullptr;
+    }
+    ptrs_.push_back(p);
+    return static_cast<T*>(p);
+  }
+  int err = IER_OK;
+
+ private:
+  hipStream_t st_;
+  std::vector<void*> ptrs_;
+};
+
+// Tree arrays on the device (0-based box ids, lists in CSR form).
+struct DTree {
+  int nb = 0;
+  const double2* ctr = nullptr;
+  const int* level = nullptr;
+  const double* bs = nullptr;    // box size per level
+  const int* nchild = nullptr;
+  const int4* child = nullptr;
+  const int2* srng = nullptr;  // sorted source range [x, y)
+  const int2* trng = nullptr;  // sorted target range
+  const int *o1 = nullptr, *l1 = nullptr, *o2 = nullptr, *l2 = nullptr;
+  const int *o3 = nullptr, *l3 = nullptr, *o4 = nullptr, *l4 = nullptr;
+};
+
+// Output of the point-sorting front end shared by every kernel family.
+struct SortedPoints {
+  int ns = 0, nt = 0;
+  double bs0 = 0, c0x = 0, c0y = 0;
+  int* perm_s = nullptr;  // sorted position -> original index
+  int* perm_t = nullptr;
+  double2* src = nullptr;  // sorted coordinates
+  double2* trg = nullptr;
+};
+
+// Host tree + its device copy + per-level work lists.
+struct Plan {
+  Tree T;
+  DTree D;
+  std::vector<int> work;          // host staging of all work lists (concatenated)
+  int* dwork = nullptr;           // device copy
+  struct Span { int off = 0, n = 0; };
+  Span formmp;                    // leaves at levels >= 2 with sources
+  std::vector<Span> m2m;          // per level: boxes with children
+  std::vector<Span> m2l;          // per level >= 2: boxes with targets and list2
+  std::vector<Span> l2l;          // per level: boxes with targets and children
+  Span list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
+  Span leafsort_s, leafsort_t;    // leaves holding sources / targets
+};
+
+struct Ctx {
+  hipStream_t st = nullptr;
+  int device = 0;
+};
+int get_ctx(Ctx& c);
+
+// Point front end: bounding box, keys, sort, tree, lists, sorted coordinates.
+int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_trg, int nt,
+               int idivflag, int ndiv, SortedPoints& P, Plan& plan, bool want_lap_work,
+               int ifpgh, int ifpghtarg);
+
+// Gathers (nd-fastest, original order) strengths into channel-major sorted
+// order: out[(c*nd + d)*n + i] = in[(perm[i]*ncomp + c)*nd + d].
+void gather_channels(hipStream_t st, const double2* in, double2* out, const int* perm, int n,
+                     int nd, int ncomp);
+void gather_real(hipStream_t st, const double* in, double* out, const int* perm, int n, int nd,
+                 int ncomp);
+// Inverse: out[(perm[i]*ncomp + c)*nd + d] = in[(c*nd + d)*n + i].
+void scatter_channels(hipStream_t st, const double2* in, double2* out, const int* perm, int n,
+                      int nd, int ncomp);
+
+// Cauchy-kernel FMM on sorted, device-resident data (cfmm2dmain semantics).
+struct LapArgs {
+  int nd = 1;
+  double eps = 1e-6;
+  int ifcharge = 0, ifdipole = 0, ifpgh = 0, ifpghtarg = 0, ifnear = 1;
+  const double2 *chg = nullptr, *dip = nullptr;      // sorted, channel-major
+  double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources
+  double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
+};
+int lap_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const LapArgs& a);
+
+int l2dterms(double eps);
+int ndiv_rule(double eps, int ns, int nt);
+
+}  // namespace fmm

@@ -1,0 +1,162 @@
+/*
+ * fmm2d_hip.h -- C ABI of the MI355X 2D FMM engine (libfmm2d_hip.so).
+ *
+ * Drop-in entry points: the gfortran-ABI symbols of the reference C
+ * translation (lowercase, trailing underscore, every argument by pointer,
+ * column-major arrays with nd fastest; c_translation/include/fmm2d_c.h:35-61).
+ * Each declaration cites the reference routine it replaces.  Argument
+ * meaning and output layout are unchanged:
+ *   - outputs are overwritten, unrequested outputs are not touched;
+ *   - *iper is written 0; timeinfo[8] receives per-step device times (s);
+ *   - *ier: 0 ok, 4 device allocation failure, 13 Helmholtz expansion order
+ *     above 10000 (reference meanings), and this engine's own codes
+ *       101 HIP runtime error          102 unsupported argument
+ *       103 tree deeper than 29 levels 104 no usable device
+ *       105 Helmholtz boxes above 16 wavelengths (high-frequency diagonal
+ *           form not built)            106 expansion order above 63.
+ *     The library never aborts or prints.
+ * Calls are synchronous and thread-safe (one HIP stream per calling thread).
+ */
+#ifndef FMM2D_HIP_H
+#define FMM2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> fmm2d_cplx;
+extern "C" {
+#else
+typedef double _Complex fmm2d_cplx;
+#endif
+
+typedef int32_t fmm2d_int;
+
+/* ---- Laplace, Cauchy form: pot = sum c log|z| + d/z, grad = d/dz, hess = d2/dz2 */
+
+/* replaces cfmm2d_ (c_translation/src/cfmm2d.c:1053; leaf size from lndiv2d,
+ * src/laplace/lndiv2d.f:17-35) */
+void cfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
+             const double *sources, const fmm2d_int *ifcharge, const fmm2d_cplx *charge,
+             const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr, fmm2d_int *iper,
+             const fmm2d_int *ifpgh, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess,
+             const fmm2d_int *nt, const double *targ, const fmm2d_int *ifpghtarg,
+             fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg, fmm2d_cplx *hesstarg, fmm2d_int *ier);
+
+/* replaces cfmm2d_ndiv_ (c_translation/src/cfmm2d_ndiv.c:94) */
+void cfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
+                  const double *sources, const fmm2d_int *ifcharge, const fmm2d_cplx *charge,
+                  const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr, fmm2d_int *iper,
+                  const fmm2d_int *ifpgh, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess,
+                  const fmm2d_int *nt, const double *targ, const fmm2d_int *ifpghtarg,
+                  fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg, fmm2d_cplx *hesstarg,
+                  const fmm2d_int *ndiv, const fmm2d_int *idivflag, const fmm2d_int *ifnear,
+                  double *timeinfo, fmm2d_int *ier);
+
+/* ---- Laplace, log kernel with complex densities: grad (nd,2,n), hess (nd,3,n) */
+
+/* replaces lfmm2d_ (src/laplace/lfmm2d.f:29-32; the C translation calls it
+ * through cfmm2d, lfmm2d_ndiv.c:166) */
+void lfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
+             const double *sources, const fmm2d_int *ifcharge, const fmm2d_cplx *charge,
+             const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr, const double *dipvec,
+             fmm2d_int *iper, const fmm2d_int *ifpgh, fmm2d_cplx *pot, fmm2d_cplx *grad,
+             fmm2d_cplx *hess, const fmm2d_int *nt, const double *targ,
+             const fmm2d_int *ifpghtarg, fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg,
+             fmm2d_cplx *hesstarg, fmm2d_int *ier);
+
+/* replaces lfmm2d_ndiv_ (c_translation/src/lfmm2d_ndiv.c:57) */
+void lfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
+                  const double *sources, const fmm2d_int *ifcharge, const fmm2d_cplx *charge,
+                  const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr, const double *dipvec,
+                  fmm2d_int *iper, const fmm2d_int *ifpgh, fmm2d_cplx *pot, fmm2d_cplx *grad,
+                  fmm2d_cplx *hess, const fmm2d_int *nt, const double *targ,
+                  const fmm2d_int *ifpghtarg, fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg,
+                  fmm2d_cplx *hesstarg, const fmm2d_int *ndiv, const fmm2d_int *idivflag,
+                  const fmm2d_int *ifnear, double *timeinfo, fmm2d_int *ier);
+
+/* ---- parameter helpers */
+
+/* replaces hndiv2d_ (c_translation/src/hndiv2d.c:15) */
+void hndiv2d_(const double *eps, const fmm2d_int *ns, const fmm2d_int *nt,
+              const fmm2d_int *ifcharge, const fmm2d_int *ifdipole, const fmm2d_int *ifpgh,
+              const fmm2d_int *ifpghtarg, fmm2d_int *ndiv, fmm2d_int *idivflag);
+/* replaces lndiv2d_ (src/laplace/lndiv2d.f:1; never translated to C) */
+void lndiv2d_(const double *eps, const fmm2d_int *ns, const fmm2d_int *nt,
+              const fmm2d_int *ifcharge, const fmm2d_int *ifdipole, const fmm2d_int *ifpgh,
+              const fmm2d_int *ifpghtarg, fmm2d_int *ndiv, fmm2d_int *idivflag);
+/* replaces l2dterms_ (c_translation/src/l2dterms.c:26) */
+void l2dterms_(const double *eps, fmm2d_int *nterms, fmm2d_int *ier);
+
+/* ---- direct sums on the GPU (the MEX c2ddir oracle, matlab/fmm2d.mw:1426-1457).
+ * Each replaces the routine of the same name in
+ * c_translation/src/cauchykernels2d.c:31-442: outputs are INCREMENTED,
+ * pairs closer than thresh are skipped with the routine's own comparison. */
+void c2d_directcp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, const double *thresh);
+void c2d_directcg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void c2d_directch_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void c2d_directdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *dipstr, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, const double *thresh);
+void c2d_directdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *dipstr, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void c2d_directdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                   const fmm2d_cplx *dipstr, const double *targ, const fmm2d_int *nt,
+                   fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void c2d_directcdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *targ,
+                    const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void c2d_directcdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *targ,
+                    const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad,
+                    const double *thresh);
+void c2d_directcdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *targ,
+                    const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess,
+                    const double *thresh);
+
+/* ---- device-resident variants (no counterpart in the reference).
+ * Same argument meaning as cfmm2d_ndiv_ / lfmm2d_ndiv_, but every array is a
+ * device pointer, and `stream` (a hipStream_t, or NULL for the library's
+ * own) orders the work.  For callers whose data already lives in HBM. */
+void fmm2dhip_cfmm2d_dev(int nd, double eps, int ns, const double *sources, int ifcharge,
+                         const void *charge, int ifdipole, const void *dipstr, int ifpgh,
+                         void *pot, void *grad, void *hess, int nt, const double *targ,
+                         int ifpghtarg, void *pottarg, void *gradtarg, void *hesstarg,
+                         int ndiv, int idivflag, int ifnear, double *timeinfo, void *stream,
+                         int *ier);
+void fmm2dhip_lfmm2d_dev(int nd, double eps, int ns, const double *sources, int ifcharge,
+                         const void *charge, int ifdipole, const void *dipstr,
+                         const double *dipvec, int ifpgh, void *pot, void *grad, void *hess,
+                         int nt, const double *targ, int ifpghtarg, void *pottarg,
+                         void *gradtarg, void *hesstarg, int ndiv, int idivflag, int ifnear,
+                         double *timeinfo, void *stream, int *ier);
+
+/* ---- test hook (no counterpart in the reference): builds the box-level
+ * tree and lists with keys computed and sorted on the CPU, and returns them
+ * in the reference itree layout (pts_tree2d.c:484-491, 1-based ids), plus
+ * the point permutations (1-based) and per-box point ranges, and lists as
+ * nlist(4,nboxes) and lists(65,nboxes) = list1(13) list2(27) list3(20)
+ * list4(5).  Call with itree == NULL to get the sizes.  The solver entry
+ * points never call it; it lets tests check the host half of the tree build
+ * on machines without a GPU. */
+int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, int idivflag,
+                       int ndiv, int *nlevels, int *nboxes, int *itree, double *centers,
+                       double *boxsize, int *isrc, int *isrcse, int *itarg, int *itargse,
+                       int *nlist, int *lists);
+
+/* library/device information for diagnostics: returns 0 when a gfx950
+ * device is usable and writes its name (at most len bytes). */
+int fmm2dhip_device_info(char *name, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMM2D_HIP_H */

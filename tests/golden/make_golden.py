@@ -1,0 +1,68 @@
+"""Writes tests/golden/*.npz: inputs and outputs of the reference C
+translation (compiled from its own sources into oracle/_ref by
+oracle/Makefile) for small seeded cases.  The reference ships no stored
+golden vectors (its tests compute expectations at run time), so these are
+the fixtures the oracle and the engine are pinned to where the reference
+library is absent.  Re-run after `make -C oracle ref`."""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+from fmm2d_c_translation_amd import abi  # noqa: E402
+
+
+def cases():
+    rng = np.random.default_rng(2024)
+    # (name, n, nt, nd, kind)
+    for name, n, nt, nd, kind in [("unif_small", 400, 150, 1, "unif"),
+                                  ("unif_nd2", 1200, 300, 2, "unif"),
+                                  ("circle", 3000, 0, 1, "circle")]:
+        if kind == "unif":
+            src = rng.random((2, n))
+        else:
+            th = rng.random(n) * 2 * np.pi
+            r = 0.4 + 1e-3 * rng.standard_normal(n)
+            src = np.stack([0.5 + r * np.cos(th), 0.5 + r * np.sin(th)])
+            src[:, : n // 10] = rng.random((2, n // 10))
+        tg = rng.random((2, nt)) if nt else None
+        ch = rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))
+        dp = rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))
+        dv = rng.standard_normal((nd, 2, n))
+        yield name, src, tg, ch, dp, dv, nd
+
+
+def main():
+    ref = oracle.ref_lib()
+    if ref is None:
+        raise SystemExit("reference library not built (make -C oracle ref)")
+    for name, src, tg, ch, dp, dv, nd in cases():
+        out = {"src": src, "charge": ch, "dipstr": dp, "dipvec": dv}
+        if tg is not None:
+            out["targ"] = tg
+        t = oracle.ref_tree(src, tg, 15, 0)
+        out.update(tree_itree=t.itree, tree_centers=t.centers, tree_boxsize=t.boxsize,
+                   tree_isrc=t.isrc, tree_isrcse=t.isrcse, tree_nlist=t.nlist, tree_lists=t.lists,
+                   tree_nlevels=t.nlevels, tree_nboxes=t.nboxes)
+        for eps in (1e-6, 1e-12):
+            tag = f"e{int(-np.log10(eps))}"
+            r = abi.cfmm2d_call(ref, "cfmm2d_ndiv_", eps, src, ch, dp, 3, tg, 3 if tg is not None else 0,
+                                nd, ndiv=15)
+            for k in ("pot", "grad", "hess", "targ_pot", "targ_grad", "targ_hess"):
+                if k in r:
+                    out[f"cfmm_{tag}_{k}"] = r[k]
+            r = abi.lfmm2d_call(ref, "lfmm2d_ndiv_", eps, src, ch, dp, dv, 3, tg,
+                                3 if tg is not None else 0, nd, ndiv=15)
+            for k in ("pot", "grad", "hess", "targ_pot", "targ_grad", "targ_hess"):
+                if k in r:
+                    out[f"lfmm_{tag}_{k}"] = r[k]
+        np.savez_compressed(os.path.join(HERE, f"lap_{name}.npz"), **out)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()
