@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X 2D FMM on one GPU.
+
+Workload (one "step" = one full FMM solve, inputs already resident in HBM):
+Helmholtz hfmm2d, ns = 1,000,000 sources and nt = 1,000,000 separate targets
+uniform in [0,1]^2 (seed 2), zk = 40, eps = 1e-9, complex charges N(0,1) plus
+dipoles with strengths N(0,1) and unit directions at angles U[0, 2 pi),
+potential and gradient at sources and targets, FP64.
+
+Prints one JSON line.  ``value`` is points per second over the whole job
+((ns + nt) / seconds per solve, in millions); ``roofline`` describes the
+near-field (list-1) kernel, the dominant one, from HIP events recorded by
+the library around each FMM step on the stream its kernels run on;
+``cpu_baseline`` times the OpenMP CPU restatement in oracle/ on a bounded
+sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+# Algorithmic flops per list-1 pair of the Helmholtz charge+dipole
+# potential+gradient kernel (H0 and H1 counted as 120 flops together).
+P2P_FLOPS_PER_PAIR = 180.0
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def make_inputs(n, nt, seed=2):
+    rng = np.random.default_rng(seed)
+    src = rng.random((2, n))
+    trg = rng.random((2, nt))
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dp = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    th = rng.random(n) * 2 * np.pi
+    dv = np.stack([np.cos(th), np.sin(th)])[None]  # (nd=1, 2, n)
+    return src, trg, ch, dp, dv
+
+
+def cpu_baseline(n_sample, zk, eps, threads):
+    """Time the oracle (C restatement, OpenMP over boxes) on n_sample sources
+    + n_sample targets of the same distribution; points/s scale linearly."""
+    import oracle
+    from fmm2d_c_translation_amd import abi
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    src, trg, ch, dp, dv = make_inputs(n_sample, n_sample, seed=2)
+    t0 = time.perf_counter()
+    r = abi.hfmm2d_call(oracle.oracle_lib(), "orc_hfmm2d_", eps, zk, src, ch, dp, dv, 2, trg, 2)
+    dt = time.perf_counter() - t0
+    assert r.ier == 0, r.ier
+    return 2 * n_sample / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; the multi-GPU path is a "
+                                   "design only (see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    import fmm2d_c_translation_amd as F
+
+    lib = F.lib()
+    rc, arch = F.device_name()
+    if rc != 0:
+        raise SystemExit(f"no gfx950 device ({arch})")
+    zk, eps = 40.0, 1e-9
+    n = nt = args.n
+    src, trg, ch, dp, dv = make_inputs(n, nt)
+    dev = torch.device("cuda:0")
+    # device-resident inputs, reference layouts (points (2,n), strengths (nd,n))
+    d_src = torch.from_numpy(np.ascontiguousarray(src.T)).to(dev)
+    d_trg = torch.from_numpy(np.ascontiguousarray(trg.T)).to(dev)
+    d_ch = torch.from_numpy(ch.astype(np.complex128)).to(dev)
+    d_dp = torch.from_numpy(dp.astype(np.complex128)).to(dev)
+    d_dv = torch.from_numpy(np.ascontiguousarray(np.transpose(dv, (2, 1, 0)))).to(dev)
+    d_pot = torch.zeros(n, dtype=torch.complex128, device=dev)
+    d_grad = torch.zeros(2 * n, dtype=torch.complex128, device=dev)
+    d_pott = torch.zeros(nt, dtype=torch.complex128, device=dev)
+    d_gradt = torch.zeros(2 * nt, dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ndiv = 20  # hndiv2d for eps = 1e-9 (c_translation/src/hndiv2d.c:31-47)
+    ti = np.zeros(8)
+    ier = C.c_int(0)
+    vp = C.c_void_p
+
+    def solve():
+        lib.fmm2dhip_hfmm2d_dev(1, C.c_double(eps), C.c_double(zk), C.c_double(0.0), n,
+                                vp(d_src.data_ptr()), 1, vp(d_ch.data_ptr()), 1,
+                                vp(d_dp.data_ptr()), vp(d_dv.data_ptr()), 2, vp(d_pot.data_ptr()),
+                                vp(d_grad.data_ptr()), None, nt, vp(d_trg.data_ptr()), 2,
+                                vp(d_pott.data_ptr()), vp(d_gradt.data_ptr()), None, ndiv, 0, 1,
+                                ti.ctypes.data_as(vp), vp(stream.cuda_stream), C.byref(ier))
+        if ier.value != 0:
+            raise RuntimeError(f"hfmm2d returned ier={ier.value}")
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    steps = []
+    p2p_s = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        solve()
+        steps.append(time.perf_counter() - a)
+        p2p_s.append(ti[7])
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    stats = np.zeros(15)
+    lib.fmm2dhip_last_stats(stats.ctypes.data_as(vp), 15)
+    ms = total / args.steps * 1e3
+    value = (n + nt) / (total / args.steps) / 1e6
+    pairs = stats[0] + stats[1]
+    p2p_time = float(np.mean(p2p_s))
+    achieved = pairs * P2P_FLOPS_PER_PAIR / p2p_time / 1e12
+    m2l_tf = stats[3] / stats[10] / 1e12 if stats[10] > 0 else None
+    out = {
+        "metric": "Mpts/s at eps 1e-6 mixed vs FP64 GPU FMM; p50/p99 served solve latency",
+        "value": round(value, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (uniform [0,1]^2, seeded)",
+        "config": {"workload": "hfmm2d C2: ns=nt=1e6 uniform, zk=40, eps=1e-9, charges+dipoles, "
+                               "pot+grad at sources and targets",
+                   "ns": n, "nt": nt, "zk": zk, "eps": eps, "ndiv": ndiv,
+                   "nboxes": int(stats[4]), "nlevels": int(stats[5]), "pmax": int(stats[6])},
+        "step_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
+            ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"], ti)},
+        "roofline": {"kernel": "k_hp2p (list-1 near field)", "bound": "mfma",
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "pairs_per_launch_avg": pairs / 2, "flops_per_pair": P2P_FLOPS_PER_PAIR,
+                     "m2l_tflops": None if m2l_tf is None else round(m2l_tf, 3)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        v, dt = cpu_baseline(args.cpu_sample, zk, eps, threads)
+        out["cpu_baseline"] = {"value": round(v, 4), "unit": "Mpts/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle/ OpenMP restatement, same workload at "
+                                         f"ns=nt={args.cpu_sample} ({dt:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -73,6 +73,7 @@ struct LapReq {
   int ndiv = 0, idivflag = 0, ifnear = 1;
   bool device = false;
   hipStream_t stream = nullptr;
+  double* timeinfo = nullptr;
 };
 
 template <class T>
@@ -171,12 +172,115 @@ int run_lap(const LapReq& r) {
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.st));
+  finish_timer(r.timeinfo);
   return IER_OK;
 }
 
 int guarded(const LapReq& r) {
   try {
     return run_lap(r);
+  } catch (const std::bad_alloc&) {
+    return IER_ALLOC;
+  } catch (...) {
+    return IER_HIP;
+  }
+}
+
+
+struct HelmReq {
+  int nd = 1;
+  double eps = 1e-6;
+  std::complex<double> zk = 1.0;
+  int ns = 0, nt = 0;
+  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
+  int ifc = 0, ifd = 0;
+  const void *chg = nullptr, *dip = nullptr;
+  int pg = 0, pgt = 0;
+  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
+  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
+  int ndiv = 0, idivflag = 0, ifnear = 1;
+  bool device = false;
+  hipStream_t stream = nullptr;
+  double* timeinfo = nullptr;
+};
+
+int run_helm(const HelmReq& r) {
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  if (r.stream) c.st = r.stream;
+  if (r.nd < 1 || r.ns < 1 || r.nt < 0 || r.pg < 0 || r.pg > 3 || r.pgt < 0 || r.pgt > 3)
+    return IER_ARGS;
+  Scratch S(c.st);
+  const int nd = r.nd, ns = r.ns, nt = r.nt;
+  const double *src = r.src, *trg = r.trg, *dv = r.dipvec;
+  const double2 *chg = static_cast<const double2*>(r.chg), *dip = static_cast<const double2*>(r.dip);
+  if (!r.device) {
+    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &src))) return e;
+    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &trg))) return e;
+    if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
+    if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
+    if (r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
+  }
+  SortedPoints P;
+  Plan plan;
+  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt);
+  if (e) return e;
+  HelmArgs a;
+  a.nd = nd; a.eps = r.eps; a.zk = r.zk;
+  a.ifcharge = r.ifc; a.ifdipole = r.ifd; a.ifpgh = r.pg; a.ifpghtarg = r.pgt; a.ifnear = r.ifnear;
+  double2* cs = r.ifc ? S.get<double2>((size_t)nd * ns) : nullptr;
+  double2* qs = r.ifd ? S.get<double2>((size_t)nd * ns) : nullptr;
+  double* vs = r.ifd ? S.get<double>(2 * (size_t)nd * ns) : nullptr;
+  if (S.err) return S.err;
+  if (cs) gather_channels(c.st, chg, cs, P.perm_s, ns, nd, 1);
+  if (qs) gather_channels(c.st, dip, qs, P.perm_s, ns, nd, 1);
+  if (vs) gather_real(c.st, dv, vs, P.perm_s, ns, nd, 2);
+  a.chg = cs; a.dip = qs; a.dipvec = vs;
+  auto zeros = [&](int flag, int need, int n, int ncomp) -> double2* {
+    if (flag < need) return nullptr;
+    size_t cnt = (size_t)nd * ncomp * std::max(n, 1);
+    double2* p = S.get<double2>(cnt);
+    if (p) (void)hipMemsetAsync(p, 0, sizeof(double2) * cnt, c.st);
+    return p;
+  };
+  a.pot = zeros(r.pg, 1, ns, 1); a.grad = zeros(r.pg, 2, ns, 2); a.hess = zeros(r.pg, 3, ns, 3);
+  a.pott = zeros(r.pgt, 1, nt, 1); a.gradt = zeros(r.pgt, 2, nt, 2); a.hesst = zeros(r.pgt, 3, nt, 3);
+  if (S.err) return S.err;
+  e = helm_run(c, S, P, plan, a);
+  if (e) return e;
+  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh; };
+  Out outs[2] = {{r.pg, ns, P.perm_s, a.pot, a.grad, a.hess, r.pot, r.grad, r.hess},
+                 {r.pgt, nt, P.perm_t, a.pott, a.gradt, a.hesst, r.pott, r.gradt, r.hesst}};
+  for (auto& o : outs) {
+    if (o.flag <= 0 || o.n == 0) continue;
+    size_t np = (size_t)nd * o.n;
+    double2 *dp = static_cast<double2*>(o.up), *dg = static_cast<double2*>(o.ug),
+            *dh = static_cast<double2*>(o.uh);
+    if (!r.device) {
+      dp = S.get<double2>(np);
+      dg = o.flag >= 2 ? S.get<double2>(2 * np) : nullptr;
+      dh = o.flag >= 3 ? S.get<double2>(3 * np) : nullptr;
+      if (S.err) return S.err;
+    }
+    scatter_channels(c.st, o.sp, dp, o.perm, o.n, nd, 1);
+    if (o.flag >= 2) scatter_channels(c.st, o.sg, dg, o.perm, o.n, nd, 2);
+    if (o.flag >= 3) scatter_channels(c.st, o.sh, dh, o.perm, o.n, nd, 3);
+    if (!r.device) {
+      HIP_TRY(hipMemcpyAsync(o.up, dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, sizeof(double2) * 2 * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c.st));
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c.st));
+  finish_timer(r.timeinfo);
+  return IER_OK;
+}
+
+int guarded_h(const HelmReq& r) {
+  try {
+    return run_helm(r);
   } catch (const std::bad_alloc&) {
     return IER_ALLOC;
   } catch (...) {
@@ -211,7 +315,7 @@ void cfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns,
   r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr;
   r.pg = *ifpgh; r.pgt = *ifpghtarg;
   r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
-  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear; r.timeinfo = timeinfo;
   *ier = fmm::guarded(r);
 }
 
@@ -244,7 +348,7 @@ void lfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns,
   r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
   r.pg = *ifpgh; r.pgt = *ifpghtarg;
   r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
-  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear; r.timeinfo = timeinfo;
   *ier = fmm::guarded(r);
 }
 
@@ -259,6 +363,65 @@ void lfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns, const 
   lfmm2d_ndiv_(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec, iper, ifpgh, pot,
                grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg, &ndiv, &idiv,
                &ifnear, ti, ier);
+}
+
+
+void hfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_cplx* zk, const fmm2d_int* ns,
+                  const double* sources, const fmm2d_int* ifcharge, const fmm2d_cplx* charge,
+                  const fmm2d_int* ifdipole, const fmm2d_cplx* dipstr, const double* dipvec,
+                  fmm2d_int* iper, const fmm2d_int* ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+                  fmm2d_cplx* hess, const fmm2d_int* nt, const double* targ,
+                  const fmm2d_int* ifpghtarg, fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg,
+                  fmm2d_cplx* hesstarg, const fmm2d_int* ndiv, const fmm2d_int* idivflag,
+                  const fmm2d_int* ifnear, double* timeinfo, fmm2d_int* ier) {
+  *iper = 0;
+  fmm::zero8(timeinfo);
+  fmm::HelmReq r;
+  r.nd = *nd; r.eps = *eps; r.zk = *zk; r.ns = *ns; r.nt = *nt; r.src = sources; r.trg = targ;
+  r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
+  r.pg = *ifpgh; r.pgt = *ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear; r.timeinfo = timeinfo;
+  *ier = fmm::guarded_h(r);
+}
+
+void hfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_cplx* zk, const fmm2d_int* ns,
+             const double* sources, const fmm2d_int* ifcharge, const fmm2d_cplx* charge,
+             const fmm2d_int* ifdipole, const fmm2d_cplx* dipstr, const double* dipvec,
+             fmm2d_int* iper, const fmm2d_int* ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+             fmm2d_cplx* hess, const fmm2d_int* nt, const double* targ, const fmm2d_int* ifpghtarg,
+             fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_int* ier) {
+  fmm2d_int ndiv = fmm::ndiv_rule(*eps, *ns, *nt), idiv = 0, ifnear = 1;
+  double ti[8];
+  hfmm2d_ndiv_(nd, eps, zk, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec, iper, ifpgh,
+               pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg, &ndiv, &idiv,
+               &ifnear, ti, ier);
+}
+
+void h2dterms_(const double* bsize, const fmm2d_cplx* zk, const double* eps, fmm2d_int* nterms,
+               fmm2d_int* ier) {
+  int e = 0;
+  *nterms = fmm::h2dterms_host(*bsize, *zk, *eps, &e);
+  *ier = e;
+}
+
+void fmm2dhip_hfmm2d_dev(int nd, double eps, double zk_re, double zk_im, int ns,
+                         const double* sources, int ifcharge, const void* charge, int ifdipole,
+                         const void* dipstr, const double* dipvec, int ifpgh, void* pot,
+                         void* grad, void* hess, int nt, const double* targ, int ifpghtarg,
+                         void* pottarg, void* gradtarg, void* hesstarg, int ndiv, int idivflag,
+                         int ifnear, double* timeinfo, void* stream, int* ier) {
+  fmm::zero8(timeinfo);
+  fmm::HelmReq r;
+  r.device = true;
+  r.stream = static_cast<hipStream_t>(stream);
+  r.nd = nd; r.eps = eps; r.zk = std::complex<double>(zk_re, zk_im); r.ns = ns; r.nt = nt;
+  r.src = sources; r.trg = targ;
+  r.ifc = ifcharge; r.ifd = ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
+  r.pg = ifpgh; r.pgt = ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear; r.timeinfo = timeinfo;
+  *ier = fmm::guarded_h(r);
 }
 
 void hndiv2d_(const double* eps, const fmm2d_int* ns, const fmm2d_int* nt, const fmm2d_int*,
@@ -293,7 +456,7 @@ void fmm2dhip_cfmm2d_dev(int nd, double eps, int ns, const double* sources, int 
   r.ifc = ifcharge; r.ifd = ifdipole; r.chg = charge; r.dip = dipstr;
   r.pg = ifpgh; r.pgt = ifpghtarg;
   r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
-  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear;
+  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear; r.timeinfo = timeinfo;
   *ier = fmm::guarded(r);
 }
 
@@ -312,7 +475,7 @@ void fmm2dhip_lfmm2d_dev(int nd, double eps, int ns, const double* sources, int 
   r.ifc = ifcharge; r.ifd = ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
   r.pg = ifpgh; r.pgt = ifpghtarg;
   r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
-  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear;
+  r.ndiv = ndiv; r.idivflag = idivflag; r.ifnear = ifnear; r.timeinfo = timeinfo;
   *ier = fmm::guarded(r);
 }
 
@@ -394,6 +557,15 @@ int fmm2dhip_test_tree(const double* src, int ns, const double* targ, int nt, in
   } catch (...) {
     return fmm::IER_ALLOC;
   }
+}
+
+int fmm2dhip_last_stats(double* out, int n) {
+  const fmm::SolveStats& s = fmm::last_stats();
+  double v[15] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
+                  s.nlevels, s.pmax};
+  for (int k = 0; k < 8; ++k) v[7 + k] = s.step_s[k];
+  for (int k = 0; k < n && k < 15; ++k) out[k] = v[k];
+  return 15;
 }
 
 int fmm2dhip_device_info(char* name, int len) {

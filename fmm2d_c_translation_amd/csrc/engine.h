@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <cstdint>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct DTree {
   const int* level = nullptr;
   const double* bs = nullptr;    // box size per level
   const int* nchild = nullptr;
+  const int* parent = nullptr;
   const int4* child = nullptr;
   const int2* srng = nullptr;  // sorted source range [x, y)
   const int2* trng = nullptr;  // sorted target range
@@ -70,9 +72,36 @@ struct Plan {
   std::vector<Span> m2m;          // per level: boxes with children
   std::vector<Span> m2l;          // per level >= 2: boxes with targets and list2
   std::vector<Span> l2l;          // per level: boxes with targets and children
+  std::vector<Span> l2l_kids;     // per level: the children of those boxes
   Span list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
   Span leafsort_s, leafsort_t;    // leaves holding sources / targets
 };
+
+// Boundaries of the eight FMM steps, recorded on the solve's stream:
+// ev[k] .. ev[k+1] bracket step order[k]; indices follow the reference's
+// timeinfo (0 form mp, 1 form lo, 2 M2M, 3 M2L, 4 L2L, 5 list 3, 6 local
+// evaluation, 7 near field).
+struct StepTimer {
+  hipEvent_t ev[9] = {};
+  int order[8] = {0, 2, 1, 3, 4, 5, 6, 7};
+  int n = 0;
+  bool ok = false;
+  void mark(hipStream_t st) {
+    if (ok && n < 9) (void)hipEventRecord(ev[n++], st);
+  }
+};
+StepTimer* step_timer();       // per thread, reset
+StepTimer* step_timer_peek();  // per thread, as left by the last solve
+void finish_timer(double* timeinfo);  // after the stream is synchronised
+
+// Work counts of the last solve on this thread (for roofline reporting).
+struct SolveStats {
+  double p2p_pairs_src = 0, p2p_pairs_trg = 0;  // sum over targets of list-1 sources
+  double m2l_pairs = 0, m2l_flops = 0;          // list-2 pairs, 8 * ncoef^2 per pair
+  double nboxes = 0, nlevels = 0, pmax = 0;
+  double step_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+SolveStats& last_stats();
 
 struct Ctx {
   hipStream_t st = nullptr;
@@ -105,6 +134,21 @@ struct LapArgs {
   double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
 };
 int lap_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const LapArgs& a);
+
+// Helmholtz FMM on sorted, device-resident data (hfmm2dmain semantics, low
+// frequency).  Sorted arrays are [component][channel][point].
+struct HelmArgs {
+  int nd = 1;
+  double eps = 1e-6;
+  std::complex<double> zk = 1.0;
+  int ifcharge = 0, ifdipole = 0, ifpgh = 0, ifpghtarg = 0, ifnear = 1;
+  const double2 *chg = nullptr, *dip = nullptr;
+  const double* dipvec = nullptr;                            // [2][nd][ns]
+  double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources
+  double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
+};
+int helm_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const HelmArgs& a);
+int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier);
 
 int l2dterms(double eps);
 int ndiv_rule(double eps, int ns, int nt);

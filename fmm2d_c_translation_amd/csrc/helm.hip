@@ -1,0 +1,751 @@
+// helm.hip -- Helmholtz FMM kernels (FP64) and their launch sequence,
+// low-frequency path (boxes of at most 16 wavelengths).
+//
+// Conventions are those of c_translation/src/helmrouts2d.c: expansions of
+// order p over n = -p..p, multipoles scaled as M_n / rs^|n| and locals as
+// L_n rs^|n| with rs = min(|zk bs / 2 pi|, 1) per level (hfmm2d.c:1768-1776);
+// a field is evaluated as (i/4) sum_n e_n v_n(k r) e^{i n theta} with
+// v = scaled H (multipole) or scaled J (local).  Expansions live box-major:
+// [box][channel][n + p], box stride nd*(2*pmax+1).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "engine.h"
+#include "special.h"
+#include "translate.h"
+
+namespace fmm {
+
+namespace {
+constexpr int WAVE = 64;
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+__device__ inline void addc(double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); }
+__device__ inline cd ld(const double2* p) { return cd{p->x, p->y}; }
+
+struct HLev {   // per-level parameters on the device
+  int p;
+  double rs;
+};
+
+// ---------------------------------------------------------------------
+// formation: multipoles of leaves (scaled J) and locals from list 4
+// (scaled H); charge part helmrouts2d.c:43-64, dipole part :67-105
+// ---------------------------------------------------------------------
+template <bool MP>
+__global__ void __launch_bounds__(WAVE)
+k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, const double2* chg,
+        const double2* dip, const double* dv, int N, int nd, int CH, size_t per, double2* out) {
+  extern __shared__ double2 sm[];
+  const int b = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int L = D.level[b];
+  const int p = lev[L].p;
+  const double rs = lev[L].rs;
+  const int Q = p + 2;                   // v_0 .. v_{p+1} (J) / .. v_{p+2} (H)
+  cd* V = reinterpret_cast<cd*>(sm);     // [Q+1][CH]
+  cd* Z = V + (size_t)(Q + 1) * CH;      // [Q+1][CH] powers of zmul
+  const double2 ctr = D.ctr[b];
+  const double rsd = MP ? rs : 1.0 / rs;  // dipole scale (locals use 1/rs)
+  const int nsb = MP ? 1 : D.o4[b + 1] - D.o4[b];
+  for (int sb = 0; sb < nsb; ++sb) {
+    int jb = MP ? b : D.l4[D.o4[b] + sb];
+    int2 rg = D.srng[jb];
+    for (int c0 = rg.x; c0 < rg.y; c0 += CH) {
+      const int cnt = min(CH, rg.y - c0);
+      for (int j = lane; j < cnt; j += WAVE) {
+        double2 s = pts[c0 + j];
+        double dx = s.x - ctr.x, dy = s.y - ctr.y;
+        double r = sqrt(dx * dx + dy * dy);
+        double th = (dx == 0 && dy == 0) ? 0.0 : atan2(dy, dx);
+        cd z = zk * r;
+        if (MP) jvals(p + 1, z, rs, V + j, CH);
+        else hvals(p + 2, z, rs, V + j, CH);
+        cd zm = cd{cos(th), -sin(th)}, pw = mk(1, 0);   // e^{-i theta}
+        for (int n = 0; n <= Q; ++n) {
+          Z[(size_t)n * CH + j] = pw;
+          pw = pw * zm;
+        }
+      }
+      __syncthreads();
+      for (int d = 0; d < nd; ++d) {
+        for (int m = lane - p; m <= p; m += WAVE) {
+          const int n = m < 0 ? -m : m;
+          cd acc = mk(0, 0);
+          for (int j = 0; j < cnt; ++j) {
+            cd zn = Z[(size_t)n * CH + j];
+            cd ph = m >= 0 ? zn : ((n & 1) ? -conj(zn) : conj(zn));  // (-zinv)^n
+            if (chg) acc += ld(chg + (size_t)d * N + c0 + j) * V[(size_t)n * CH + j] * ph;
+            if (dip) {
+              cd q = ld(dip + (size_t)d * N + c0 + j);
+              double v1 = dv[(size_t)d * N + c0 + j], v2 = dv[(size_t)(nd + d) * N + c0 + j];
+              cd zm = Z[(size_t)CH + j], zi = conj(zm);
+              if (m == 0) {
+                acc += -(q * V[(size_t)CH + j] * zk * 0.5 * rsd) *
+                       ((zm + zi) * v1 + mul_i(zm - zi) * v2);
+              } else if (m > 0) {
+                cd t7 = (zi / rsd) * mk(-v1, v2), t8 = (zm * rsd) * mk(v1, v2);
+                acc += (V[(size_t)(n - 1) * CH + j] * t7 + V[(size_t)(n + 1) * CH + j] * t8) *
+                       (-(zk * 0.5)) * ph * q;
+              } else {
+                cd t9 = (zm / rsd) * mk(-v1, -v2), t10 = (zi * rsd) * mk(v1, -v2);
+                acc += (V[(size_t)(n - 1) * CH + j] * t9 + V[(size_t)(n + 1) * CH + j] * t10) *
+                       (-(zk * 0.5)) * ph * q;
+              }
+            }
+          }
+          addc(out + (size_t)b * per + (size_t)d * (2 * p + 1) + (m + p), acc);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// evaluation of an expansion and its derivatives at a point.
+// Derivative expansions (helmrouts2d.c:187-287) are formed coefficient by
+// coefficient: with z1 = zk/(2 rs'), z2 = zk rs'/2,
+//   d/dx: e'_k = z2 e_{k+1} - z1 e_{k-1} (k>0), z2 (e_1 - e_-1) (k=0),
+//               z1 e_{k+1} - z2 e_{k-1} (k<0);
+//   d/dy: e'_k = i (z2 e_{k+1} + z1 e_{k-1}), i z2 (e_1 + e_-1), i (z1 e_{k+1} + z2 e_{k-1}).
+// ---------------------------------------------------------------------
+struct Ex {
+  const double2* e;
+  int p;
+  cd z1, z2;
+  __device__ cd c(int k) const { return (k < -p || k > p) ? mk(0, 0) : ld(e + k + p); }
+  __device__ cd dx(int k) const {  // valid for |k| <= p+1
+    if (k > 0) return z2 * c(k + 1) - z1 * c(k - 1);
+    if (k < 0) return z1 * c(k + 1) - z2 * c(k - 1);
+    return z2 * (c(1) - c(-1));
+  }
+  __device__ cd dy(int k) const {
+    if (k > 0) return mul_i(z2 * c(k + 1) + z1 * c(k - 1));
+    if (k < 0) return mul_i(z1 * c(k + 1) + z2 * c(k - 1));
+    return mul_i(z2 * (c(1) + c(-1)));
+  }
+  template <bool Y>
+  __device__ cd d1(int k) const { return Y ? dy(k) : dx(k); }
+  // second derivatives: apply the x or y rule to the first-derivative expansion
+  template <bool Y1, bool Y2>
+  __device__ cd d2(int k) const {
+    if (k > 0) {
+      cd a = d1<Y1>(k + 1), b = d1<Y1>(k - 1);
+      return Y2 ? mul_i(z2 * a + z1 * b) : z2 * a - z1 * b;
+    }
+    if (k < 0) {
+      cd a = d1<Y1>(k + 1), b = d1<Y1>(k - 1);
+      return Y2 ? mul_i(z1 * a + z2 * b) : z1 * a - z2 * b;
+    }
+    cd a = d1<Y1>(1), b = d1<Y1>(-1);
+    return Y2 ? mul_i(z2 * (a + b)) : z2 * (a - b);
+  }
+};
+
+struct EvalAcc {
+  cd P, Gx, Gy, Hxx, Hxy, Hyy;
+};
+struct NearAcc {
+  cd Pc, Pd, Gx, Gy, Hxx, Hxy, Hyy;  // Pc carries i/4, the rest i k/4
+};
+
+// contribution of order n (with value v = scaled H or J at the point,
+// pos = e^{i n theta}, neg = (-e^{-i theta})^n)
+template <int PG>
+__device__ inline void eval_term(const Ex& E, int n, cd v, cd pos, cd neg, EvalAcc& A) {
+  if (n == 0) {
+    A.P += v * E.c(0);
+    if (PG >= 2) { A.Gx += v * E.dx(0); A.Gy += v * E.dy(0); }
+    if (PG >= 3) {
+      A.Hxx += v * E.d2<false, false>(0);
+      A.Hxy += v * E.d2<false, true>(0);
+      A.Hyy += v * E.d2<true, true>(0);
+    }
+    return;
+  }
+  const int p = E.p;
+  if (n <= p) A.P += v * (E.c(n) * pos + E.c(-n) * neg);
+  if (PG >= 2 && n <= p + 1) {
+    A.Gx += v * (E.dx(n) * pos + E.dx(-n) * neg);
+    A.Gy += v * (E.dy(n) * pos + E.dy(-n) * neg);
+  }
+  if (PG >= 3 && n <= p + 2) {
+    A.Hxx += v * (E.d2<false, false>(n) * pos + E.d2<false, false>(-n) * neg);
+    A.Hxy += v * (E.d2<false, true>(n) * pos + E.d2<false, true>(-n) * neg);
+    A.Hyy += v * (E.d2<true, true>(n) * pos + E.d2<true, true>(-n) * neg);
+  }
+}
+
+// multipole at a point: v_n = H_n rs^n by upward recurrence, n = 0..p+2
+template <int PG>
+__device__ void eval_mp_at(const Ex& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+  cd z = zk * r;
+  if (cabs_(z) < 1e-200) return;
+  cd h0, h1;
+  hank01(z, h0, h1);
+  cd e1 = mk(cos(th), sin(th)), pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  cd zi = inv(z) * rs;
+  double s2 = rs * rs;
+  cd vm = h0, v = h1 * rs;
+  eval_term<PG>(E, 0, h0, pos, neg, A);
+  const int top = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
+  for (int n = 1; n <= top; ++n) {
+    pos = pos * e1;
+    neg = neg * mi;
+    eval_term<PG>(E, n, v, pos, neg, A);
+    cd vn = zi * v * (2.0 * n) - vm * s2;
+    vm = v;
+    v = vn;
+  }
+}
+
+// local at a point: v_n = J_n / rs^n by Miller's downward recurrence with
+// the generating-function normalisation applied at the end
+template <int PG>
+__device__ void eval_lo_at(const Ex& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+  cd z = zk * r;
+  const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
+  if (cabs_(z) < 1e-200) {  // J_0 = 1, others 0
+    eval_term<PG>(E, 0, mk(1, 0), mk(1, 0), mk(1, 0), A);
+    return;
+  }
+  int top = miller_top(top_n, z);
+  cd zr = inv(z) * rs;
+  double r2 = rs * rs;
+  cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1), ur = u * rs;
+  cd e1 = mk(cos(th), sin(th));
+  // phases at n = top_n downward
+  cd pos = mk(1, 0);
+  for (int k = 0; k < top_n; ++k) pos = pos * e1;
+  cd ie1 = conj(e1);
+  EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  cd gp = mk(0, 0), g = mk(1, 0), H = mk(0, 0);
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= top_n) {
+      cd neg = (n & 1) ? -conj(pos) : conj(pos);
+      eval_term<PG>(E, n, g, pos, neg, B);
+      pos = pos * ie1;
+    }
+    H = ur * (g + H);
+    cd gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (norm2(g) > 1e40) {
+      g = g * 1e-40; gp = gp * 1e-40; H = H * 1e-40;
+      B.P = B.P * 1e-40; B.Gx = B.Gx * 1e-40; B.Gy = B.Gy * 1e-40;
+      B.Hxx = B.Hxx * 1e-40; B.Hxy = B.Hxy * 1e-40; B.Hyy = B.Hyy * 1e-40;
+    }
+  }
+  eval_term<PG>(E, 0, g, mk(1, 0), mk(1, 0), B);
+  cd sc = inv((g + H * 2.0) * cexp_(-(u * z)));
+  A.P += B.P * sc; A.Gx += B.Gx * sc; A.Gy += B.Gy * sc;
+  A.Hxx += B.Hxx * sc; A.Hxy += B.Hxy * sc; A.Hyy += B.Hyy * sc;
+}
+
+// wave per box, lane per point; MPK: list-3 multipoles, else own local
+template <bool MPK, int PG>
+__global__ void __launch_bounds__(WAVE)
+k_heval(const int* work, DTree D, const HLev* lev, cd zk, const int2* rng, const double2* pts,
+        int N, int nd, size_t per, const double2* ex, double2* pot, double2* grad, double2* hess) {
+  const int b = work[blockIdx.x];
+  const int2 rg = rng[b];
+  const cd i4 = mk(0, 0.25);
+  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += WAVE) {
+    double2 x = pts[i];
+    const int ns = MPK ? D.o3[b + 1] - D.o3[b] : 1;
+    for (int d = 0; d < nd; ++d) {
+      EvalAcc A = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+      for (int k = 0; k < ns; ++k) {
+        int s = MPK ? D.l3[D.o3[b] + k] : b;
+        int L = D.level[s];
+        HLev lv = lev[L];
+        double2 c = D.ctr[s];
+        double ddx = x.x - c.x, ddy = x.y - c.y;
+        double r = sqrt(ddx * ddx + ddy * ddy);
+        double th = (ddx == 0 && ddy == 0) ? 0.0 : atan2(ddy, ddx);
+        double rsd = MPK ? lv.rs : 1.0 / lv.rs;
+        Ex E{ex + (size_t)s * per + (size_t)d * (2 * lv.p + 1), lv.p, zk * (0.5 / rsd), zk * (0.5 * rsd)};
+        if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
+        else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+      }
+      addc(pot + (size_t)d * N + i, A.P * i4);
+      if (PG >= 2) {
+        addc(grad + (size_t)d * N + i, A.Gx * i4);
+        addc(grad + (size_t)(nd + d) * N + i, A.Gy * i4);
+      }
+      if (PG >= 3) {
+        addc(hess + (size_t)d * N + i, A.Hxx * i4);
+        addc(hess + (size_t)(nd + d) * N + i, A.Hxy * i4);
+        addc(hess + (size_t)(2 * nd + d) * N + i, A.Hyy * i4);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// list-1 near field (helmkernels2d.c:21-520); same wave layout as the
+// Laplace kernel.  CP: charge-only potential, which skips on |z| <= thresh
+// (all other routines on <).
+// ---------------------------------------------------------------------
+constexpr int HCAP = 256;
+
+template <bool C, bool Q, int PG>
+__global__ void __launch_bounds__(WAVE)
+k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
+       const double2* dip, const double* dvv, int NS, const int2* trng, const double2* tpts,
+       int NT, int nd, double thresh, double2* pot, double2* grad, double2* hess) {
+  __shared__ double2 sx[HCAP];
+  __shared__ double2 sc[HCAP];
+  __shared__ double2 sq[HCAP];
+  __shared__ double2 sv[HCAP];
+  __shared__ double2 red[7 * WAVE];
+  __shared__ int boff[14], bsrc[14];
+  const int b = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int n1 = D.o1[b + 1] - D.o1[b];
+  if (lane == 0) {
+    int acc = 0;
+    for (int k = 0; k < n1; ++k) {
+      int2 r = D.srng[D.l1[D.o1[b] + k]];
+      boff[k] = acc;
+      bsrc[k] = r.x;
+      acc += r.y - r.x;
+    }
+    boff[n1] = acc;
+  }
+  __syncthreads();
+  const int S = boff[n1];
+  const int2 tr = trng[b];
+  const cd ik4 = zk * mk(0, 0.25);
+  const bool cp_mode = C && !Q && PG == 1;
+  for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
+    const int Tc = min(WAVE, tr.y - t0);
+    const int K = WAVE / Tc;
+    const int t = lane / K, k = lane % K;
+    const bool act = t < Tc;
+    const double2 x = act ? tpts[t0 + t] : make_double2(0, 0);
+    for (int d = 0; d < nd; ++d) {
+      NearAcc A = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+      for (int g0 = 0; g0 < S; g0 += HCAP) {
+        const int cnt = min(HCAP, S - g0);
+        for (int m = lane; m < cnt; m += WAVE) {
+          int g = g0 + m, kb = 0;
+          while (boff[kb + 1] <= g) ++kb;
+          int si = bsrc[kb] + (g - boff[kb]);
+          sx[m] = src[si];
+          if (C) sc[m] = chg[(size_t)d * NS + si];
+          if (Q) {
+            sq[m] = dip[(size_t)d * NS + si];
+            sv[m] = make_double2(dvv[(size_t)d * NS + si], dvv[(size_t)(nd + d) * NS + si]);
+          }
+        }
+        __syncthreads();
+        if (act) {
+          for (int m = k; m < cnt; m += K) {
+            double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
+            double rr = dx * dx + dy * dy, r = sqrt(rr);
+            cd z = zk * r;
+            double az = cabs_(z);
+            if (cp_mode ? az <= thresh : az < thresh) continue;
+            cd h0, h1;
+            hank01(z, h0, h1);
+            if (C) {
+              cd c = ld(&sc[m]);
+              A.Pc += h0 * c;
+              if (PG >= 2) {
+                cd cd_ = -(h1 * c * (1.0 / r));
+                A.Gx += cd_ * dx;
+                A.Gy += cd_ * dy;
+              }
+              if (PG >= 3) {
+                cd f = c * (1.0 / (r * rr)), h2z = -(z * h0) + h1 * 2.0;
+                A.Hxx += f * (h2z * (dx * dx) - h1 * rr);
+                A.Hxy += f * (h2z * (dx * dy));
+                A.Hyy += f * (h2z * (dy * dy) - h1 * rr);
+              }
+            }
+            if (Q) {
+              cd q = ld(&sq[m]);
+              double v1 = sv[m].x, v2 = sv[m].y;
+              double ct = dx / r, st = dy / r;
+              A.Pd += h1 * (1.0 / r) * q * (dx * v1 + dy * v2);
+              if (PG >= 2) {
+                cd h2 = h1 * inv(z) * 2.0 - h0;
+                cd f1 = h2 * (ct * ct - 0.5) - h0 * 0.5, f2 = h2 * (ct * st);
+                cd f3 = h2 * (st * st - 0.5) - h0 * 0.5;
+                cd qk = -(q * zk);
+                A.Gx += qk * (f1 * v1 + f2 * v2);
+                A.Gy += qk * (f2 * v1 + f3 * v2);
+                if (PG >= 3) {
+                  cd h3 = h2 * inv(z) * 4.0 - h1;
+                  cd a = (h1 * 0.75 - h3 * (0.5 * (ct * ct - 0.5 - st * st))) * ct;
+                  cd bb = (h1 * 0.25 - h3 * (0.5 * (1.5 * ct * ct - 0.5 * st * st))) * st;
+                  cd c2 = (h1 * 0.25 - h3 * (0.5 * (1.5 * st * st - 0.5 * ct * ct))) * ct;
+                  cd e = (h1 * 0.75 - h3 * (0.5 * (st * st - 0.5 - ct * ct))) * st;
+                  cd qk2 = -(q * zk * zk);
+                  A.Hxx += qk2 * (a * v1 + bb * v2);
+                  A.Hxy += qk2 * (bb * v1 + c2 * v2);
+                  A.Hyy += qk2 * (c2 * v1 + e * v2);
+                }
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // slice reduction; charge potential carries i/4, every other sum i k/4
+      cd vals[7] = {A.Pc, A.Pd, A.Gx, A.Gy, A.Hxx, A.Hxy, A.Hyy};
+      const int nv = PG >= 3 ? 7 : PG >= 2 ? 4 : 2;
+      for (int v = 0; v < nv; ++v) red[v * WAVE + lane] = make_double2(vals[v].x, vals[v].y);
+      __syncthreads();
+      if (act && k == 0) {
+        cd s[7];
+        for (int v = 0; v < nv; ++v) {
+          s[v] = mk(0, 0);
+          for (int u = 0; u < K; ++u) s[v] += ld(&red[v * WAVE + t * K + u]);
+        }
+        const size_t o = t0 + t;
+        addc(pot + (size_t)d * NT + o, s[0] * mk(0, 0.25) + s[1] * ik4);
+        if (PG >= 2) {
+          addc(grad + (size_t)d * NT + o, s[2] * ik4);
+          addc(grad + (size_t)(nd + d) * NT + o, s[3] * ik4);
+        }
+        if (PG >= 3) {
+          addc(hess + (size_t)d * NT + o, s[4] * ik4);
+          addc(hess + (size_t)(nd + d) * NT + o, s[5] * ik4);
+          addc(hess + (size_t)(2 * nd + d) * NT + o, s[6] * ik4);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// translation operators per level (helmrouts2d.c:736-1042):
+//   out_i = sum_j in_j B(i-j) F(i,j),  B(m) = v_|m| e^{-i m theta} (-1)^{m<0 ? m : 0}
+// with theta = arg(c2 - c1) - pi, v = scaled J (shifts) or H (M2L) at
+// zk |c2 - c1|, and F the scale factors of the scaled conventions:
+//   M2M  rs1^(|j|+|i-j|-|i|) (rs1/rs2)^|i|
+//   L2L  rs2^|i| rs1^(|i-j|-|j|)
+//   M2L  rs1^(|j|-|i-j|) rs2^|i|
+// One block per (level, slot); T stored [j][i].
+// ---------------------------------------------------------------------
+struct OpJob {
+  int kind;          // 0 M2M, 1 L2L, 2 M2L
+  int p1, p2;
+  double rs1, rs2;
+  double dx, dy;     // c2 - c1
+  size_t off;        // into the table buffer (double2 units)
+};
+
+__global__ void __launch_bounds__(256) k_hops(const OpJob* jobs, cd zk, double2* T) {
+  extern __shared__ double2 sv[];
+  const OpJob J = jobs[blockIdx.x];
+  const int P = J.p1 + J.p2;
+  cd* v = reinterpret_cast<cd*>(sv);  // P + 4 entries
+  double r = sqrt(J.dx * J.dx + J.dy * J.dy);
+  double th = atan2(J.dy, J.dx) - kPi;
+  if (threadIdx.x == 0) {
+    cd z = zk * r;
+    if (J.kind == 2) hvals(P + 1, z, J.rs1, v, 1);
+    else jvals(P + 3, z, J.rs1, v, 1);
+  }
+  __syncthreads();
+  const int nin = 2 * J.p1 + 1, nout = 2 * J.p2 + 1;
+  const double l1 = log(J.rs1), l2 = log(J.rs2);
+  for (int e = threadIdx.x; e < nin * nout; e += blockDim.x) {
+    int jj = e / nout, ii = e - jj * nout;
+    int j = jj - J.p1, i = ii - J.p2;
+    int m = i - j, am = abs(m), ai = abs(i), aj = abs(j);
+    double lf;
+    if (J.kind == 0) lf = (aj + am - ai) * l1 + ai * (l1 - l2);
+    else if (J.kind == 1) lf = ai * l2 + (am - aj) * l1;
+    else lf = (aj - am) * l1 + ai * l2;
+    double ang = -m * th;
+    cd ph = mk(cos(ang), sin(ang));
+    if (m < 0 && (am & 1)) ph = -ph;
+    cd va = v[am];
+    double mag = cabs_(va);
+    cd t = mag > 0 ? (va * ph) * (exp(lf + log(mag)) / mag) : mk(0, 0);
+    T[J.off + e] = make_double2(t.x, t.y);
+  }
+}
+
+template <bool C, bool Q>
+void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& D, cd zk,
+                 const double2* src, const double2* chg, const double2* dip, const double* dv,
+                 int NS, const int2* trng, const double2* tpts, int NT, int nd, double th,
+                 double2* pot, double2* grad, double2* hess) {
+  if (pg == 1) k_hp2p<C, Q, 1><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
+  if (pg == 2) k_hp2p<C, Q, 2><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
+  if (pg == 3) k_hp2p<C, Q, 3><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
+}
+
+template <bool MPK>
+void launch_heval(int pg, int nbx, hipStream_t st, const int* work, const DTree& D,
+                  const HLev* lev, cd zk, const int2* rng, const double2* pts, int N, int nd,
+                  size_t per, const double2* ex, double2* pot, double2* grad, double2* hess) {
+  if (pg == 1) k_heval<MPK, 1><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
+  if (pg == 2) k_heval<MPK, 2><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
+  if (pg == 3) k_heval<MPK, 3><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------
+namespace {
+
+// expansion order per box size (h2dterms.c:22-78), memoised: the rule needs
+// 50000-term recurrences and depends only on (zk, box size, eps)
+int h2dterms_cached(double bs, std::complex<double> zk, double eps, int* ier) {
+  static std::mutex mu;
+  static std::map<std::tuple<double, double, double, double>, std::pair<int, int>> memo;
+  auto key = std::make_tuple(bs, zk.real(), zk.imag(), eps);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = memo.find(key);
+    if (it != memo.end()) {
+      *ier = it->second.second;
+      return it->second.first;
+    }
+  }
+  const int ntmax = 50000;
+  double rs = 1.0;
+  cd zb = cd{zk.real() * bs, zk.imag() * bs};
+  if (cabs_(zb) < 2 * kPi) rs = cabs_(zb);
+  std::vector<cd> hf(ntmax + 2), jf(ntmax + 2);
+  hvals(ntmax, zb * 1.5, rs, hf.data(), 1);
+  jvals(ntmax, zb * (std::sqrt(2.0) / 2.0), rs, jf.data(), 1);
+  auto ab = [](cd a) { return cabs_(a); };
+  double x0 = ab(jf[0] * hf[0]) + ab(jf[1] * hf[1]);
+  int nt = 10001, e = 13;
+  for (int j = 2; j <= ntmax; ++j) {
+    double x = (ab(jf[j] * hf[j]) + ab(jf[j - 1] * hf[j - 1])) * ab(hf[0]);
+    if (x < eps * x0) {
+      nt = j + 1;
+      e = 0;
+      break;
+    }
+  }
+  std::lock_guard<std::mutex> g(mu);
+  memo[key] = {nt, e};
+  *ier = e;
+  return nt;
+}
+
+}  // namespace
+
+int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier) {
+  return h2dterms_cached(bs, zk, eps, ier);
+}
+
+int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const HelmArgs& a) {
+  const Tree& T = plan.T;
+  const DTree& D = plan.D;
+  const int nd = a.nd, L = T.nlevels, nb = T.nboxes;
+  const cd zk = cd{a.zk.real(), a.zk.imag()};
+  const double zi = a.zk.imag();
+  // per-level order and scale (hfmm2d.c:1768-1779)
+  std::vector<HLev> lev(L + 1);
+  std::vector<char> gate(L + 2, 0);
+  int pmax = 0;
+  for (int l = 0; l <= L; ++l) {
+    std::complex<double> zt = a.zk * T.bs[l] / (2.0 * kPi);
+    lev[l].rs = std::min(std::abs(zt), 1.0);
+    int e = 0;
+    lev[l].p = h2dterms_cached(T.bs[l], a.zk, a.eps, &e);
+    gate[l] = zi * T.bs[l] < 40.0;
+    // levels 0 and 1 take part in no M2L, list 3 or list 4 interaction, so
+    // their expansions stay zero and are neither formed nor stored
+    if (gate[l] && l >= 2) pmax = std::max(pmax, lev[l].p);
+  }
+  if (pmax > 2000) return IER_P_TOO_LARGE;
+  // the diagonal (high-frequency) form of boxes above 16 wavelengths is not built
+  double dlam = 1.0 / (a.zk.real() / (2 * kPi));
+  for (int l = 1; l <= L; ++l)
+    if (T.bs[l] / dlam > 16.0 && gate[l]) return IER_HIGHFREQ;
+  const double2* chg = a.ifcharge ? a.chg : nullptr;
+  const double2* dip = a.ifdipole ? a.dip : nullptr;
+  if (!chg && !dip) return IER_OK;
+  const size_t per = (size_t)nd * (2 * pmax + 1);
+  HLev* dlev = S.get<HLev>(L + 1);
+  double2* mp = S.get<double2>(per * nb);
+  double2* lo = S.get<double2>(per * nb);
+  if (S.err) return S.err;
+  HIP_TRY(hipMemcpyAsync(dlev, lev.data(), sizeof(HLev) * (L + 1), hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemsetAsync(mp, 0, sizeof(double2) * per * nb, c.st));
+  HIP_TRY(hipMemsetAsync(lo, 0, sizeof(double2) * per * nb, c.st));
+
+  // ---- operator tables
+  std::vector<OpJob> jobs;
+  size_t toff = 0;
+  struct Tab { size_t m2m[4], l2l[4], m2l[49]; };
+  std::vector<Tab> tab(L + 1);
+  int vmax = 0;
+  for (int l = 0; l <= L; ++l) {
+    if (!gate[l]) continue;
+    if (l >= 2 && l <= L - 1) {
+      double h = T.bs[l + 1] / 2;
+      size_t sz1 = (size_t)(2 * lev[l + 1].p + 1) * (2 * lev[l].p + 1);
+      size_t base_m = toff, base_l = toff + 4 * sz1;
+      toff += 8 * sz1;
+      for (int q = 0; q < 4; ++q) {
+        double sx = (q & 1) ? h : -h, sy = (q & 2) ? h : -h;
+        // M2M child (l+1) -> parent (l): c2 - c1 = parent - child
+        jobs.push_back({0, lev[l + 1].p, lev[l].p, lev[l + 1].rs, lev[l].rs, -sx, -sy, base_m + q * sz1});
+        // L2L parent (l) -> child (l+1): same matrix size, transposed roles
+        jobs.push_back({1, lev[l].p, lev[l + 1].p, lev[l].rs, lev[l + 1].rs, sx, sy, base_l + q * sz1});
+      }
+      tab[l].m2m[0] = base_m;
+      tab[l].l2l[0] = base_l;
+      vmax = std::max(vmax, lev[l].p + lev[l + 1].p + 4);
+    }
+    if (l >= 2) {
+      size_t sz = (size_t)(2 * lev[l].p + 1) * (2 * lev[l].p + 1);
+      size_t base = toff;
+      toff += 49 * sz;  // near slots stay unused (no list-2 pair maps to them)
+      tab[l].m2l[0] = base;
+      for (int s = 0; s < 49; ++s) {
+        int dx = s % 7 - 3, dy = s / 7 - 3;
+        if (std::abs(dx) <= 1 && std::abs(dy) <= 1) continue;
+        // source at (dx, dy) bs from the target: c2 - c1 = target - source
+        jobs.push_back({2, lev[l].p, lev[l].p, lev[l].rs, lev[l].rs, -dx * T.bs[l], -dy * T.bs[l], base + s * sz});
+      }
+      vmax = std::max(vmax, 2 * lev[l].p + 4);
+    }
+  }
+  double2* dT = S.get<double2>(std::max<size_t>(toff, 1));
+  OpJob* djobs = S.get<OpJob>(std::max<size_t>(jobs.size(), 1));
+  if (S.err) return S.err;
+  if (!jobs.empty()) {
+    HIP_TRY(hipMemcpyAsync(djobs, jobs.data(), sizeof(OpJob) * jobs.size(), hipMemcpyHostToDevice, c.st));
+    k_hops<<<(int)jobs.size(), 256, sizeof(double2) * (vmax + 8), c.st>>>(djobs, zk, dT);
+  }
+  // M2M/L2L slot tables are the 4 quadrant matrices in a row; M2L slots are
+  // addressed through a per-level array of 49 offsets
+  const int* W = plan.dwork;
+  const double* dv = a.dipvec;
+  int CH = 64;
+  auto shm_of = [&](int p) { return (size_t)2 * (p + 3) * CH * sizeof(double2); };
+  while (CH > 8 && shm_of(pmax) > 48 * 1024) CH /= 2;
+  size_t shm = shm_of(pmax);
+
+  // host-side filtering when some level is gated off (complex zk only)
+  bool all_on = true;
+  for (int l = 0; l <= L; ++l) all_on = all_on && gate[l];
+  std::vector<int> extra;
+  auto filtered = [&](const Plan::Span& sp, auto pred) -> Plan::Span {
+    if (all_on) return sp;
+    Plan::Span o;
+    o.off = -1 - (int)extra.size();  // marks the extra array
+    for (int i = sp.off; i < sp.off + sp.n; ++i)
+      if (pred(plan.work[i])) extra.push_back(plan.work[i]);
+    o.n = (int)extra.size() - (-1 - o.off);
+    return o;
+  };
+  Plan::Span s_form = filtered(plan.formmp, [&](int b) { return (bool)gate[T.level[b]]; });
+  Plan::Span s_l4 = filtered(plan.list4, [&](int b) { return (bool)gate[T.level[b]]; });
+  Plan::Span s_l3s = filtered(plan.list3_s, [&](int b) { return (bool)gate[T.level[b] + 1]; });
+  Plan::Span s_l3t = filtered(plan.list3_t, [&](int b) { return (bool)gate[T.level[b] + 1]; });
+  Plan::Span s_lvs = filtered(plan.leaves_s, [&](int b) { return (bool)gate[T.level[b]]; });
+  Plan::Span s_lvt = filtered(plan.leaves_t, [&](int b) { return (bool)gate[T.level[b]]; });
+  int* dextra = nullptr;
+  if (!extra.empty()) {
+    dextra = S.get<int>(extra.size());
+    if (S.err) return S.err;
+    HIP_TRY(hipMemcpyAsync(dextra, extra.data(), sizeof(int) * extra.size(), hipMemcpyHostToDevice, c.st));
+  }
+  auto wptr = [&](const Plan::Span& sp) { return sp.off >= 0 ? W + sp.off : dextra + (-1 - sp.off); };
+
+  StepTimer* tm = step_timer();
+  {
+    SolveStats& st = last_stats();
+    for (int l = 2; l <= L; ++l) {
+      if (!gate[l]) continue;
+      double np = 0;
+      for (int i = plan.m2l[l].off; i < plan.m2l[l].off + plan.m2l[l].n; ++i) np += T.n2[plan.work[i]];
+      double nc = 2 * lev[l].p + 1;
+      st.m2l_pairs += np;
+      st.m2l_flops += np * 8.0 * nc * nc;
+    }
+    st.pmax = pmax;
+  }
+  tm->mark(c.st);
+  // step 1: leaf multipoles
+  if (s_form.n)
+    k_hform<true><<<s_form.n, WAVE, shm, c.st>>>(wptr(s_form), D, dlev, zk, Pt.src, chg, dip, dv,
+                                                Pt.ns, nd, CH, per, mp);
+  tm->mark(c.st);
+  // step 3: M2M
+  int maxw = 1;
+  for (int l = 0; l <= L; ++l)
+    maxw = std::max({maxw, plan.m2m[l].n, plan.m2l[l].n, plan.l2l_kids[l].n});
+  int* map = S.get<int>((size_t)maxw * 49);
+  if (S.err) return S.err;
+  for (int l = L - 1; l >= 2; --l) {
+    const Plan::Span& sp = plan.m2m[l];
+    if (!sp.n || !gate[l]) continue;
+    k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
+                             2 * lev[l].p + 1, mp, per, mp, per, nd, 0.0));
+  }
+  tm->mark(c.st);
+  // step 2: list 4
+  if (s_l4.n)
+    k_hform<false><<<s_l4.n, WAVE, shm, c.st>>>(wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv,
+                                               Pt.ns, nd, CH, per, lo);
+  tm->mark(c.st);
+  // step 4: M2L (49 slot matrices per level, contiguous)
+  for (int l = 2; l <= L; ++l) {
+    const Plan::Span& sp = plan.m2l[l];
+    if (!sp.n || !gate[l]) continue;
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, T.bs[l], map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 49, dT + tab[l].m2l[0], 2 * lev[l].p + 1,
+                             2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
+  }
+  tm->mark(c.st);
+  // step 5: L2L
+  for (int l = 2; l <= L - 1; ++l) {
+    const Plan::Span& sp = plan.l2l_kids[l];
+    if (!sp.n || !gate[l]) continue;
+    k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
+                             2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
+  }
+  // steps 6-8 at sources and targets
+  struct Set { int pg; Plan::Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
+  Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s, D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
+                 {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t, D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
+  const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
+  tm->mark(c.st);
+  for (auto& s : sets)
+    if (s.pg > 0 && s.N > 0 && s.l3.n)
+      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, s.po, s.gr, s.he);
+  tm->mark(c.st);
+  for (auto& s : sets)
+    if (s.pg > 0 && s.N > 0 && s.lv.n)
+      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, s.po, s.gr, s.he);
+  tm->mark(c.st);
+  for (auto& s : sets) {
+    if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
+    if (chg && dip)
+      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+    else if (chg)
+      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+    else
+      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+  }
+  tm->mark(c.st);
+  HIP_TRY(hipGetLastError());
+  return IER_OK;
+}
+
+}  // namespace fmm

@@ -6,6 +6,7 @@
 // with r the box size of the level (cfmm2d.c:1206-1211).  Expansions live in
 // HBM box-major: [box][channel][0..p] complex.  Sorted point data is
 // channel-major ([channel][point]).
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -14,6 +15,7 @@
 #include <mutex>
 
 #include "engine.h"
+#include "translate.h"
 
 namespace fmm {
 
@@ -208,93 +210,6 @@ __global__ void __launch_bounds__(WAVE) k_form(const int* work, DTree D, const d
         }
       }
       __syncthreads();
-    }
-  }
-}
-
-// ---------------------------------------------------------------------
-// dense translations: wave per output box, lane i = output coefficient
-// ---------------------------------------------------------------------
-// M2M: parent += sum_children T_q * child   (children with sources only)
-__global__ void __launch_bounds__(WAVE) k_m2m(const int* work, DTree D, int nd, int p,
-                                              const double2* T, double2* mp) {
-  const int b = work[blockIdx.x];
-  const int P = p + 1;
-  int4 ch = D.child[b];
-  int kids[4] = {ch.x, ch.y, ch.z, ch.w};
-  for (int d = 0; d < nd; ++d)
-    for (int i = threadIdx.x; i <= p; i += WAVE) {
-      cd acc = mk(0, 0);
-      for (int q = 0; q < 4; ++q) {
-        int k = kids[q];
-        if (k < 0) continue;
-        int2 r = D.srng[k];
-        if (r.y <= r.x) continue;
-        const double2* x = mp + ((size_t)k * nd + d) * P;
-        const double2* Tq = T + (size_t)q * P * P;
-        for (int j = 0; j <= i; ++j) {
-          double2 t = Tq[(size_t)j * P + i], v = x[j];
-          acc += mk(t.x, t.y) * mk(v.x, v.y);
-        }
-      }
-      double2* o = mp + ((size_t)b * nd + d) * P + i;
-      *o = make_double2(o->x + acc.x, o->y + acc.y);
-    }
-}
-
-// M2L: local(b) += sum_{s in list2(b)} T_off(s,b) * mp(s) (+ mp_0 log bs)
-__global__ void __launch_bounds__(WAVE) k_m2l(const int* work, DTree D, int nd, int p,
-                                              const double2* T, const double2* mp,
-                                              double2* lo) {
-  const int b = work[blockIdx.x];
-  const int P = p + 1;
-  const double2 cb = D.ctr[b];
-  const double bs = D.bs[D.level[b]];
-  const double lbs = log(bs);
-  const int k0 = D.o2[b], k1 = D.o2[b + 1];
-  for (int d = 0; d < nd; ++d)
-    for (int i = threadIdx.x; i <= p; i += WAVE) {
-      cd acc = mk(0, 0);
-      for (int k = k0; k < k1; ++k) {
-        int s = D.l2[k];
-        double2 cs = D.ctr[s];
-        int dx = (int)rint((cs.x - cb.x) / bs), dy = (int)rint((cs.y - cb.y) / bs);
-        if (dx < -3 || dx > 3 || dy < -3 || dy > 3) continue;  // not a list-2 offset
-        const double2* Ts = T + (size_t)((dy + 3) * 7 + (dx + 3)) * P * P;
-        const double2* x = mp + ((size_t)s * nd + d) * P;
-        for (int j = 0; j <= p; ++j) {
-          double2 t = Ts[(size_t)j * P + i], v = x[j];
-          acc += mk(t.x, t.y) * mk(v.x, v.y);
-        }
-        if (i == 0) acc += mk(x[0].x, x[0].y) * lbs;
-      }
-      double2* o = lo + ((size_t)b * nd + d) * P + i;
-      *o = make_double2(o->x + acc.x, o->y + acc.y);
-    }
-}
-
-// L2L: child += T_q * parent
-__global__ void __launch_bounds__(WAVE) k_l2l(const int* work, DTree D, int nd, int p,
-                                              const double2* T, double2* lo) {
-  const int b = work[blockIdx.x];
-  const int P = p + 1;
-  int4 ch = D.child[b];
-  int kids[4] = {ch.x, ch.y, ch.z, ch.w};
-  for (int q = 0; q < 4; ++q) {
-    int k = kids[q];
-    if (k < 0) continue;
-    const double2* Tq = T + (size_t)q * P * P;
-    for (int d = 0; d < nd; ++d) {
-      const double2* x = lo + ((size_t)b * nd + d) * P;
-      for (int i = threadIdx.x; i <= p; i += WAVE) {
-        cd acc = mk(0, 0);
-        for (int j = i; j <= p; ++j) {
-          double2 t = Tq[(size_t)j * P + i], v = x[j];
-          acc += mk(t.x, t.y) * mk(v.x, v.y);
-        }
-        double2* o = lo + ((size_t)k * nd + d) * P + i;
-        *o = make_double2(o->x + acc.x, o->y + acc.y);
-      }
     }
   }
 }
@@ -530,23 +445,55 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   int CH = 64;
   while (CH > 8 && (size_t)2 * P * CH * sizeof(double2) > 40960) CH /= 2;
   size_t shm = (size_t)2 * P * CH * sizeof(double2);
+  StepTimer* tm = step_timer();
+  {
+    SolveStats& st = last_stats();
+    for (int l = 2; l <= L; ++l)
+      for (int i = plan.m2l[l].off; i < plan.m2l[l].off + plan.m2l[l].n; ++i)
+        st.m2l_pairs += T.n2[plan.work[i]];
+    st.m2l_flops = st.m2l_pairs * 8.0 * P * P;
+    st.pmax = p;
+  }
+  tm->mark(c.st);
   // step 1: leaf multipoles
   if (plan.formmp.n)
     k_form<true><<<plan.formmp.n, WAVE, shm, c.st>>>(W + plan.formmp.off, D, Pt.src, chg, dip,
                                                     Pt.ns, nd, p, CH, mp);
+  tm->mark(c.st);
   // step 3: upward pass
-  for (int l = L - 1; l >= 1; --l)
-    if (plan.m2m[l].n) k_m2m<<<plan.m2m[l].n, WAVE, 0, c.st>>>(W + plan.m2m[l].off, D, nd, p, T2, mp);
+  int maxw = 1;
+  for (int l = 0; l <= L; ++l)
+    maxw = std::max({maxw, plan.m2m[l].n, plan.m2l[l].n, plan.l2l_kids[l].n});
+  int* map = S.get<int>((size_t)maxw * 49);
+  if (S.err) return S.err;
+  for (int l = L - 1; l >= 1; --l) {
+    const Plan::Span& sp = plan.m2m[l];
+    if (!sp.n) continue;
+    k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, T2, P, P, mp, per, mp, per, nd, 0.0));
+  }
+  tm->mark(c.st);
   // step 2: list 4
   if (plan.list4.n)
     k_form<false><<<plan.list4.n, WAVE, shm, c.st>>>(W + plan.list4.off, D, Pt.src, chg, dip,
                                                     Pt.ns, nd, p, CH, lo);
-  // step 4 + 5: M2L level by level, then L2L into the next level
-  for (int l = 2; l <= L; ++l)
-    if (plan.m2l[l].n)
-      k_m2l<<<plan.m2l[l].n, WAVE, 0, c.st>>>(W + plan.m2l[l].off, D, nd, p, TM, mp, lo);
-  for (int l = 1; l <= L - 1; ++l)
-    if (plan.l2l[l].n) k_l2l<<<plan.l2l[l].n, WAVE, 0, c.st>>>(W + plan.l2l[l].off, D, nd, p, TL, lo);
+  tm->mark(c.st);
+  // step 4: M2L level by level (log of the box size enters entry (0,0) only)
+  for (int l = 2; l <= L; ++l) {
+    const Plan::Span& sp = plan.m2l[l];
+    if (!sp.n) continue;
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, T.bs[l], map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 49, TM, P, P, mp, per, lo, per, nd,
+                             std::log(T.bs[l])));
+  }
+  tm->mark(c.st);
+  // step 5: L2L into the children of boxes that hold targets
+  for (int l = 1; l <= L - 1; ++l) {
+    const Plan::Span& sp = plan.l2l_kids[l];
+    if (!sp.n) continue;
+    k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
+    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, TL, P, P, lo, per, lo, per, nd, 0.0));
+  }
   // steps 6 + 7: list 3 and local expansions at sources / targets
   struct Set { int pg; Plan::Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
   Set sets[2] = {{a.ifpgh, plan.list3_s, plan.leaves_s, plan.p2p_s, D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
@@ -554,12 +501,17 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   // reference skip tests: charges (with or without dipoles) use rr vs thresh^2,
   // '<=' for potential only and '<' otherwise; dipole-only routines compare |z|
   const double thresh = Pt.bs0 * 0x1p-51;
+  tm->mark(c.st);
+  for (auto& s : sets)
+    if (s.pg > 0 && s.N > 0 && s.l3.n)
+      k_eval_mp<<<s.l3.n, WAVE, 0, c.st>>>(W + s.l3.off, D, s.rng, s.pts, s.N, nd, p, s.pg, mp, s.po, s.gr, s.he);
+  tm->mark(c.st);
+  for (auto& s : sets)
+    if (s.pg > 0 && s.N > 0 && s.lv.n)
+      k_eval_lo<<<s.lv.n, WAVE, 0, c.st>>>(W + s.lv.off, D, s.rng, s.pts, s.N, nd, p, s.pg, lo, s.po, s.gr, s.he);
+  tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0) continue;
-    if (s.l3.n)
-      k_eval_mp<<<s.l3.n, WAVE, 0, c.st>>>(W + s.l3.off, D, s.rng, s.pts, s.N, nd, p, s.pg, mp, s.po, s.gr, s.he);
-    if (s.lv.n)
-      k_eval_lo<<<s.lv.n, WAVE, 0, c.st>>>(W + s.lv.off, D, s.rng, s.pts, s.N, nd, p, s.pg, lo, s.po, s.gr, s.he);
     if (a.ifnear && s.nn.n) {
       int mode = s.pg == 1 ? 0 : 1;
       double th2 = thresh * thresh;
@@ -571,6 +523,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
         launch_p2p_pg<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he);
     }
   }
+  tm->mark(c.st);
   HIP_TRY(hipGetLastError());
   return IER_OK;
 }

@@ -10,6 +10,36 @@
 
 namespace fmm {
 
+static thread_local StepTimer g_timer;
+StepTimer* step_timer_peek() { return &g_timer; }
+StepTimer* step_timer() {
+  StepTimer& t = g_timer;
+  if (!t.ok) {
+    bool good = true;
+    for (auto& e : t.ev) good = good && hipEventCreate(&e) == hipSuccess;
+    t.ok = good;
+  }
+  t.n = 0;
+  return &t;
+}
+
+void finish_timer(double* timeinfo) {
+  StepTimer& t = *step_timer_peek();
+  SolveStats& st = last_stats();
+  for (int k = 0; k < 8; ++k) {
+    float ms = 0.f;
+    if (t.ok && t.n >= k + 2 && hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]) == hipSuccess) {
+      st.step_s[t.order[k]] = ms * 1e-3;
+      if (timeinfo) timeinfo[t.order[k]] = ms * 1e-3;
+    }
+  }
+}
+
+SolveStats& last_stats() {
+  static thread_local SolveStats s;
+  return s;
+}
+
 int get_ctx(Ctx& c) {
   static thread_local hipStream_t st = nullptr;
   static thread_local int dev = -1;
@@ -250,11 +280,18 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   plan.m2m.assign(L + 1, {});
   plan.m2l.assign(L + 1, {});
   plan.l2l.assign(L + 1, {});
+  plan.l2l_kids.assign(L + 1, {});
   for (int l = 0; l <= L; ++l) {
     int f = T.lfirst[l], n = T.lcount[l];
     plan.m2m[l] = span([&](int b) { return T.nchild[b] > 0; }, f, f + n);
     plan.m2l[l] = span([&](int b) { return l >= 2 && npts(b) > 0 && T.n2[b] > 0; }, f, f + n);
     plan.l2l[l] = span([&](int b) { return npts(b) > 0 && T.nchild[b] > 0; }, f, f + n);
+    Plan::Span k;
+    k.off = (int)W.size();
+    for (int i = plan.l2l[l].off; i < plan.l2l[l].off + plan.l2l[l].n; ++i)
+      for (int q = 0; q < 4; ++q) W.push_back(T.child[4 * W[i] + q]);
+    k.n = (int)W.size() - k.off;
+    plan.l2l_kids[l] = k;
   }
   plan.list4 = span([&](int b) { return T.level[b] >= 2 && npts(b) > 0 && T.n4[b] > 0; }, 0, nb);
   plan.list3_s = span([&](int b) { return T.n3[b] > 0 && ns_(b) > 0; }, 0, nb);
@@ -266,6 +303,20 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   plan.leafsort_s = plan.leaves_s;
   plan.leafsort_t = plan.leaves_t;
 
+  // ---- work counts of this solve
+  {
+    SolveStats& st = last_stats();
+    st = SolveStats();
+    for (int b = 0; b < nb; ++b) {
+      double srcs = 0;
+      for (int k = 0; k < std::min(T.n1[b], 13); ++k) srcs += ns_(T.l1[13 * b + k]);
+      st.p2p_pairs_src += srcs * ns_(b);
+      st.p2p_pairs_trg += srcs * nt_(b);
+    }
+    st.nboxes = nb;
+    st.nlevels = L;
+  }
+
   // ---- upload: box arrays, CSR lists, work lists
   std::vector<int> o1, v1, o2, v2, o3, v3, o4, v4;
   csr(T.n1, T.l1, 13, o1, v1);
@@ -274,7 +325,7 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   csr(T.n4, T.l4, 5, o4, v4);
   size_t ints = (size_t)nb * (1 + 4 + 2 + 2) + o1.size() + v1.size() + o2.size() + v2.size() +
                 o3.size() + v3.size() + o4.size() + v4.size() + W.size();
-  size_t bytes = sizeof(double2) * nb + sizeof(double) * (L + 1) + sizeof(int) * (ints + nb) + 64 * 16;
+  size_t bytes = sizeof(double2) * nb + sizeof(double) * (L + 1) + sizeof(int) * (ints + 2 * nb) + 64 * 16;
   char* hbuf = static_cast<char*>(pinned(bytes, 3));
   char* dbuf = S.get<char>(bytes);
   if (!hbuf || S.err) return IER_ALLOC;
@@ -299,6 +350,7 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   size_t a_bs = put(T.bs.data(), sizeof(double) * (L + 1), 16);
   size_t a_lev = put(T.level.data(), sizeof(int) * nb, 16);
   size_t a_nch = put(T.nchild.data(), sizeof(int) * nb, 16);
+  size_t a_par = put(T.parent.data(), sizeof(int) * nb, 16);
   size_t a_chi = put(child.data(), sizeof(int) * 4 * nb, 16);
   size_t a_sr = put(sr.data(), sizeof(int2) * nb, 16);
   size_t a_tr = put(tr.data(), sizeof(int2) * nb, 16);
@@ -315,6 +367,7 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   D.bs = reinterpret_cast<const double*>(dbuf + a_bs);
   D.level = reinterpret_cast<const int*>(dbuf + a_lev);
   D.nchild = reinterpret_cast<const int*>(dbuf + a_nch);
+  D.parent = reinterpret_cast<const int*>(dbuf + a_par);
   D.child = reinterpret_cast<const int4*>(dbuf + a_chi);
   D.srng = reinterpret_cast<const int2*>(dbuf + a_sr);
   D.trng = reinterpret_cast<const int2*>(dbuf + a_tr);

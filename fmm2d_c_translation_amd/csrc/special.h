@@ -1,0 +1,207 @@
+// special.h -- FP64 Hankel/Bessel functions for device code.
+//
+// H0^(1), H1^(1) of complex argument, our own evaluation by region:
+//   |z| <= 4          ascending series (Abramowitz & Stegun 9.1.10-9.1.11)
+//   4 < |z| < 20      Laplace-type integral
+//                       H_nu(z) = sqrt(2/(pi z)) e^{i(z - nu pi/2 - pi/4)} / Gamma(nu+1/2)
+//                                 * int_0^inf e^{-t} t^{nu-1/2} (1 + i t/(2z))^{nu-1/2} dt
+//                     with a 20-point Gauss-Laguerre rule (gl20.h)
+//   |z| >= 20         Hankel's asymptotic expansion (A&S 9.2.7)
+// in the upper half plane; below it H^(1)(z) = 2 J(z) - conj(H^(1)(conj z)).
+// Scaled sequences as in the reference (c_translation/src/h2dcommon.c:35-90,
+// cdjseval2d.c:11-164): H_n(z) rscale^n by upward recurrence, J_n(z)/rscale^n
+// by Miller's downward recurrence normalised through the generating
+// function (Horner form, so no power of rscale is ever formed).
+#pragma once
+
+#include "common.h"
+#include "gl20.h"
+
+namespace fmm {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEuler = 0.57721566490153286061;
+
+FMM_HD cd cexp_(cd z) {
+  double e = exp(z.x);
+  return cd{e * cos(z.y), e * sin(z.y)};
+}
+FMM_HD cd clog_(cd z) { return cd{log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
+FMM_HD cd csqrt_(cd z) {
+  double r = hypot(z.x, z.y);
+  if (r == 0.0) return cd{0, 0};
+  double a = sqrt(0.5 * (r + fabs(z.x)));
+  double b = 0.5 * z.y / a;
+  if (z.x >= 0) return cd{a, b};
+  return z.y >= 0 ? cd{fabs(b), a} : cd{fabs(b), -a};
+}
+FMM_HD double cabs_(cd z) { return hypot(z.x, z.y); }
+FMM_HD cd ci() { return cd{0, 1}; }
+FMM_HD cd mul_i(cd z) { return cd{-z.y, z.x}; }
+
+FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
+  cd w = z * z * 0.25, lz = clog_(z * 0.5);
+  cd t0 = mk(1, 0), t1 = mk(1, 0), j0 = mk(0, 0), j1 = mk(0, 0), s0 = mk(0, 0), s1 = mk(0, 0);
+  double hk = 0.0, p1 = -kEuler, p2 = 1.0 - kEuler;
+  for (int k = 0; k < 40; ++k) {
+    double sg = (k & 1) ? -1.0 : 1.0;
+    j0 += t0 * sg;
+    j1 += t1 * sg;
+    s0 += t0 * (-sg * hk);
+    s1 += t1 * (sg * (p1 + p2));
+    if (k > 2 && norm2(t0) < 1e-36 * (norm2(j0) + 1e-300)) break;
+    t0 = t0 * w / ((double)(k + 1) * (k + 1));
+    t1 = t1 * w / ((double)(k + 1) * (k + 2));
+    hk += 1.0 / (k + 1);
+    p1 += 1.0 / (k + 1);
+    p2 += 1.0 / (k + 2);
+  }
+  j1 = j1 * (z * 0.5);
+  cd y0 = ((lz + mk(kEuler, 0)) * j0 + s0) * (2.0 / kPi);
+  cd y1 = -(inv(z) * (2.0 / kPi)) + (lz * j1) * (2.0 / kPi) - (z * 0.5) * s1 * (1.0 / kPi);
+  h0 = j0 + mul_i(y0);
+  h1 = j1 + mul_i(y1);
+}
+
+FMM_HD void hank_laguerre(cd z, cd& h0, cd& h1) {
+  cd a = mul_i(inv(z * 2.0)), i0 = mk(0, 0), i1 = mk(0, 0);
+  for (int k = 0; k < GL20_N; ++k) {
+    cd s = csqrt_(mk(1, 0) + a * GL20_X[k]);
+    i0 += inv(s) * GL20_W[k];
+    i1 += s * (GL20_W[k] * GL20_X[k]);
+  }
+  cd pre = csqrt_(inv(z) * (2.0 / kPi)) * (1.0 / sqrt(kPi));
+  h0 = pre * cexp_(mul_i(z - mk(kPi / 4, 0))) * i0;
+  h1 = pre * cexp_(mul_i(z - mk(3 * kPi / 4, 0))) * i1 * 2.0;
+}
+
+FMM_HD void hank_asym(cd z, cd& h0, cd& h1) {
+  cd pre = csqrt_(inv(z) * (2.0 / kPi));
+  cd iz = mul_i(inv(z)), t0 = mk(1, 0), t1 = mk(1, 0), s0 = mk(0, 0), s1 = mk(0, 0);
+  for (int k = 0; k < 40; ++k) {
+    s0 += t0;
+    s1 += t1;
+    double m0 = (0.0 - (2.0 * k + 1) * (2.0 * k + 1)) / (8.0 * (k + 1));
+    double m1 = (4.0 - (2.0 * k + 1) * (2.0 * k + 1)) / (8.0 * (k + 1));
+    cd n0 = t0 * iz * m0, n1 = t1 * iz * m1;
+    if (k > 4 && norm2(n0) > norm2(t0)) break;
+    t0 = n0;
+    t1 = n1;
+    if (norm2(t0) < 1e-36 && norm2(t1) < 1e-36) {
+      s0 += t0;
+      s1 += t1;
+      break;
+    }
+  }
+  h0 = pre * cexp_(mul_i(z - mk(kPi / 4, 0))) * s0;
+  h1 = pre * cexp_(mul_i(z - mk(3 * kPi / 4, 0))) * s1;
+}
+
+FMM_HD void hank_upper(cd z, cd& h0, cd& h1) {
+  double a = cabs_(z);
+  if (a <= 4.0) hank_series(z, h0, h1);
+  else if (a < 20.0) hank_laguerre(z, h0, h1);
+  else hank_asym(z, h0, h1);
+}
+
+// J0, J1 by Miller's recurrence with generating-function normalisation
+FMM_HD void bessel_j01(cd z, cd& j0, cd& j1) {
+  int top = 24 + (int)(cabs_(z) * 1.5);
+  cd zi = inv(z), fp = mk(0, 0), f = mk(1e-300, 0), s = mk(0, 0);
+  cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1);  // -i above the axis, +i below
+  // phase u^n for n = top .. 1, built downward from u^top
+  cd ph = mk(1, 0);
+  for (int k = 0; k < (top & 3); ++k) ph = ph * u;
+  cd ui = conj(u);
+  cd f1 = mk(0, 0);
+  for (int n = top; n >= 1; --n) {
+    s += ph * f;
+    ph = ph * ui;
+    cd fm = zi * f * (2.0 * n) - fp;
+    fp = f;
+    f = fm;
+    if (n == 1) f1 = fp;
+    if (norm2(f) > 1e300) {
+      f = f * 1e-250; fp = fp * 1e-250; s = s * 1e-250; f1 = f1 * 1e-250;
+    }
+  }
+  cd lam = (f + s * 2.0) * cexp_(-(u * z));
+  cd li = inv(lam);
+  j0 = f * li;
+  j1 = f1 * li;
+}
+
+FMM_HD void hank01(cd z, cd& h0, cd& h1) {
+  if (z.y >= 0) {
+    hank_upper(z, h0, h1);
+    return;
+  }
+  cd g0, g1, j0, j1;
+  hank_upper(conj(z), g0, g1);
+  bessel_j01(z, j0, j1);
+  h0 = j0 * 2.0 - conj(g0);
+  h1 = j1 * 2.0 - conj(g1);
+}
+
+// Start index of the downward recurrence (cdjseval2d.c:55-71): upward from
+// nterms until |f|^2 exceeds 1e40.
+FMM_HD int miller_top(int nterms, cd z) {
+  cd zi = inv(z), fn = mk(1, 0), fm = mk(0, 0);
+  for (int i = nterms; i <= nterms + 10000; ++i) {
+    cd fp = zi * fn * (2.0 * i) - fm;
+    if (norm2(fp) > 1e40) return i + 1;
+    fm = fn;
+    fn = fp;
+  }
+  return nterms + 10000;
+}
+
+// Scaled J_n(z)/rs^n for n = 0..nmax into out[n*stride].
+FMM_HD void jvals(int nmax, cd z, double rs, cd* out, int stride) {
+  if (cabs_(z) < 1e-200) {
+    out[0] = mk(1, 0);
+    for (int n = 1; n <= nmax; ++n) out[n * stride] = mk(0, 0);
+    return;
+  }
+  int top = miller_top(nmax, z);
+  cd zr = inv(z) * rs;       // g_{n-1} = 2n (rs/z) g_n - rs^2 g_{n+1}
+  double r2 = rs * rs;
+  cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1);
+  cd ur = u * rs;
+  cd gp = mk(0, 0), g = mk(1, 0), H = mk(0, 0);  // g = g_{top-1}, gp = g_top
+  // Horner: H <- u rs (g_n + H), n = top-1 .. 1
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= nmax) out[n * stride] = g;
+    H = ur * (g + H);
+    cd gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (norm2(g) > 1e40) {
+      g = g * 1e-40; gp = gp * 1e-40; H = H * 1e-40;
+      for (int k = n; k <= nmax && k <= top - 1; ++k) out[k * stride] = out[k * stride] * 1e-40;
+    }
+  }
+  for (int n = top; n <= nmax; ++n) out[n * stride] = mk(0, 0);
+  cd lam = (g + H * 2.0) * cexp_(-(u * z));   // e^{-uz} (g_0 + 2 sum u^n rs^n g_n) = const
+  cd sc = inv(lam);
+  out[0] = g * sc;
+  for (int n = 1; n <= nmax && n <= top - 1; ++n) out[n * stride] = out[n * stride] * sc;
+}
+
+// Scaled H_n(z) rs^n, n = 0..nmax (h2dcommon.c:35-90)
+FMM_HD void hvals(int nmax, cd z, double rs, cd* out, int stride) {
+  if (cabs_(z) < 1e-200) {
+    for (int n = 0; n <= nmax; ++n) out[n * stride] = mk(0, 0);
+    return;
+  }
+  cd h0, h1;
+  hank01(z, h0, h1);
+  out[0] = h0;
+  if (nmax >= 1) out[stride] = h1 * rs;
+  cd zi = inv(z) * rs;
+  double s2 = rs * rs;
+  for (int i = 1; i <= nmax - 1; ++i)
+    out[(i + 1) * stride] = zi * out[i * stride] * (2.0 * i) - out[(i - 1) * stride] * s2;
+}
+
+}  // namespace fmm

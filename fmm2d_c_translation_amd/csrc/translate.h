@@ -1,0 +1,203 @@
+// translate.h -- dense expansion translations as small batched GEMMs.
+//
+// Every translation in the FMM (M2M, M2L, L2L, Laplace or Helmholtz) is
+//     Y[b] += sum_s  T_s  X[src(b, s)]
+// with a handful of operator "slots" s that depend only on the relative
+// position of the two boxes (4 child quadrants, 49 offsets of list 2).
+// Blocks own a tile of TB target boxes and walk the slots: the slot's
+// operator is staged once in LDS and applied to the whole tile, so each
+// operator is read from HBM/L2 once per tile instead of once per box pair.
+// Operators are stored T[s][j][i] (j = input coefficient, i = output).
+#pragma once
+
+#include "common.h"
+
+namespace fmm {
+// every translation unit gets its own copy of these kernels and launcher
+namespace {
+
+constexpr int XT_THREADS = 256;
+
+// map[w * nslot + s]: source box of work item w in slot s, or -1.
+// Y/X are [box][channel][coef] with per-box strides ystride/xstride
+// (in double2 units), channel stride nout/nin.
+// lap_log: Laplace M2L adds x_0 * log(bs) to output 0 per source (the only
+// level-dependent entry of the otherwise scale-free operator).
+template <int TB>
+__global__ void __launch_bounds__(XT_THREADS)
+k_translate(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
+            const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+            size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log) {
+  extern __shared__ double2 lds[];
+  double2* sT = lds;                           // nin * nout
+  double2* sX = lds + (size_t)nin * nout;      // TB * nin
+  __shared__ int ssrc[TB];
+  __shared__ int sany;
+  const int w0 = blockIdx.x * TB;
+  const int nb = min(TB, nwork - w0);
+  const int npair = nb * nout;
+  constexpr int MAXP = (TB * 128 + XT_THREADS - 1) / XT_THREADS;  // nout <= 128
+  for (int d = 0; d < nd; ++d) {
+    cd acc[MAXP];
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) acc[k] = mk(0, 0);
+    for (int s = 0; s < nslot; ++s) {
+      if (threadIdx.x == 0) sany = 0;
+      __syncthreads();
+      if ((int)threadIdx.x < TB) {
+        int src = (int)threadIdx.x < nb ? map[(size_t)(w0 + threadIdx.x) * nslot + s] : -1;
+        ssrc[threadIdx.x] = src;
+        if (src >= 0) atomicOr(&sany, 1);
+      }
+      __syncthreads();
+      if (!sany) continue;  // block-uniform
+      for (int e = threadIdx.x; e < nin * nout; e += XT_THREADS) sT[e] = T[(size_t)s * nin * nout + e];
+      for (int e = threadIdx.x; e < TB * nin; e += XT_THREADS) {
+        int b = e / nin, j = e - b * nin;
+        int src = ssrc[b];
+        sX[e] = src >= 0 ? X[(size_t)src * xstride + (size_t)d * nin + j] : make_double2(0, 0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k) {
+        int q = threadIdx.x + k * XT_THREADS;
+        if (q < npair) {
+          int b = q / nout, i = q - b * nout;
+          if (ssrc[b] >= 0) {
+            const double2* xb = sX + b * nin;
+            double ar = 0, ai = 0;
+            for (int j = 0; j < nin; ++j) {
+              double2 t = sT[j * nout + i], x = xb[j];
+              ar += t.x * x.x - t.y * x.y;
+              ai += t.x * x.y + t.y * x.x;
+            }
+            if (lap_log != 0.0 && i == 0) {
+              ar += xb[0].x * lap_log;
+              ai += xb[0].y * lap_log;
+            }
+            acc[k] += mk(ar, ai);
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      int q = threadIdx.x + k * XT_THREADS;
+      if (q < npair) {
+        int b = q / nout, i = q - b * nout;
+        double2* y = Y + (size_t)work[w0 + b] * ystride + (size_t)d * nout + i;
+        *y = make_double2(y->x + acc[k].x, y->y + acc[k].y);
+      }
+    }
+  }
+}
+
+// list-2 slot of a box pair: (dy+3)*7 + (dx+3) from centre offsets
+__device__ inline int m2l_slot(double2 cs, double2 ct, double bs) {
+  int dx = (int)rint((cs.x - ct.x) / bs), dy = (int)rint((cs.y - ct.y) / bs);
+  if (dx < -3 || dx > 3 || dy < -3 || dy > 3) return -1;
+  return (dy + 3) * 7 + (dx + 3);
+}
+
+// maps for the three translation kinds, one thread per work item
+// M2L: slot map from list 2
+__global__ void k_map_m2l(const int* work, int nwork, const double2* ctr, const int* o2,
+                          const int* l2, double bs, int* map) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwork) return;
+  int b = work[w];
+  int* m = map + (size_t)w * 49;
+  for (int s = 0; s < 49; ++s) m[s] = -1;
+  for (int k = o2[b]; k < o2[b + 1]; ++k) {
+    int s = m2l_slot(ctr[l2[k]], ctr[b], bs);
+    if (s >= 0) m[s] = l2[k];
+  }
+}
+// M2M: slot q = child quadrant, children with sources only
+__global__ void k_map_m2m(const int* work, int nwork, const int4* child, const int2* srng,
+                          int* map) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwork) return;
+  int4 c = child[work[w]];
+  int k[4] = {c.x, c.y, c.z, c.w};
+  for (int q = 0; q < 4; ++q) {
+    int s = k[q];
+    map[(size_t)w * 4 + q] = (s >= 0 && srng[s].y > srng[s].x) ? s : -1;
+  }
+}
+
+// L2L: work items are children; slot = the child's quadrant in its parent
+__global__ void k_map_l2l(const int* work, int nwork, const int* parent, const int4* child,
+                          int* map) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwork) return;
+  int c = work[w], p = parent[c];
+  int4 k = child[p];
+  map[(size_t)w * 4 + 0] = k.x == c ? p : -1;
+  map[(size_t)w * 4 + 1] = k.y == c ? p : -1;
+  map[(size_t)w * 4 + 2] = k.z == c ? p : -1;
+  map[(size_t)w * 4 + 3] = k.w == c ? p : -1;
+}
+
+// Operators too large for LDS (coarse Helmholtz levels): one block per
+// target, operator read from global memory.
+__global__ void __launch_bounds__(XT_THREADS)
+k_translate_direct(const int* __restrict__ work, const int* __restrict__ map, int nslot,
+                   const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+                   size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log) {
+  const int w = blockIdx.x, b = work[w];
+  for (int d = 0; d < nd; ++d)
+    for (int i = threadIdx.x; i < nout; i += XT_THREADS) {
+      double ar = 0, ai = 0;
+      for (int s = 0; s < nslot; ++s) {
+        int src = map[(size_t)w * nslot + s];
+        if (src < 0) continue;
+        const double2* x = X + (size_t)src * xstride + (size_t)d * nin;
+        const double2* Ts = T + (size_t)s * nin * nout;
+        for (int j = 0; j < nin; ++j) {
+          double2 t = Ts[(size_t)j * nout + i], v = x[j];
+          ar += t.x * v.x - t.y * v.y;
+          ai += t.x * v.y + t.y * v.x;
+        }
+        if (lap_log != 0.0 && i == 0) {
+          ar += x[0].x * lap_log;
+          ai += x[0].y * lap_log;
+        }
+      }
+      double2* y = Y + (size_t)b * ystride + (size_t)d * nout + i;
+      *y = make_double2(y->x + ar, y->y + ai);
+    }
+}
+
+constexpr int XT_TB = 16;
+constexpr size_t XT_LDS_MAX = 150 * 1024;
+
+// Y[work] += sum_s T_s X[map]; picks the LDS-tiled kernel when the operator
+// fits, the direct one otherwise.
+inline hipError_t launch_translate(hipStream_t st, const int* work, int nwork, const int* map,
+                                   int nslot, const double2* T, int nin, int nout,
+                                   const double2* X, size_t xs, double2* Y, size_t ys, int nd,
+                                   double lap_log) {
+  if (nwork <= 0) return hipSuccess;
+  size_t lds = ((size_t)nin * nout + (size_t)XT_TB * nin) * sizeof(double2);
+  if (nout <= 128 && lds <= XT_LDS_MAX) {
+    static bool init = false;
+    if (!init) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_translate<XT_TB>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)XT_LDS_MAX);
+      if (e != hipSuccess) return e;
+      init = true;
+    }
+    k_translate<XT_TB><<<(nwork + XT_TB - 1) / XT_TB, XT_THREADS, lds, st>>>(
+        work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, nd, lap_log);
+  } else {
+    k_translate_direct<<<nwork, XT_THREADS, 0, st>>>(work, map, nslot, T, nin, nout, X, xs, Y, ys,
+                                                     nd, lap_log);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace fmm

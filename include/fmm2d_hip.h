@@ -13,7 +13,8 @@
  *       101 HIP runtime error          102 unsupported argument
  *       103 tree deeper than 29 levels 104 no usable device
  *       105 Helmholtz boxes above 16 wavelengths (high-frequency diagonal
- *           form not built)            106 expansion order above 63.
+ *           form not built)            106 expansion order beyond the
+ *           kernels' limits (Laplace 63, Helmholtz 2000).
  *     The library never aborts or prints.
  * Calls are synchronous and thread-safe (one HIP stream per calling thread).
  */
@@ -75,7 +76,33 @@ void lfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
                   fmm2d_cplx *hesstarg, const fmm2d_int *ndiv, const fmm2d_int *idivflag,
                   const fmm2d_int *ifnear, double *timeinfo, fmm2d_int *ier);
 
+/* ---- Helmholtz: pot = (i/4) sum c H0(k r) + dipoles, grad (nd,2,n), hess (nd,3,n) */
+
+/* replaces hfmm2d_ (c_translation/src/hfmm2d.c:1529) */
+void hfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_cplx *zk, const fmm2d_int *ns,
+             const double *sources, const fmm2d_int *ifcharge, const fmm2d_cplx *charge,
+             const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr, const double *dipvec,
+             fmm2d_int *iper, const fmm2d_int *ifpgh, fmm2d_cplx *pot, fmm2d_cplx *grad,
+             fmm2d_cplx *hess, const fmm2d_int *nt, const double *targ,
+             const fmm2d_int *ifpghtarg, fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg,
+             fmm2d_cplx *hesstarg, fmm2d_int *ier);
+
+/* replaces hfmm2d_ndiv_ (c_translation/src/hfmm2d_ndiv.c:68) */
+void hfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_cplx *zk,
+                  const fmm2d_int *ns, const double *sources, const fmm2d_int *ifcharge,
+                  const fmm2d_cplx *charge, const fmm2d_int *ifdipole, const fmm2d_cplx *dipstr,
+                  const double *dipvec, fmm2d_int *iper, const fmm2d_int *ifpgh,
+                  fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const fmm2d_int *nt,
+                  const double *targ, const fmm2d_int *ifpghtarg, fmm2d_cplx *pottarg,
+                  fmm2d_cplx *gradtarg, fmm2d_cplx *hesstarg, const fmm2d_int *ndiv,
+                  const fmm2d_int *idivflag, const fmm2d_int *ifnear, double *timeinfo,
+                  fmm2d_int *ier);
+
 /* ---- parameter helpers */
+
+/* replaces h2dterms_ (c_translation/src/h2dterms.c:22) */
+void h2dterms_(const double *bsize, const fmm2d_cplx *zk, const double *eps, fmm2d_int *nterms,
+               fmm2d_int *ier);
 
 /* replaces hndiv2d_ (c_translation/src/hndiv2d.c:15) */
 void hndiv2d_(const double *eps, const fmm2d_int *ns, const fmm2d_int *nt,
@@ -139,6 +166,13 @@ void fmm2dhip_lfmm2d_dev(int nd, double eps, int ns, const double *sources, int 
                          void *gradtarg, void *hesstarg, int ndiv, int idivflag, int ifnear,
                          double *timeinfo, void *stream, int *ier);
 
+void fmm2dhip_hfmm2d_dev(int nd, double eps, double zk_re, double zk_im, int ns,
+                         const double *sources, int ifcharge, const void *charge, int ifdipole,
+                         const void *dipstr, const double *dipvec, int ifpgh, void *pot,
+                         void *grad, void *hess, int nt, const double *targ, int ifpghtarg,
+                         void *pottarg, void *gradtarg, void *hesstarg, int ndiv, int idivflag,
+                         int ifnear, double *timeinfo, void *stream, int *ier);
+
 /* ---- test hook (no counterpart in the reference): builds the box-level
  * tree and lists with keys computed and sorted on the CPU, and returns them
  * in the reference itree layout (pts_tree2d.c:484-491, 1-based ids), plus
@@ -151,6 +185,13 @@ int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, in
                        int ndiv, int *nlevels, int *nboxes, int *itree, double *centers,
                        double *boxsize, int *isrc, int *isrcse, int *itarg, int *itargse,
                        int *nlist, int *lists);
+
+/* work counts and step times of the calling thread's last solve (no
+ * counterpart in the reference): out[0] list-1 pairs at sources, [1] at
+ * targets, [2] list-2 pairs, [3] M2L flops (8 ncoef^2 per pair), [4] boxes,
+ * [5] levels, [6] largest expansion order, [7..14] step times in seconds in
+ * timeinfo order.  Returns the number of values available (15). */
+int fmm2dhip_last_stats(double *out, int n);
 
 /* library/device information for diagnostics: returns 0 when a gfx950
  * device is usable and writes its name (at most len bytes). */
