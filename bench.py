@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -122,8 +122,8 @@ def main():
         p2p_s.append(ti[7])
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    stats = np.zeros(15)
-    lib.fmm2dhip_last_stats(stats.ctypes.data_as(vp), 15)
+    stats = np.zeros(23)
+    lib.fmm2dhip_last_stats(stats.ctypes.data_as(vp), 23)
     ms = total / args.steps * 1e3
     value = (n + nt) / (total / args.steps) / 1e6
     pairs = stats[0] + stats[1]
@@ -149,6 +149,9 @@ def main():
                    "nboxes": int(stats[4]), "nlevels": int(stats[5]), "pmax": int(stats[6])},
         "step_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
             ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"], ti)},
+        "host_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
+            ["bbox", "keys_sort_copy", "tree", "lists", "upload", "leaf_order_gather",
+             "operator_setup", "call_total"], stats[15:23])},
         "roofline": {"kernel": "k_hp2p (list-1 near field)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,

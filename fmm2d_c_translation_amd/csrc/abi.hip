@@ -86,6 +86,7 @@ int upload(Scratch& S, hipStream_t st, const T* h, size_t n, const T** d) {
 }
 
 int run_lap(const LapReq& r) {
+  const double t_start = wall();
   Ctx c;
   int e = get_ctx(c);
   if (e) return e;
@@ -173,6 +174,7 @@ int run_lap(const LapReq& r) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.st));
   finish_timer(r.timeinfo);
+  last_stats().host_s[7] = wall() - t_start;
   return IER_OK;
 }
 
@@ -205,6 +207,7 @@ struct HelmReq {
 };
 
 int run_helm(const HelmReq& r) {
+  const double t_start = wall();
   Ctx c;
   int e = get_ctx(c);
   if (e) return e;
@@ -275,6 +278,7 @@ int run_helm(const HelmReq& r) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.st));
   finish_timer(r.timeinfo);
+  last_stats().host_s[7] = wall() - t_start;
   return IER_OK;
 }
 
@@ -561,11 +565,12 @@ int fmm2dhip_test_tree(const double* src, int ns, const double* targ, int nt, in
 
 int fmm2dhip_last_stats(double* out, int n) {
   const fmm::SolveStats& s = fmm::last_stats();
-  double v[15] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
+  double v[23] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
                   s.nlevels, s.pmax};
   for (int k = 0; k < 8; ++k) v[7 + k] = s.step_s[k];
-  for (int k = 0; k < n && k < 15; ++k) out[k] = v[k];
-  return 15;
+  for (int k = 0; k < 8; ++k) v[15 + k] = s.host_s[k];
+  for (int k = 0; k < n && k < 23; ++k) out[k] = v[k];
+  return 23;
 }
 
 int fmm2dhip_device_info(char* name, int len) {

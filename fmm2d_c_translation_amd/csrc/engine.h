@@ -100,7 +100,12 @@ struct SolveStats {
   double m2l_pairs = 0, m2l_flops = 0;          // list-2 pairs, 8 * ncoef^2 per pair
   double nboxes = 0, nlevels = 0, pmax = 0;
   double step_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // host wall time of the front end: 0 bbox, 1 keys+sort+copy, 2 tree,
+  // 3 lists, 4 work lists+upload, 5 leaf order+gather, 6 operator setup,
+  // 7 whole call
+  double host_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+double wall();  // seconds, monotonic
 SolveStats& last_stats();
 
 struct Ctx {

@@ -553,6 +553,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   const int nd = a.nd, L = T.nlevels, nb = T.nboxes;
   const cd zk = cd{a.zk.real(), a.zk.imag()};
   const double zi = a.zk.imag();
+  const double t_setup = wall();
   // per-level order and scale (hfmm2d.c:1768-1779)
   std::vector<HLev> lev(L + 1);
   std::vector<char> gate(L + 2, 0);
@@ -668,6 +669,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   StepTimer* tm = step_timer();
   {
     SolveStats& st = last_stats();
+    st.host_s[6] = wall() - t_setup;
     for (int l = 2; l <= L; ++l) {
       if (!gate[l]) continue;
       double np = 0;

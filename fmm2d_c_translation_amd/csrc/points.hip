@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "engine.h"
@@ -11,6 +12,10 @@
 namespace fmm {
 
 static thread_local StepTimer g_timer;
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 StepTimer* step_timer_peek() { return &g_timer; }
 StepTimer* step_timer() {
   StepTimer& t = g_timer;
@@ -224,6 +229,8 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   const double2* trg = reinterpret_cast<const double2*>(d_trg);
   P.ns = ns;
   P.nt = nt;
+  double th[7];
+  th[0] = wall();
   // ---- bounding box of sources and targets (pts_tree2d.c:493-516)
   int nbk = std::min(1024, std::max(1, nblk(ns + nt, kBB)));
   double4* part = S.get<double4>(nbk);
@@ -242,6 +249,7 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   if (ymax - ymin > P.bs0) P.bs0 = ymax - ymin;
   P.c0x = (xmin + xmax) / 2;
   P.c0y = (ymin + ymax) / 2;
+  th[1] = wall();
 
   // ---- keys + sort
   P.perm_s = S.get<int>(ns);
@@ -254,10 +262,13 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   if (e) return e;
 
   // ---- box-level tree and lists on the host
+  th[2] = wall();
   Tree& T = plan.T;
   e = build_tree(ks.data(), ns, kt.data(), nt, P.c0x, P.c0y, P.bs0, idivflag, ndiv, T);
   if (e) return e;
+  th[3] = wall();
   build_lists(T);
+  th[4] = wall();
   const int nb = T.nboxes, L = T.nlevels;
   for (int b = 0; b < nb; ++b)
     if (T.n1[b] > 13 || T.n2[b] > 27 || T.n3[b] > 20 || T.n4[b] > 5) return IER_ARGS;
@@ -377,6 +388,7 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   D.o4 = reinterpret_cast<const int*>(dbuf + a_o4); D.l4 = reinterpret_cast<const int*>(dbuf + a_v4);
   plan.dwork = reinterpret_cast<int*>(dbuf + a_w);
 
+  th[5] = wall();
   // ---- original-index order inside leaves, then sorted coordinates
   if (plan.leafsort_s.n)
     k_leaf_sort<<<nblk(plan.leafsort_s.n, 64), 64, 0, c.st>>>(P.perm_s, D.srng,
@@ -394,6 +406,9 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   HIP_TRY(hipGetLastError());
   // the pinned upload buffer is reused by the next call: finish the copy now
   HIP_TRY(hipStreamSynchronize(c.st));
+  th[6] = wall();
+  SolveStats& st = last_stats();
+  for (int k = 0; k < 6; ++k) st.host_s[k] = th[k + 1] - th[k];
   return IER_OK;
 }
 

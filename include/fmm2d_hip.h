@@ -190,7 +190,9 @@ int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, in
  * counterpart in the reference): out[0] list-1 pairs at sources, [1] at
  * targets, [2] list-2 pairs, [3] M2L flops (8 ncoef^2 per pair), [4] boxes,
  * [5] levels, [6] largest expansion order, [7..14] step times in seconds in
- * timeinfo order.  Returns the number of values available (15). */
+ * timeinfo order, [15..22] host wall times of the front end (bounding box,
+ * keys and sort, tree, lists, work lists and upload, leaf order and gather,
+ * operator setup, whole call).  Returns the number of values available (23). */
 int fmm2dhip_last_stats(double *out, int n);
 
 /* library/device information for diagnostics: returns 0 when a gfx950
