@@ -1,6 +1,7 @@
 // abi.hip -- the C-ABI layer: gfortran-ABI drop-in entry points and the
 // device-resident variants.  Host pointers are staged through HBM; all
-// compute runs in the device engine (points.hip, tree.cpp, lap.hip).
+// compute runs in the device engine (points.hip, dtree.hip, lap.hip,
+// helm.hip); tree.cpp serves only the host tree test hook.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -558,6 +559,88 @@ int fmm2dhip_test_tree(const double* src, int ns, const double* targ, int nt, in
     for (int i = 0; i < ns; ++i) isrc[i] = is[i] + 1;
     for (int i = 0; i < nt; ++i) itarg[i] = it[i] + 1;
     return 0;
+  } catch (...) {
+    return fmm::IER_ALLOC;
+  }
+}
+
+int fmm2dhip_test_devtree(const double* src, int ns, const double* targ, int nt, int idivflag,
+                          int ndiv, int cap_boxes, int cap_lists, int* nlevels, int* nboxes,
+                          int* nlists, int* boxinfo, double* centers, int* lists, int* isrc,
+                          int* itarg) {
+  try {
+    fmm::Ctx c;
+    int e = fmm::get_ctx(c);
+    if (e) return e;
+    if (ns < 1 || nt < 0) return fmm::IER_ARGS;
+    int out = 0;
+    {
+      fmm::Scratch S(c.st);
+      const double *ds = nullptr, *dt = nullptr;
+      if ((e = fmm::upload(S, c.st, src, 2 * (size_t)ns, &ds))) return e;
+      if ((e = fmm::upload(S, c.st, targ, 2 * (size_t)nt, &dt))) return e;
+      fmm::SortedPoints P;
+      fmm::Plan plan;
+      e = fmm::build_plan(c, S, ds, ns, dt, nt, idivflag, ndiv, P, plan, true, 1, 1);
+      if (e) return e;
+      const fmm::DTree& D = plan.D;
+      const int nb = plan.nb;
+      std::vector<int> o(4 * ((size_t)nb + 1));
+      const int* od[4] = {D.o1, D.o2, D.o3, D.o4};
+      for (int k = 0; k < 4; ++k)
+        HIP_TRY(hipMemcpyAsync(o.data() + (size_t)k * (nb + 1), od[k], sizeof(int) * (nb + 1),
+                               hipMemcpyDeviceToHost, c.st));
+      HIP_TRY(hipStreamSynchronize(c.st));
+      int tot = 0;
+      for (int k = 0; k < 4; ++k) tot += o[(size_t)k * (nb + 1) + nb];
+      *nlevels = plan.L;
+      *nboxes = nb;
+      *nlists = tot;
+      if (nb > cap_boxes || tot > cap_lists) {
+        out = fmm::IER_ARGS;
+      } else {
+        std::vector<uint64_t> key(nb);
+        std::vector<int> level(nb), nch(nb);
+        std::vector<int2> sr(nb), tr(nb);
+        std::vector<double2> ctr(nb);
+        std::vector<std::vector<int>> lv(4);
+        const int* ld[4] = {D.l1, D.l2, D.l3, D.l4};
+        for (int k = 0; k < 4; ++k) {
+          int n = o[(size_t)k * (nb + 1) + nb];
+          lv[k].resize(std::max(n, 1));
+          HIP_TRY(hipMemcpyAsync(lv[k].data(), ld[k], sizeof(int) * n, hipMemcpyDeviceToHost, c.st));
+        }
+        HIP_TRY(hipMemcpyAsync(key.data(), D.key, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(level.data(), D.level, sizeof(int) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(nch.data(), D.nchild, sizeof(int) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(sr.data(), D.srng, sizeof(int2) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(tr.data(), D.trng, sizeof(int2) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(ctr.data(), D.ctr, sizeof(double2) * nb, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipMemcpyAsync(isrc, P.perm_s, sizeof(int) * ns, hipMemcpyDeviceToHost, c.st));
+        if (nt) HIP_TRY(hipMemcpyAsync(itarg, P.perm_t, sizeof(int) * nt, hipMemcpyDeviceToHost, c.st));
+        HIP_TRY(hipStreamSynchronize(c.st));
+        int* lp = lists;
+        for (int b = 0; b < nb; ++b) {
+          int ix = 0, iy = 0;
+          for (int j = 0; j < level[b]; ++j) {
+            ix |= (int)((key[b] >> (2 * j)) & 1) << j;
+            iy |= (int)((key[b] >> (2 * j + 1)) & 1) << j;
+          }
+          int* bi = boxinfo + 12 * (size_t)b;
+          bi[0] = level[b]; bi[1] = ix; bi[2] = iy; bi[3] = nch[b];
+          bi[4] = sr[b].x; bi[5] = sr[b].y; bi[6] = tr[b].x; bi[7] = tr[b].y;
+          for (int k = 0; k < 4; ++k) {
+            int a = o[(size_t)k * (nb + 1) + b], z = o[(size_t)k * (nb + 1) + b + 1];
+            bi[8 + k] = z - a;
+            for (int i = a; i < z; ++i) *lp++ = lv[k][i];
+          }
+          centers[2 * b] = ctr[b].x;
+          centers[2 * b + 1] = ctr[b].y;
+        }
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(c.st));
+    return out;
   } catch (...) {
     return fmm::IER_ALLOC;
   }

@@ -1,0 +1,676 @@
+// dtree.hip -- the adaptive, level-restricted quadtree, its interaction lists
+// and the per-step work lists, built on the device from sorted Morton keys.
+//
+// A box is (level l, Morton prefix key of 2l bits); box ids are level-major
+// with Morton order inside a level, so every per-level range of boxes is a
+// contiguous id range and every work list compacted in id order is sorted
+// by level.  The box set is the reference's:
+//   1. adaptive refinement (c_translation/src/pts_tree2d.c:530-610): a box
+//      is split while its point count (the length of its key range in the
+//      sorted key array) exceeds ndiv;
+//   2. level restriction (pts_tree2d.c:767-960 with tree_routs2d.c:197-318):
+//      the reference's flag passes split exactly the leaves that the 2:1
+//      rule forces ("every same-level neighbour of a split box exists"),
+//      so the result is the closure of the adaptive tree under that rule;
+//      here the closure is taken directly, finest level first, on the sorted
+//      key sets of split boxes: for each split box r at level k, the parents
+//      of r's in-domain neighbours join the split set of level k-1;
+//   3. the boxes of level l+1 are the four children of each split box of
+//      level l.
+// Centres use the reference's recurrence c + (+-1) bs/2 (tree_routs2d.c:81-84)
+// along the key's digits, so they are bit-identical to the reference's.
+// Colleagues and lists 1-4 (tree_routs2d.c:140-185, 369-510) are computed in
+// integer cell coordinates; the reference's 1.05-scaled centre-distance tests
+// separate exactly the same cell offsets.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "engine.h"
+
+namespace fmm {
+namespace {
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+FMM_HD uint32_t compact_bits(uint64_t v) {
+  v &= 0x5555555555555555ull;
+  v = (v | (v >> 1)) & 0x3333333333333333ull;
+  v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+  v = (v | (v >> 16)) & 0x00000000ffffffffull;
+  return (uint32_t)v;
+}
+FMM_HD uint64_t spread_bits(uint32_t x) {
+  uint64_t v = x;
+  v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+  v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+// digit bit 0 is the x half, bit 1 the y half (common.h quadrant)
+FMM_HD void key_cell(uint64_t key, int& ix, int& iy) {
+  ix = (int)compact_bits(key);
+  iy = (int)compact_bits(key >> 1);
+}
+FMM_HD uint64_t cell_key(int ix, int iy) {
+  return spread_bits((uint32_t)ix) | (spread_bits((uint32_t)iy) << 1);
+}
+
+__device__ inline int lbound(const uint64_t* a, int n, uint64_t v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (a[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// points of box (key, level) in a sorted key array
+__device__ inline int2 key_range(const uint64_t* k, int n, uint64_t key, int level) {
+  int a = lbound(k, n, box_key_lo(key, level));
+  int b = lbound(k, n, box_key_lo(key + 1, level));
+  return make_int2(a, b);
+}
+
+struct SplitPred {
+  const uint64_t *ks, *kt;
+  int ns, nt, level, idivflag, ndiv;
+  __device__ bool operator()(const uint64_t& key) const {
+    int cs = 0, ct = 0;
+    if (idivflag != 1) {
+      int2 r = key_range(ks, ns, key, level);
+      cs = r.y - r.x;
+    }
+    if (idivflag != 0) {
+      int2 r = key_range(kt, nt, key, level);
+      ct = r.y - r.x;
+    }
+    // idivflag outside 0..2 counts nothing, so nothing is split
+    int c = idivflag == 0 ? cs : idivflag == 1 ? ct : idivflag == 2 ? max(cs, ct) : 0;
+    return c > ndiv;
+  }
+};
+
+__global__ void k_children(const uint64_t* R, int n, uint64_t* K) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * n) return;
+  K[i] = 4 * R[i >> 2] + (uint64_t)(i & 3);
+}
+
+// parents of the 8 in-domain neighbours of each split box of level k (an
+// out-of-domain neighbour repeats the box's own parent)
+__global__ void k_balance_cand(const uint64_t* R, int n, int k, uint64_t* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix, iy;
+  key_cell(R[i], ix, iy);
+  const int side = 1 << k;
+  for (int s = 0; s < 9; ++s) {
+    int jx = ix + s % 3 - 1, jy = iy + s / 3 - 1;
+    bool in = jx >= 0 && jy >= 0 && jx < side && jy < side;
+    out[(size_t)9 * i + s] = (in ? cell_key(jx, jy) : R[i]) >> 2;
+  }
+}
+
+struct Levels {
+  int L;
+  int loff[kMaxDepth + 2];  // first box id of level l; loff[L+1] = nb
+  int roff[kMaxDepth + 2];  // first split box of level l in the split-key array
+  double bs[kMaxDepth + 1];
+  double c0x, c0y;
+};
+
+__device__ inline int level_of(const Levels& lv, int b) {
+  int l = 0;
+  while (l < lv.L && b >= lv.loff[l + 1]) ++l;
+  return l;
+}
+
+__global__ void k_box_keys(Levels lv, const uint64_t* R, int nb, uint64_t* key) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int l = level_of(lv, b), i = b - lv.loff[l];
+  key[b] = l == 0 ? 0 : 4 * R[lv.roff[l - 1] + (i >> 2)] + (uint64_t)(i & 3);
+}
+
+// per box: level, centre, point ranges, children (and their parent link),
+// colleagues in fixed slots s = (dy+1)*3 + (dx+1)
+__global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int nb,
+                        const uint64_t* ks, int ns, const uint64_t* kt, int nt, int* level,
+                        double2* ctr, int2* srng, int2* trng, int* nchild, int4* child,
+                        int* parent, int* nbr, double* bs_out) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0)
+    for (int l = 0; l <= lv.L; ++l) bs_out[l] = lv.bs[l];
+  if (b >= nb) return;
+  const int l = level_of(lv, b);
+  const uint64_t k = key[b];
+  level[b] = l;
+  double cx = lv.c0x, cy = lv.c0y;
+  for (int j = 1; j <= l; ++j) {
+    int q = (int)(k >> (2 * (l - j))) & 3;
+    cx = child_coord(cx, q & 1, lv.bs[j]);
+    cy = child_coord(cy, q & 2, lv.bs[j]);
+  }
+  ctr[b] = make_double2(cx, cy);
+  srng[b] = key_range(ks, ns, k, l);
+  trng[b] = key_range(kt, nt, k, l);
+  const int nr = lv.roff[l + 1] - lv.roff[l];
+  const uint64_t* Rl = R + lv.roff[l];
+  int rank = lbound(Rl, nr, k);
+  if (rank < nr && Rl[rank] == k) {
+    int c0 = lv.loff[l + 1] + 4 * rank;
+    nchild[b] = 4;
+    child[b] = make_int4(c0, c0 + 1, c0 + 2, c0 + 3);
+    for (int q = 0; q < 4; ++q) parent[c0 + q] = b;
+  } else {
+    nchild[b] = 0;
+    child[b] = make_int4(-1, -1, -1, -1);
+  }
+  if (b == 0) parent[0] = -1;
+  int ix, iy;
+  key_cell(k, ix, iy);
+  const int side = 1 << l, first = lv.loff[l], n = lv.loff[l + 1] - first;
+  for (int s = 0; s < 9; ++s) {
+    int jx = ix + s % 3 - 1, jy = iy + s / 3 - 1, id = -1;
+    if (jx >= 0 && jy >= 0 && jx < side && jy < side) {
+      uint64_t kk = cell_key(jx, jy);
+      int p = lbound(key + first, n, kk);
+      if (p < n && key[first + p] == kk) id = first + p;
+    }
+    nbr[9 * b + s] = id;
+  }
+}
+
+struct Int4Sum {
+  __device__ int4 operator()(const int4& a, const int4& b) const {
+    return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+  }
+};
+
+struct ListOut {
+  int *o1, *o2, *o3, *o4, *l1, *l2, *l3, *l4;
+};
+
+// Lists of box b (tree_routs2d.c:369-510):
+//   list 2: children of the parent's colleagues two or more cells away;
+//   list 1 (leaf b): leaf colleagues (b included), children of split
+//           colleagues touching b, and -- for any b, as the reference adds
+//           them from the coarse side -- leaf colleagues of the parent that
+//           touch b;
+//   list 3 (leaf b): children of split colleagues not touching b;
+//   list 4: leaf colleagues of the parent not touching b.
+// FILL = false counts into cnt[b]; FILL = true writes at the scanned offsets.
+template <bool FILL>
+__global__ void k_lists(const uint64_t* key, const int* level, const int* nchild,
+                        const int4* child, const int* parent, const int* nbr, int nb,
+                        int4* cnt, const int4* off, ListOut out) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  if (b == nb) {
+    if (FILL) {
+      int4 o = off[nb];
+      out.o1[nb] = o.x; out.o2[nb] = o.y; out.o3[nb] = o.z; out.o4[nb] = o.w;
+    } else {
+      cnt[nb] = make_int4(0, 0, 0, 0);
+    }
+    return;
+  }
+  int4 o = FILL ? off[b] : make_int4(0, 0, 0, 0);
+  if (FILL) {
+    out.o1[b] = o.x; out.o2[b] = o.y; out.o3[b] = o.z; out.o4[b] = o.w;
+  }
+  int c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  auto put = [&](int* lst, int base, int& c, int v) {
+    if (FILL) lst[base + c] = v;
+    ++c;
+  };
+  const int l = level[b];
+  const bool leaf = nchild[b] == 0;
+  int ix, iy;
+  key_cell(key[b], ix, iy);
+  if (l == 0) {
+    if (leaf) put(out.l1, o.x, c1, 0);
+  } else {
+    const int p = parent[b];
+    for (int s = 0; s < 9; ++s) {
+      int P = nbr[9 * p + s];
+      if (P < 0) continue;
+      if (nchild[P]) {
+        int4 ch = child[P];
+        int kk[4] = {ch.x, ch.y, ch.z, ch.w};
+        for (int q = 0; q < 4; ++q) {
+          int kx, ky;
+          key_cell(key[kk[q]], kx, ky);
+          if (abs(kx - ix) >= 2 || abs(ky - iy) >= 2) put(out.l2, o.y, c2, kk[q]);
+        }
+      } else if (s != 4) {
+        int px, py;
+        key_cell(key[P], px, py);
+        bool touch = ix >= 2 * px - 1 && ix <= 2 * px + 2 && iy >= 2 * py - 1 && iy <= 2 * py + 2;
+        if (touch) put(out.l1, o.x, c1, P);
+        else put(out.l4, o.w, c4, P);
+      }
+    }
+    if (leaf) {
+      for (int s = 0; s < 9; ++s) {
+        int J = nbr[9 * b + s];
+        if (J < 0) continue;
+        if (!nchild[J]) {
+          put(out.l1, o.x, c1, J);
+          continue;
+        }
+        int4 ch = child[J];
+        int kk[4] = {ch.x, ch.y, ch.z, ch.w};
+        for (int q = 0; q < 4; ++q) {
+          int kx, ky;
+          key_cell(key[kk[q]], kx, ky);
+          bool touch = kx >= 2 * ix - 1 && kx <= 2 * ix + 2 && ky >= 2 * iy - 1 && ky <= 2 * iy + 2;
+          if (touch) put(out.l1, o.x, c1, kk[q]);
+          else put(out.l3, o.z, c3, kk[q]);
+        }
+      }
+    }
+  }
+  if (!FILL) cnt[b] = make_int4(c1, c2, c3, c4);
+}
+
+// ---- work lists: one ordered multi-list compaction over the boxes
+enum { WL_FORMMP, WL_M2M, WL_M2L, WL_L2L, WL_LIST4, WL_LIST3_S, WL_LIST3_T, WL_LEAVES_S,
+       WL_LEAVES_T, WL_P2P_S, WL_P2P_T, WL_N };
+constexpr int WLB = 256;  // boxes per block (4 waves)
+
+struct WlArgs {
+  const int* level;
+  const int* nchild;
+  const int2* srng;
+  const int2* trng;
+  const int4* cnt;  // list lengths n1..n4
+  int nb, pg, pgt;
+};
+
+__device__ inline unsigned wl_mask(const WlArgs& a, int b) {
+  if (b >= a.nb) return 0;
+  int l = a.level[b];
+  bool leaf = a.nchild[b] == 0;
+  int2 s = a.srng[b], t = a.trng[b];
+  int ns = s.y - s.x, nt = t.y - t.x;
+  int np = (a.pgt > 0 ? nt : 0) + (a.pg > 0 ? ns : 0);
+  int4 n = a.cnt[b];
+  unsigned m = 0;
+  m |= (unsigned)(l >= 2 && leaf && ns > 0) << WL_FORMMP;
+  m |= (unsigned)(!leaf && ns > 0) << WL_M2M;
+  m |= (unsigned)(l >= 2 && np > 0 && n.y > 0) << WL_M2L;
+  m |= (unsigned)(np > 0 && !leaf) << WL_L2L;
+  m |= (unsigned)(l >= 2 && np > 0 && n.w > 0) << WL_LIST4;
+  m |= (unsigned)(n.z > 0 && ns > 0) << WL_LIST3_S;
+  m |= (unsigned)(n.z > 0 && nt > 0) << WL_LIST3_T;
+  m |= (unsigned)(leaf && ns > 0) << WL_LEAVES_S;
+  m |= (unsigned)(leaf && nt > 0) << WL_LEAVES_T;
+  m |= (unsigned)(n.x > 0 && ns > 0) << WL_P2P_S;
+  m |= (unsigned)(n.x > 0 && nt > 0) << WL_P2P_T;
+  return m;
+}
+
+// per-block list counts; also the solve's work counts (list-1 pairs, list-2
+// pairs per level)
+__global__ void __launch_bounds__(WLB)
+k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, double* pairs,
+           unsigned long long* m2l_lev) {
+  __shared__ int sc[WL_N];
+  __shared__ unsigned long long sm2l[kMaxDepth + 1];
+  if (threadIdx.x < WL_N) sc[threadIdx.x] = 0;
+  if (threadIdx.x <= kMaxDepth) sm2l[threadIdx.x] = 0;
+  __syncthreads();
+  int b = blockIdx.x * WLB + threadIdx.x;
+  unsigned m = wl_mask(a, b);
+  const int lane = threadIdx.x & 63;
+  for (int j = 0; j < WL_N; ++j) {
+    unsigned long long bal = __ballot((m >> j) & 1);
+    if (lane == 0 && bal) atomicAdd(&sc[j], __popcll(bal));
+  }
+  double ps = 0, pt = 0;
+  if (b < a.nb) {
+    if ((m >> WL_M2L) & 1) atomicAdd(&sm2l[a.level[b]], (unsigned long long)a.cnt[b].y);
+    if (a.cnt[b].x > 0) {
+      double srcs = 0;
+      for (int k = o1[b]; k < o1[b + 1]; ++k) {
+        int2 r = a.srng[l1[k]];
+        srcs += r.y - r.x;
+      }
+      ps = srcs * (a.srng[b].y - a.srng[b].x);
+      pt = srcs * (a.trng[b].y - a.trng[b].x);
+    }
+  }
+  for (int w = 32; w >= 1; w >>= 1) {
+    ps += __shfl_xor(ps, w);
+    pt += __shfl_xor(pt, w);
+  }
+  if (lane == 0 && (ps != 0 || pt != 0)) {
+    atomicAdd(&pairs[0], ps);
+    atomicAdd(&pairs[1], pt);
+  }
+  __syncthreads();
+  if (threadIdx.x < WL_N) bcnt[threadIdx.x * nblocks + blockIdx.x] = sc[threadIdx.x];
+  if (threadIdx.x <= kMaxDepth && sm2l[threadIdx.x])
+    atomicAdd(&m2l_lev[threadIdx.x], sm2l[threadIdx.x]);
+}
+
+// exclusive scan of the per-block counts of every list (one block);
+// info[j] = list base in the output, info[WL_N + 1 + j] = list length
+__global__ void __launch_bounds__(1024) k_wl_scan(int* bcnt, int nblocks, int* info) {
+  __shared__ int part[1024];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  const int per = (nblocks + 1023) / 1024;
+  for (int j = 0; j < WL_N; ++j) {
+    int* c = bcnt + (size_t)j * nblocks;
+    int lo = threadIdx.x * per, hi = min(nblocks, lo + per), s = 0;
+    for (int i = lo; i < hi; ++i) s += c[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      int v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+    for (int i = lo; i < hi; ++i) {
+      int v = c[i];
+      c[i] = run;
+      run += v;
+    }
+    if (threadIdx.x == 1023) {
+      info[j] = base;
+      info[WL_N + 1 + j] = part[1023];
+      base += part[1023];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) info[WL_N] = base;  // start of the list-2 L2L children
+}
+
+__global__ void __launch_bounds__(WLB)
+k_wl_fill(WlArgs a, const int* boff, int nblocks, const int* info, int* out) {
+  __shared__ int wc[WLB / 64][WL_N];
+  int b = blockIdx.x * WLB + threadIdx.x;
+  unsigned m = wl_mask(a, b);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned long long below = (1ull << lane) - 1;
+  int rank[WL_N];
+  for (int j = 0; j < WL_N; ++j) {
+    unsigned long long bal = __ballot((m >> j) & 1);
+    rank[j] = __popcll(bal & below);
+    if (lane == 0) wc[wv][j] = __popcll(bal);
+  }
+  __syncthreads();
+  for (int j = 0; j < WL_N; ++j) {
+    if (!((m >> j) & 1)) continue;
+    int pos = info[j] + boff[(size_t)j * nblocks + blockIdx.x] + rank[j];
+    for (int w = 0; w < wv; ++w) pos += wc[w][j];
+    out[pos] = b;
+  }
+}
+
+// children of the boxes of the L2L list, four per box, after all lists
+__global__ void k_wl_kids(const int* info, const int4* child, int* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = info[WL_N + 1 + WL_L2L];
+  if (i >= n) return;
+  int4 c = child[out[info[WL_L2L] + i]];
+  int* k = out + info[WL_N] + 4 * i;
+  k[0] = c.x; k[1] = c.y; k[2] = c.z; k[3] = c.w;
+}
+
+// lev[j*(L+2) + l] = first entry of list j whose box is at level >= l
+__global__ void k_wl_levels(Levels lv, const int* info, const int* out, int* lev) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int W = lv.L + 2;
+  if (t >= WL_N * W) return;
+  int j = t / W, l = t - j * W;
+  const int* v = out + info[j];
+  int n = info[WL_N + 1 + j];
+  int key = lv.loff[l];
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (v[m] < key) lo = m + 1;
+    else hi = m;
+  }
+  lev[t] = lo;
+}
+
+// FMM2DHIP_DEBUG=1 in the environment prints the tree's shape per solve
+bool debug_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("FMM2DHIP_DEBUG");
+    return v && *v && *v != '0';
+  }();
+  return on;
+}
+
+int sync_read(Ctx& c, const int* d, int n, int* h) {
+  int* pin = static_cast<int*>(pinned(sizeof(int) * n, 4));
+  if (!pin) return IER_ALLOC;
+  HIP_TRY(hipMemcpyAsync(pin, d, sizeof(int) * n, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipStreamSynchronize(c.st));
+  std::memcpy(h, pin, sizeof(int) * n);
+  return IER_OK;
+}
+
+}  // namespace
+
+int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
+                double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
+                int ifpghtarg, Plan& plan, double* host_s) {
+  double t0 = wall();
+  // ---- 1. adaptive refinement, one level per pass
+  std::vector<uint64_t*> R;
+  std::vector<int> nR;
+  uint64_t* K = S.get<uint64_t>(1);
+  int* d_n = S.get<int>(1);
+  if (S.err) return S.err;
+  HIP_TRY(hipMemsetAsync(K, 0, sizeof(uint64_t), c.st));
+  int nK = 1, L = 0;
+  for (int l = 0;; ++l) {
+    uint64_t* Rl = S.get<uint64_t>(nK);
+    if (S.err) return S.err;
+    SplitPred pred{ks, kt, ns, nt, l, idivflag, ndiv};
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceSelect::If(nullptr, tb, K, Rl, d_n, nK, pred, c.st));
+    void* tmp = S.get<char>(tb);
+    if (S.err) return S.err;
+    HIP_TRY(hipcub::DeviceSelect::If(tmp, tb, K, Rl, d_n, nK, pred, c.st));
+    int nr = 0;
+    int e = sync_read(c, d_n, 1, &nr);
+    if (e) return e;
+    R.push_back(Rl);
+    nR.push_back(nr);
+    if (nr == 0) {
+      L = l;
+      break;
+    }
+    if (l + 1 > kMaxDepth) return IER_DEPTH;
+    K = S.get<uint64_t>((size_t)4 * nr);
+    if (S.err) return S.err;
+    k_children<<<nblk(4L * nr, 256), 256, 0, c.st>>>(Rl, nr, K);
+    nK = 4 * nr;
+  }
+  double t1 = wall();
+  // ---- 2. level restriction: closure from the finest split level upward
+  for (int k = L - 1; k >= 2; --k) {
+    size_t nc = (size_t)nR[k - 1] + 9 * (size_t)nR[k];
+    uint64_t* cand = S.get<uint64_t>(nc);
+    uint64_t* sorted = S.get<uint64_t>(nc);
+    uint64_t* Rk1 = S.get<uint64_t>(nc);
+    if (S.err) return S.err;
+    HIP_TRY(hipMemcpyAsync(cand, R[k - 1], sizeof(uint64_t) * nR[k - 1], hipMemcpyDeviceToDevice, c.st));
+    k_balance_cand<<<nblk(nR[k], 256), 256, 0, c.st>>>(R[k], nR[k], k, cand + nR[k - 1]);
+    size_t tb1 = 0, tb2 = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, 2 * (k - 1), c.st));
+    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rk1, d_n, (int)nc, c.st));
+    void* tmp = S.get<char>(std::max(tb1, tb2));
+    if (S.err) return S.err;
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, 2 * (k - 1), c.st));
+    HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rk1, d_n, (int)nc, c.st));
+    int e = sync_read(c, d_n, 1, &nR[k - 1]);
+    if (e) return e;
+    R[k - 1] = Rk1;
+  }
+  double t2 = wall();
+  // ---- 3. boxes
+  Levels lv{};
+  lv.L = L;
+  lv.c0x = c0x;
+  lv.c0y = c0y;
+  lv.bs[0] = bs0;
+  for (int l = 1; l <= L; ++l) lv.bs[l] = lv.bs[l - 1] / 2;
+  lv.loff[0] = 0;
+  lv.roff[0] = 0;
+  for (int l = 0; l <= L; ++l) {
+    int nl = l == 0 ? 1 : 4 * nR[l - 1];
+    lv.loff[l + 1] = lv.loff[l] + nl;
+    lv.roff[l + 1] = lv.roff[l] + nR[l];
+  }
+  const int nb = lv.loff[L + 1];
+  uint64_t* Rall = S.get<uint64_t>(std::max(lv.roff[L + 1], 1));
+  if (S.err) return S.err;
+  for (int l = 0; l < L; ++l)
+    HIP_TRY(hipMemcpyAsync(Rall + lv.roff[l], R[l], sizeof(uint64_t) * nR[l], hipMemcpyDeviceToDevice, c.st));
+  uint64_t* key = S.get<uint64_t>(nb);
+  int* level = S.get<int>(nb);
+  double2* ctr = S.get<double2>(nb);
+  int2* srng = S.get<int2>(nb);
+  int2* trng = S.get<int2>(nb);
+  int* nchild = S.get<int>(nb);
+  int4* child = S.get<int4>(nb);
+  int* parent = S.get<int>(nb);
+  int* nbr = S.get<int>((size_t)9 * nb);
+  double* dbs = S.get<double>(L + 1);
+  int4* cnt = S.get<int4>(nb + 1);
+  int4* off = S.get<int4>(nb + 1);
+  int* o = S.get<int>(4 * ((size_t)nb + 1));
+  if (S.err) return S.err;
+  k_box_keys<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, nb, key);
+  k_boxes<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, key, nb, ks, ns, kt, nt, level, ctr, srng,
+                                          trng, nchild, child, parent, nbr, dbs);
+  // ---- lists 1-4 (count, scan, fill)
+  ListOut lo{o, o + nb + 1, o + 2 * (nb + 1), o + 3 * (nb + 1), nullptr, nullptr, nullptr, nullptr};
+  k_lists<false><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
+                                                     cnt, nullptr, lo);
+  size_t tb = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveScan(nullptr, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
+  void* tmp = S.get<char>(tb);
+  int* tot4 = S.get<int>(4);
+  if (S.err) return S.err;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveScan(tmp, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
+  int tot[4];
+  HIP_TRY(hipMemcpyAsync(tot4, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
+  int e = sync_read(c, tot4, 4, tot);
+  if (e) return e;
+  lo.l1 = S.get<int>(std::max(tot[0], 1));
+  lo.l2 = S.get<int>(std::max(tot[1], 1));
+  lo.l3 = S.get<int>(std::max(tot[2], 1));
+  lo.l4 = S.get<int>(std::max(tot[3], 1));
+  if (S.err) return S.err;
+  k_lists<true><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
+                                                    nullptr, off, lo);
+  // ---- work lists
+  const int nbk = nblk(nb, WLB);
+  const int W = L + 2;
+  const int ninfo = 2 * WL_N + 1;
+  // small results read back in one copy: info, level offsets, m2l pairs
+  // (as 2 ints each), list-1 pairs (2 doubles as 4 ints)
+  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 4;
+  int* small = S.get<int>(nsmall + 2);
+  int* bcnt = S.get<int>((size_t)WL_N * nbk);
+  if (S.err) return S.err;
+  int* info = small;
+  int* lev = small + ninfo;
+  unsigned long long* m2l_lev = reinterpret_cast<unsigned long long*>(small + ((ninfo + WL_N * W + 1) & ~1));
+  double* pairs = reinterpret_cast<double*>(m2l_lev + kMaxDepth + 1);
+  HIP_TRY(hipMemsetAsync(small, 0, sizeof(int) * (nsmall + 2), c.st));
+  WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg};
+  k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, bcnt, nbk, pairs, m2l_lev);
+  k_wl_scan<<<1, 1024, 0, c.st>>>(bcnt, nbk, info);
+  // every list holds at most nb entries; the L2L children at most 4 nb
+  int* work = S.get<int>((size_t)WL_N * nb + 4 * (size_t)nb + 1);
+  if (S.err) return S.err;
+  k_wl_fill<<<nbk, WLB, 0, c.st>>>(wa, bcnt, nbk, info, work);
+  k_wl_kids<<<nblk(nb, 256), 256, 0, c.st>>>(info, child, work);
+  k_wl_levels<<<nblk(WL_N * W, 64), 64, 0, c.st>>>(lv, info, work, lev);
+  HIP_TRY(hipGetLastError());
+  std::vector<int> hs(nsmall + 2);
+  e = sync_read(c, small, nsmall + 2, hs.data());
+  if (e) return e;
+
+  // ---- host view
+  plan.nb = nb;
+  plan.L = L;
+  plan.bs.assign(lv.bs, lv.bs + L + 1);
+  plan.loff.assign(lv.loff, lv.loff + L + 2);
+  plan.dwork = work;
+  WorkList* wl[WL_N] = {&plan.formmp, &plan.m2m, &plan.m2l, &plan.l2l, &plan.list4,
+                        &plan.list3_s, &plan.list3_t, &plan.leaves_s, &plan.leaves_t,
+                        &plan.p2p_s, &plan.p2p_t};
+  for (int j = 0; j < WL_N; ++j) {
+    wl[j]->off = hs[j];
+    wl[j]->n = hs[WL_N + 1 + j];
+    wl[j]->lev.assign(hs.begin() + ninfo + j * W, hs.begin() + ninfo + (j + 1) * W);
+  }
+  plan.l2l_kids.off = hs[WL_N];
+  plan.l2l_kids.n = 4 * plan.l2l.n;
+  plan.l2l_kids.lev.resize(W);
+  for (int l = 0; l < W; ++l) plan.l2l_kids.lev[l] = 4 * plan.l2l.lev[l];
+  const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
+  const unsigned long long* hm = reinterpret_cast<const unsigned long long*>(hs.data() + m2l_at);
+  plan.m2l_pairs.assign(hm, hm + L + 1);
+  const double* hp = reinterpret_cast<const double*>(hm + kMaxDepth + 1);
+
+  DTree& D = plan.D;
+  D.nb = nb;
+  D.key = key;
+  D.ctr = ctr;
+  D.level = level;
+  D.bs = dbs;
+  D.nchild = nchild;
+  D.parent = parent;
+  D.child = child;
+  D.srng = srng;
+  D.trng = trng;
+  D.nbr = nbr;
+  D.o1 = lo.o1; D.l1 = lo.l1; D.o2 = lo.o2; D.l2 = lo.l2;
+  D.o3 = lo.o3; D.l3 = lo.l3; D.o4 = lo.o4; D.l4 = lo.l4;
+
+  if (debug_enabled()) {
+    std::fprintf(stderr, "fmm2dhip tree: L=%d nb=%d lists=%d %d %d %d split/level:", L, nb,
+                 tot[0], tot[1], tot[2], tot[3]);
+    for (int l = 0; l <= L; ++l) std::fprintf(stderr, " %d", nR[l]);
+    std::fprintf(stderr, "\nfmm2dhip work lists (off,n):");
+    for (int j = 0; j < WL_N; ++j) std::fprintf(stderr, " (%d,%d)", wl[j]->off, wl[j]->n);
+    std::fprintf(stderr, " kids (%d,%d)\n", plan.l2l_kids.off, plan.l2l_kids.n);
+  }
+  SolveStats& st = last_stats();
+  st.p2p_pairs_src = hp[0];
+  st.p2p_pairs_trg = hp[1];
+  st.nboxes = nb;
+  st.nlevels = L;
+  double t3 = wall();
+  if (host_s) {
+    host_s[0] = t1 - t0;
+    host_s[1] = t2 - t1;
+    host_s[2] = t3 - t2;
+  }
+  return IER_OK;
+}
+
+}  // namespace fmm

@@ -36,9 +36,16 @@ class Scratch {
   std::vector<void*> ptrs_;
 };
 
+struct Ctx {
+  hipStream_t st = nullptr;
+  int device = 0;
+};
+int get_ctx(Ctx& c);
+
 // Tree arrays on the device (0-based box ids, lists in CSR form).
 struct DTree {
   int nb = 0;
+  const uint64_t* key = nullptr;  // Morton prefix of each box at its level
   const double2* ctr = nullptr;
   const int* level = nullptr;
   const double* bs = nullptr;    // box size per level
@@ -47,6 +54,7 @@ struct DTree {
   const int4* child = nullptr;
   const int2* srng = nullptr;  // sorted source range [x, y)
   const int2* trng = nullptr;  // sorted target range
+  const int* nbr = nullptr;    // colleagues nbr[9*b + (dy+1)*3 + (dx+1)], -1 none
   const int *o1 = nullptr, *l1 = nullptr, *o2 = nullptr, *l2 = nullptr;
   const int *o3 = nullptr, *l3 = nullptr, *o4 = nullptr, *l4 = nullptr;
 };
@@ -61,21 +69,46 @@ struct SortedPoints {
   double2* trg = nullptr;
 };
 
-// Host tree + its device copy + per-level work lists.
-struct Plan {
-  Tree T;
-  DTree D;
-  std::vector<int> work;          // host staging of all work lists (concatenated)
-  int* dwork = nullptr;           // device copy
-  struct Span { int off = 0, n = 0; };
-  Span formmp;                    // leaves at levels >= 2 with sources
-  std::vector<Span> m2m;          // per level: boxes with children
-  std::vector<Span> m2l;          // per level >= 2: boxes with targets and list2
-  std::vector<Span> l2l;          // per level: boxes with targets and children
-  std::vector<Span> l2l_kids;     // per level: the children of those boxes
-  Span list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
-  Span leafsort_s, leafsort_t;    // leaves holding sources / targets
+// Device tree + work lists.  Box ids are level-major (Morton order inside a
+// level); every work list holds box ids in increasing order, hence sorted by
+// level, with lev[l] the first entry at level >= l (lev[L+1] = n).
+struct Span {
+  int off = 0, n = 0;
 };
+struct WorkList {
+  int off = 0, n = 0;    // entries dwork[off .. off+n)
+  std::vector<int> lev;  // size L+2
+  Span all() const { return {off, n}; }
+  Span at(int l) const { return {off + lev[l], lev[l + 1] - lev[l]}; }
+  Span from(int l) const {
+    if (l < 0) l = 0;
+    if (l >= (int)lev.size()) return {off + n, 0};
+    return {off + lev[l], n - lev[l]};
+  }
+};
+struct Plan {
+  int nb = 0, L = 0;            // boxes, deepest level
+  std::vector<double> bs;       // box size per level
+  std::vector<int> loff;        // first box of each level, loff[L+1] = nb
+  DTree D;
+  int* dwork = nullptr;         // device: all work lists
+  WorkList formmp;              // leaves at levels >= 2 with sources
+  WorkList m2m;                 // boxes with children and sources
+  WorkList m2l;                 // levels >= 2: boxes with output points and list 2
+  WorkList l2l;                 // boxes with output points and children
+  WorkList l2l_kids;            // the children of those (lev[] by parent level)
+  WorkList list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
+  std::vector<double> m2l_pairs;  // list-2 pairs of the m2l boxes, per level
+};
+
+// Device tree and lists from sorted keys (dtree.hip).  host_s receives the
+// wall time of refinement, level restriction and boxes+lists.
+int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
+                double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
+                int ifpghtarg, Plan& plan, double* host_s);
+
+// Pinned host staging reused across the calls of one thread (slot < 8).
+void* pinned(size_t bytes, int slot);
 
 // Boundaries of the eight FMM steps, recorded on the solve's stream:
 // ev[k] .. ev[k+1] bracket step order[k]; indices follow the reference's
@@ -100,19 +133,14 @@ struct SolveStats {
   double m2l_pairs = 0, m2l_flops = 0;          // list-2 pairs, 8 * ncoef^2 per pair
   double nboxes = 0, nlevels = 0, pmax = 0;
   double step_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // host wall time of the front end: 0 bbox, 1 keys+sort+copy, 2 tree,
-  // 3 lists, 4 work lists+upload, 5 leaf order+gather, 6 operator setup,
-  // 7 whole call
+  // host wall time of the front end: 0 bbox, 1 keys+sort, 2 adaptive
+  // refinement, 3 level restriction, 4 boxes+lists+work lists, 5 leaf
+  // order+gather, 6 operator setup, 7 whole call
   double host_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 double wall();  // seconds, monotonic
 SolveStats& last_stats();
 
-struct Ctx {
-  hipStream_t st = nullptr;
-  int device = 0;
-};
-int get_ctx(Ctx& c);
 
 // Point front end: bounding box, keys, sort, tree, lists, sorted coordinates.
 int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_trg, int nt,

@@ -548,9 +548,9 @@ int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier) {
 }
 
 int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const HelmArgs& a) {
-  const Tree& T = plan.T;
   const DTree& D = plan.D;
-  const int nd = a.nd, L = T.nlevels, nb = T.nboxes;
+  const int nd = a.nd, L = plan.L, nb = plan.nb;
+  const std::vector<double>& bs = plan.bs;
   const cd zk = cd{a.zk.real(), a.zk.imag()};
   const double zi = a.zk.imag();
   const double t_setup = wall();
@@ -559,11 +559,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   std::vector<char> gate(L + 2, 0);
   int pmax = 0;
   for (int l = 0; l <= L; ++l) {
-    std::complex<double> zt = a.zk * T.bs[l] / (2.0 * kPi);
+    std::complex<double> zt = a.zk * bs[l] / (2.0 * kPi);
     lev[l].rs = std::min(std::abs(zt), 1.0);
     int e = 0;
-    lev[l].p = h2dterms_cached(T.bs[l], a.zk, a.eps, &e);
-    gate[l] = zi * T.bs[l] < 40.0;
+    lev[l].p = h2dterms_cached(bs[l], a.zk, a.eps, &e);
+    gate[l] = zi * bs[l] < 40.0;
     // levels 0 and 1 take part in no M2L, list 3 or list 4 interaction, so
     // their expansions stay zero and are neither formed nor stored
     if (gate[l] && l >= 2) pmax = std::max(pmax, lev[l].p);
@@ -572,7 +572,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // the diagonal (high-frequency) form of boxes above 16 wavelengths is not built
   double dlam = 1.0 / (a.zk.real() / (2 * kPi));
   for (int l = 1; l <= L; ++l)
-    if (T.bs[l] / dlam > 16.0 && gate[l]) return IER_HIGHFREQ;
+    if (bs[l] / dlam > 16.0 && gate[l]) return IER_HIGHFREQ;
   const double2* chg = a.ifcharge ? a.chg : nullptr;
   const double2* dip = a.ifdipole ? a.dip : nullptr;
   if (!chg && !dip) return IER_OK;
@@ -594,7 +594,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   for (int l = 0; l <= L; ++l) {
     if (!gate[l]) continue;
     if (l >= 2 && l <= L - 1) {
-      double h = T.bs[l + 1] / 2;
+      double h = bs[l + 1] / 2;
       size_t sz1 = (size_t)(2 * lev[l + 1].p + 1) * (2 * lev[l].p + 1);
       size_t base_m = toff, base_l = toff + 4 * sz1;
       toff += 8 * sz1;
@@ -618,7 +618,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
         int dx = s % 7 - 3, dy = s / 7 - 3;
         if (std::abs(dx) <= 1 && std::abs(dy) <= 1) continue;
         // source at (dx, dy) bs from the target: c2 - c1 = target - source
-        jobs.push_back({2, lev[l].p, lev[l].p, lev[l].rs, lev[l].rs, -dx * T.bs[l], -dy * T.bs[l], base + s * sz});
+        jobs.push_back({2, lev[l].p, lev[l].p, lev[l].rs, lev[l].rs, -dx * bs[l], -dy * bs[l], base + s * sz});
       }
       vmax = std::max(vmax, 2 * lev[l].p + 4);
     }
@@ -639,32 +639,17 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   while (CH > 8 && shm_of(pmax) > 48 * 1024) CH /= 2;
   size_t shm = shm_of(pmax);
 
-  // host-side filtering when some level is gated off (complex zk only)
-  bool all_on = true;
-  for (int l = 0; l <= L; ++l) all_on = all_on && gate[l];
-  std::vector<int> extra;
-  auto filtered = [&](const Plan::Span& sp, auto pred) -> Plan::Span {
-    if (all_on) return sp;
-    Plan::Span o;
-    o.off = -1 - (int)extra.size();  // marks the extra array
-    for (int i = sp.off; i < sp.off + sp.n; ++i)
-      if (pred(plan.work[i])) extra.push_back(plan.work[i]);
-    o.n = (int)extra.size() - (-1 - o.off);
-    return o;
-  };
-  Plan::Span s_form = filtered(plan.formmp, [&](int b) { return (bool)gate[T.level[b]]; });
-  Plan::Span s_l4 = filtered(plan.list4, [&](int b) { return (bool)gate[T.level[b]]; });
-  Plan::Span s_l3s = filtered(plan.list3_s, [&](int b) { return (bool)gate[T.level[b] + 1]; });
-  Plan::Span s_l3t = filtered(plan.list3_t, [&](int b) { return (bool)gate[T.level[b] + 1]; });
-  Plan::Span s_lvs = filtered(plan.leaves_s, [&](int b) { return (bool)gate[T.level[b]]; });
-  Plan::Span s_lvt = filtered(plan.leaves_t, [&](int b) { return (bool)gate[T.level[b]]; });
-  int* dextra = nullptr;
-  if (!extra.empty()) {
-    dextra = S.get<int>(extra.size());
-    if (S.err) return S.err;
-    HIP_TRY(hipMemcpyAsync(dextra, extra.data(), sizeof(int) * extra.size(), hipMemcpyHostToDevice, c.st));
-  }
-  auto wptr = [&](const Plan::Span& sp) { return sp.off >= 0 ? W + sp.off : dextra + (-1 - sp.off); };
+  // levels with gate[l] false (complex zk, boxes far above the decay length)
+  // form no expansions; gate[l] only switches on as boxes shrink, so the
+  // gated work is a level suffix of every (level-sorted) work list
+  int lg = 0;
+  while (lg <= L && !gate[lg]) ++lg;
+  for (int l = lg; l <= L; ++l)
+    if (!gate[l]) return IER_ARGS;
+  const Span s_form = plan.formmp.from(lg), s_l4 = plan.list4.from(lg);
+  const Span s_l3s = plan.list3_s.from(lg - 1), s_l3t = plan.list3_t.from(lg - 1);
+  const Span s_lvs = plan.leaves_s.from(lg), s_lvt = plan.leaves_t.from(lg);
+  auto wptr = [&](const Span& sp) { return W + sp.off; };
 
   StepTimer* tm = step_timer();
   {
@@ -672,8 +657,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     st.host_s[6] = wall() - t_setup;
     for (int l = 2; l <= L; ++l) {
       if (!gate[l]) continue;
-      double np = 0;
-      for (int i = plan.m2l[l].off; i < plan.m2l[l].off + plan.m2l[l].n; ++i) np += T.n2[plan.work[i]];
+      double np = plan.m2l_pairs[l];
       double nc = 2 * lev[l].p + 1;
       st.m2l_pairs += np;
       st.m2l_flops += np * 8.0 * nc * nc;
@@ -689,11 +673,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // step 3: M2M
   int maxw = 1;
   for (int l = 0; l <= L; ++l)
-    maxw = std::max({maxw, plan.m2m[l].n, plan.m2l[l].n, plan.l2l_kids[l].n});
+    maxw = std::max({maxw, plan.m2m.at(l).n, plan.m2l.at(l).n, plan.l2l_kids.at(l).n});
   int* map = S.get<int>((size_t)maxw * 49);
   if (S.err) return S.err;
   for (int l = L - 1; l >= 2; --l) {
-    const Plan::Span& sp = plan.m2m[l];
+    const Span sp = plan.m2m.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
@@ -707,25 +691,25 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   // step 4: M2L (49 slot matrices per level, contiguous)
   for (int l = 2; l <= L; ++l) {
-    const Plan::Span& sp = plan.m2l[l];
+    const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, T.bs[l], map);
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, bs[l], map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 49, dT + tab[l].m2l[0], 2 * lev[l].p + 1,
                              2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
   }
   tm->mark(c.st);
   // step 5: L2L
   for (int l = 2; l <= L - 1; ++l) {
-    const Plan::Span& sp = plan.l2l_kids[l];
+    const Span sp = plan.l2l_kids.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
                              2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
   }
   // steps 6-8 at sources and targets
-  struct Set { int pg; Plan::Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
-  Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s, D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
-                 {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t, D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
+  struct Set { int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
+  Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
+                 {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
   const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
   tm->mark(c.st);
   for (auto& s : sets)

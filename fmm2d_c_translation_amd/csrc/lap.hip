@@ -419,9 +419,8 @@ void launch_p2p_pg(int pg, int mode, int nbx, hipStream_t st, const int* work, c
 }  // namespace
 
 int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const LapArgs& a) {
-  const Tree& T = plan.T;
   const DTree& D = plan.D;
-  const int nd = a.nd, L = T.nlevels, nb = T.nboxes;
+  const int nd = a.nd, L = plan.L, nb = plan.nb;
   const int p = l2dterms(a.eps);
   if (p > 63) return IER_P_TOO_LARGE;
   const int P = p + 1;
@@ -448,9 +447,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   StepTimer* tm = step_timer();
   {
     SolveStats& st = last_stats();
-    for (int l = 2; l <= L; ++l)
-      for (int i = plan.m2l[l].off; i < plan.m2l[l].off + plan.m2l[l].n; ++i)
-        st.m2l_pairs += T.n2[plan.work[i]];
+    for (int l = 2; l <= L; ++l) st.m2l_pairs += plan.m2l_pairs[l];
     st.m2l_flops = st.m2l_pairs * 8.0 * P * P;
     st.pmax = p;
   }
@@ -463,11 +460,11 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   // step 3: upward pass
   int maxw = 1;
   for (int l = 0; l <= L; ++l)
-    maxw = std::max({maxw, plan.m2m[l].n, plan.m2l[l].n, plan.l2l_kids[l].n});
+    maxw = std::max({maxw, plan.m2m.at(l).n, plan.m2l.at(l).n, plan.l2l_kids.at(l).n});
   int* map = S.get<int>((size_t)maxw * 49);
   if (S.err) return S.err;
   for (int l = L - 1; l >= 1; --l) {
-    const Plan::Span& sp = plan.m2m[l];
+    const Span sp = plan.m2m.at(l);
     if (!sp.n) continue;
     k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, T2, P, P, mp, per, mp, per, nd, 0.0));
@@ -480,24 +477,24 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   tm->mark(c.st);
   // step 4: M2L level by level (log of the box size enters entry (0,0) only)
   for (int l = 2; l <= L; ++l) {
-    const Plan::Span& sp = plan.m2l[l];
+    const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, T.bs[l], map);
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, plan.bs[l], map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 49, TM, P, P, mp, per, lo, per, nd,
-                             std::log(T.bs[l])));
+                             std::log(plan.bs[l])));
   }
   tm->mark(c.st);
   // step 5: L2L into the children of boxes that hold targets
   for (int l = 1; l <= L - 1; ++l) {
-    const Plan::Span& sp = plan.l2l_kids[l];
+    const Span sp = plan.l2l_kids.at(l);
     if (!sp.n) continue;
     k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
     HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, TL, P, P, lo, per, lo, per, nd, 0.0));
   }
   // steps 6 + 7: list 3 and local expansions at sources / targets
-  struct Set { int pg; Plan::Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
-  Set sets[2] = {{a.ifpgh, plan.list3_s, plan.leaves_s, plan.p2p_s, D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
-                 {a.ifpghtarg, plan.list3_t, plan.leaves_t, plan.p2p_t, D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
+  struct Set { int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
+  Set sets[2] = {{a.ifpgh, plan.list3_s.all(), plan.leaves_s.all(), plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
+                 {a.ifpghtarg, plan.list3_t.all(), plan.leaves_t.all(), plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
   // reference skip tests: charges (with or without dipoles) use rr vs thresh^2,
   // '<=' for potential only and '<' otherwise; dipole-only routines compare |z|
   const double thresh = Pt.bs0 * 0x1p-51;

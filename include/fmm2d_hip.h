@@ -174,25 +174,41 @@ void fmm2dhip_hfmm2d_dev(int nd, double eps, double zk_re, double zk_im, int ns,
                          int ifnear, double *timeinfo, void *stream, int *ier);
 
 /* ---- test hook (no counterpart in the reference): builds the box-level
- * tree and lists with keys computed and sorted on the CPU, and returns them
- * in the reference itree layout (pts_tree2d.c:484-491, 1-based ids), plus
- * the point permutations (1-based) and per-box point ranges, and lists as
+ * tree and lists on the host, from Morton keys computed with the engine's
+ * key recurrence and sorted on the CPU, in the reference's own box numbering
+ * and itree layout (pts_tree2d.c:484-491, 1-based ids), plus the point
+ * permutations (1-based) and per-box point ranges, and lists as
  * nlist(4,nboxes) and lists(65,nboxes) = list1(13) list2(27) list3(20)
- * list4(5).  Call with itree == NULL to get the sizes.  The solver entry
- * points never call it; it lets tests check the host half of the tree build
- * on machines without a GPU. */
+ * list4(5).  Call with itree == NULL to get the sizes.  The solvers build
+ * their tree on the device (fmm2dhip_test_devtree); this host restatement
+ * lets CPU-only tests pin the key recurrence and box arithmetic against the
+ * reference's trees. */
 int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, int idivflag,
                        int ndiv, int *nlevels, int *nboxes, int *itree, double *centers,
                        double *boxsize, int *isrc, int *isrcse, int *itarg, int *itargse,
                        int *nlist, int *lists);
+
+/* ---- test hook (no counterpart in the reference): the solvers' device
+ * tree for the given points (copied to the device first; pg = pgt = 1 work
+ * lists).  Boxes are numbered level-major, Morton order inside a level
+ * (0-based).  boxinfo(12,nboxes) = level, ix, iy (cell indices at the
+ * box's level), nchild, sorted source range [s0, s1), target range [t0, t1),
+ * n1..n4; centers(2,nboxes); lists = for each box its list 1, 2, 3 and 4
+ * entries (box ids) back to back; isrc/itarg = original index of each sorted
+ * position (0-based, original order inside each leaf).  Returns 102 with
+ * nlevels/nboxes/nlists set when nboxes > cap_boxes or nlists > cap_lists. */
+int fmm2dhip_test_devtree(const double *src, int ns, const double *targ, int nt, int idivflag,
+                          int ndiv, int cap_boxes, int cap_lists, int *nlevels, int *nboxes,
+                          int *nlists, int *boxinfo, double *centers, int *lists, int *isrc,
+                          int *itarg);
 
 /* work counts and step times of the calling thread's last solve (no
  * counterpart in the reference): out[0] list-1 pairs at sources, [1] at
  * targets, [2] list-2 pairs, [3] M2L flops (8 ncoef^2 per pair), [4] boxes,
  * [5] levels, [6] largest expansion order, [7..14] step times in seconds in
  * timeinfo order, [15..22] host wall times of the front end (bounding box,
- * keys and sort, tree, lists, work lists and upload, leaf order and gather,
- * operator setup, whole call).  Returns the number of values available (23). */
+ * keys and sort, adaptive refinement, level restriction, boxes + lists +
+ * work lists, leaf order and gather, operator setup, whole call).  Returns the number of values available (23). */
 int fmm2dhip_last_stats(double *out, int n);
 
 /* library/device information for diagnostics: returns 0 when a gfx950
