@@ -150,8 +150,8 @@ def main():
         "step_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
             ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"], ti)},
         "host_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
-            ["bbox", "keys_sort_copy", "tree", "lists", "upload", "leaf_order_gather",
-             "operator_setup", "call_total"], stats[15:23])},
+            ["bbox", "keys_sort", "refine", "level_restrict", "boxes_lists_worklists",
+             "leaf_order_gather", "operator_setup", "call_total"], stats[15:23])},
         "roofline": {"kernel": "k_hp2p (list-1 near field)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,

@@ -12,29 +12,35 @@
 
 namespace fmm {
 
-// Stream-ordered device allocations released when the object dies.
+// Device memory of one solve, carved from a per-thread, per-device
+// workspace that persists across solves: allocation is a pointer bump, and
+// the last Scratch of a solve to die synchronises its stream and rewinds the
+// workspace.  A solve that outgrows the workspace takes extra blocks; at its
+// end they are merged into one block sized for that solve.
 class Scratch {
  public:
-  explicit Scratch(hipStream_t s) : st_(s) {}
-  ~Scratch() {
-    for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
-  }
+  explicit Scratch(hipStream_t s);
+  ~Scratch();
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
   template <class T>
   T* get(size_t n) {
-    void* p = nullptr;
-    if (hipMallocAsync(&p, (n ? n : 1) * sizeof(T), st_) != hipSuccess) {
+    void* p = take((n ? n : 1) * sizeof(T));
+    if (!p) {
       err = IER_ALLOC;
       return nullptr;
     }
-    ptrs_.push_back(p);
     return static_cast<T*>(p);
   }
   int err = IER_OK;
 
  private:
+  void* take(size_t bytes);
   hipStream_t st_;
-  std::vector<void*> ptrs_;
+  int dev_ = 0;
 };
+// Frees the calling thread's workspace on the current device.
+void release_workspace();
 
 struct Ctx {
   hipStream_t st = nullptr;

@@ -53,15 +53,70 @@ int get_ctx(Ctx& c) {
   if (!st || dev != d) {
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return IER_NOGPU;
     dev = d;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
-      uint64_t keep = UINT64_MAX;
-      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    }
   }
   c.st = st;
   c.device = d;
   return IER_OK;
+}
+
+namespace {
+constexpr int kMaxDevices = 16;
+struct Arena {
+  std::vector<std::pair<char*, size_t>> blocks;  // the last one is being filled
+  size_t used = 0;   // bytes taken from the last block
+  size_t total = 0;  // bytes taken by the current solve
+  int depth = 0;     // live Scratch objects
+};
+thread_local Arena g_arena[kMaxDevices];
+constexpr size_t kBlockMin = size_t(64) << 20;
+}  // namespace
+
+Scratch::Scratch(hipStream_t s) : st_(s) {
+  if (hipGetDevice(&dev_) != hipSuccess || dev_ < 0 || dev_ >= kMaxDevices) dev_ = 0;
+  g_arena[dev_].depth++;
+}
+
+void* Scratch::take(size_t bytes) {
+  Arena& A = g_arena[dev_];
+  bytes = (bytes + 255) & ~size_t(255);
+  A.total += bytes;
+  if (!A.blocks.empty() && A.used + bytes <= A.blocks.back().second) {
+    void* p = A.blocks.back().first + A.used;
+    A.used += bytes;
+    return p;
+  }
+  size_t want = std::max(bytes, kBlockMin);
+  char* p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) return nullptr;
+  A.blocks.push_back({p, want});
+  A.used = bytes;
+  return p;
+}
+
+Scratch::~Scratch() {
+  Arena& A = g_arena[dev_];
+  if (--A.depth > 0) return;
+  (void)hipStreamSynchronize(st_);
+  if (A.blocks.size() > 1) {
+    size_t want = std::max(A.total + A.total / 4, kBlockMin);
+    for (auto& b : A.blocks) (void)hipFree(b.first);
+    A.blocks.clear();
+    char* p = nullptr;
+    if (hipMalloc(&p, want) == hipSuccess) A.blocks.push_back({p, want});
+  }
+  A.used = 0;
+  A.total = 0;
+}
+
+void release_workspace() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) return;
+  Arena& A = g_arena[d];
+  if (A.depth > 0) return;
+  (void)hipDeviceSynchronize();
+  for (auto& b : A.blocks) (void)hipFree(b.first);
+  A.blocks.clear();
+  A.used = A.total = 0;
 }
 
 void* pinned(size_t bytes, int slot) {

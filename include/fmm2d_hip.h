@@ -208,8 +208,15 @@ int fmm2dhip_test_devtree(const double *src, int ns, const double *targ, int nt,
  * [5] levels, [6] largest expansion order, [7..14] step times in seconds in
  * timeinfo order, [15..22] host wall times of the front end (bounding box,
  * keys and sort, adaptive refinement, level restriction, boxes + lists +
- * work lists, leaf order and gather, operator setup, whole call).  Returns the number of values available (23). */
+ * work lists, leaf order and gather, operator setup, whole call including
+ * the end-of-solve synchronisation).  Returns the number of values available (23). */
 int fmm2dhip_last_stats(double *out, int n);
+
+/* frees the calling thread's device workspace on the current device (no
+ * counterpart in the reference).  Each thread keeps, per device, the device
+ * memory of its largest solve so far and reuses it for the next solve;
+ * this returns it to the system.  Safe to call between solves only. */
+void fmm2dhip_release_workspace(void);
 
 /* library/device information for diagnostics: returns 0 when a gfx950
  * device is usable and writes its name (at most len bytes). */
