@@ -292,18 +292,25 @@ k_heval(const int* work, DTree D, const HLev* lev, cd zk, const int2* rng, const
 // Laplace kernel.  CP: charge-only potential, which skips on |z| <= thresh
 // (all other routines on <).
 // ---------------------------------------------------------------------
-constexpr int HCAP = 256;
+constexpr int HCAP = 128;  // sources staged per pass
 
-template <bool C, bool Q, int PG>
+// NR > 0: real positive wavenumber and every k r of the launch at most
+// JY_XMAX for the series length NR, so H0/H1 come from hank_series_real<NR>;
+// NR = 0: any wavenumber, complex evaluator
+template <bool C, bool Q, int PG, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
        const double2* dip, const double* dvv, int NS, const int2* trng, const double2* tpts,
        int NT, int nd, double thresh, double2* pot, double2* grad, double2* hess) {
-  __shared__ double2 sx[HCAP];
-  __shared__ double2 sc[HCAP];
-  __shared__ double2 sq[HCAP];
-  __shared__ double2 sv[HCAP];
-  __shared__ double2 red[7 * WAVE];
+  // staging (x, charge, dipole strength, dipole vector) and, after each
+  // target group, the slice reduction share one LDS buffer
+  __shared__ double2 lds[4 * HCAP];
+  double2* sx = lds;
+  double2* sc = lds + HCAP;
+  double2* sq = lds + 2 * HCAP;
+  double2* sv = lds + 3 * HCAP;
+  double2* red = lds;
+  static_assert(7 * WAVE <= 4 * HCAP, "reduction buffer");
   __shared__ int boff[14], bsrc[14];
   const int b = work[blockIdx.x];
   const int lane = threadIdx.x;
@@ -349,21 +356,29 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
           for (int m = k; m < cnt; m += K) {
             double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
             double rr = dx * dx + dy * dy, r = sqrt(rr);
-            cd z = zk * r;
-            double az = cabs_(z);
+            cd z, h0, h1;
+            double az;
+            if (NR > 0) {
+              az = zk.x * r;
+              z = cd{az, 0.0};
+            } else {
+              z = zk * r;
+              az = cabs_(z);
+            }
             if (cp_mode ? az <= thresh : az < thresh) continue;
-            cd h0, h1;
-            hank01(z, h0, h1);
+            if (NR > 0) hank_series_real<(NR > 0 ? NR : 1)>(az, h0, h1);
+            else hank01(z, h0, h1);
+            const double ri = 1.0 / r;
             if (C) {
               cd c = ld(&sc[m]);
               A.Pc += h0 * c;
               if (PG >= 2) {
-                cd cd_ = -(h1 * c * (1.0 / r));
+                cd cd_ = -(h1 * c * ri);
                 A.Gx += cd_ * dx;
                 A.Gy += cd_ * dy;
               }
               if (PG >= 3) {
-                cd f = c * (1.0 / (r * rr)), h2z = -(z * h0) + h1 * 2.0;
+                cd f = c * (ri / rr), h2z = -(z * h0) + h1 * 2.0;
                 A.Hxx += f * (h2z * (dx * dx) - h1 * rr);
                 A.Hxy += f * (h2z * (dx * dy));
                 A.Hyy += f * (h2z * (dy * dy) - h1 * rr);
@@ -372,17 +387,18 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
             if (Q) {
               cd q = ld(&sq[m]);
               double v1 = sv[m].x, v2 = sv[m].y;
-              double ct = dx / r, st = dy / r;
-              A.Pd += h1 * (1.0 / r) * q * (dx * v1 + dy * v2);
+              double ct = dx * ri, st = dy * ri;
+              A.Pd += h1 * ri * q * (dx * v1 + dy * v2);
               if (PG >= 2) {
-                cd h2 = h1 * inv(z) * 2.0 - h0;
+                cd zi = NR > 0 ? cd{1.0 / az, 0.0} : inv(z);
+                cd h2 = h1 * zi * 2.0 - h0;
                 cd f1 = h2 * (ct * ct - 0.5) - h0 * 0.5, f2 = h2 * (ct * st);
                 cd f3 = h2 * (st * st - 0.5) - h0 * 0.5;
                 cd qk = -(q * zk);
                 A.Gx += qk * (f1 * v1 + f2 * v2);
                 A.Gy += qk * (f2 * v1 + f3 * v2);
                 if (PG >= 3) {
-                  cd h3 = h2 * inv(z) * 4.0 - h1;
+                  cd h3 = h2 * zi * 4.0 - h1;
                   cd a = (h1 * 0.75 - h3 * (0.5 * (ct * ct - 0.5 - st * st))) * ct;
                   cd bb = (h1 * 0.25 - h3 * (0.5 * (1.5 * ct * ct - 0.5 * st * st))) * st;
                   cd c2 = (h1 * 0.25 - h3 * (0.5 * (1.5 * st * st - 0.5 * ct * ct))) * ct;
@@ -477,14 +493,28 @@ __global__ void __launch_bounds__(256) k_hops(const OpJob* jobs, cd zk, double2*
   }
 }
 
+// xmax: bound on k r over the launch's pairs (0 when zk is not real and
+// positive)
 template <bool C, bool Q>
 void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& D, cd zk,
                  const double2* src, const double2* chg, const double2* dip, const double* dv,
                  int NS, const int2* trng, const double2* tpts, int NT, int nd, double th,
-                 double2* pot, double2* grad, double2* hess) {
-  if (pg == 1) k_hp2p<C, Q, 1><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
-  if (pg == 2) k_hp2p<C, Q, 2><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
-  if (pg == 3) k_hp2p<C, Q, 3><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, NT, nd, th, pot, grad, hess);
+                 double xmax, double2* pot, double2* grad, double2* hess) {
+  const int nr = xmax <= 0.0 ? 0 : xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
+#define FMM_HP2P(PGV, NRV)                                                                  \
+  k_hp2p<C, Q, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, \
+                                               NT, nd, th, pot, grad, hess)
+#define FMM_HP2P_NR(PGV)                                 \
+  do {                                                   \
+    if (nr == JY_NTERMS0) FMM_HP2P(PGV, JY_NTERMS0);     \
+    else if (nr == JY_NTERMS1) FMM_HP2P(PGV, JY_NTERMS1); \
+    else FMM_HP2P(PGV, 0);                               \
+  } while (0)
+  if (pg == 1) FMM_HP2P_NR(1);
+  if (pg == 2) FMM_HP2P_NR(2);
+  if (pg == 3) FMM_HP2P_NR(3);
+#undef FMM_HP2P_NR
+#undef FMM_HP2P
 }
 
 template <bool MPK>
@@ -711,6 +741,15 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
                  {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
   const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
+  // k r over list 1: a leaf of size s meets boxes of size 2s at most, so
+  // |dx|, |dy| <= 3 s and r <= 4.25 s, s the largest such leaf
+  double xmax = 0.0;
+  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) {
+    int lmin = L + 1;
+    for (int l = 0; l <= L && lmin > L; ++l)
+      if (plan.p2p_s.at(l).n || plan.p2p_t.at(l).n) lmin = l;
+    if (lmin <= L) xmax = a.zk.real() * 4.25 * bs[lmin];
+  }
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.l3.n)
@@ -723,11 +762,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
     if (chg && dip)
-      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
     else if (chg)
-      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
     else
-      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, s.po, s.gr, s.he);
+      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
   }
   tm->mark(c.st);
   HIP_TRY(hipGetLastError());

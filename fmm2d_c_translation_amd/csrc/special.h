@@ -1,7 +1,10 @@
 // special.h -- FP64 Hankel/Bessel functions for device code.
 //
 // H0^(1), H1^(1) of complex argument, our own evaluation by region:
-//   |z| <= 4          ascending series (Abramowitz & Stegun 9.1.10-9.1.11)
+//   |z| <= 4          ascending series (Abramowitz & Stegun 9.1.10-9.1.11);
+//                     for a real positive argument the same series in real
+//                     arithmetic from tabulated coefficients
+//                     (hank_series_real)
 //   4 < |z| < 20      Laplace-type integral
 //                       H_nu(z) = sqrt(2/(pi z)) e^{i(z - nu pi/2 - pi/4)} / Gamma(nu+1/2)
 //                                 * int_0^inf e^{-t} t^{nu-1/2} (1 + i t/(2z))^{nu-1/2} dt
@@ -16,6 +19,7 @@
 
 #include "common.h"
 #include "gl20.h"
+#include "jy_series.h"
 
 namespace fmm {
 
@@ -61,6 +65,31 @@ FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
   cd y1 = -(inv(z) * (2.0 / kPi)) + (lz * j1) * (2.0 / kPi) - (z * 0.5) * s1 * (1.0 / kPi);
   h0 = j0 + mul_i(y0);
   h1 = j1 + mul_i(y1);
+}
+
+// H0 = J0 + i Y0 and H1 = J1 + i Y1 of a real argument 0 < x <= JY_XMAX
+// of the chosen N (jy_series.h): the ascending series of hank_series in
+// real arithmetic, J0 = P(w) and the Y0 remainder Q(w) in w = x^2/4 by
+// Horner with their derivatives, J1 = -P'(w) x/2 and
+// Y1 = -(2/pi) [J0/x - (ln(x/2) + gamma) J1 + Q'(w) x/2]
+// (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
+template <int N>
+FMM_HD void hank_series_real(double x, cd& h0, cd& h1) {
+  const double w = x * x * 0.25;
+  double p = JY_JA[N - 1], dp = 0.0, q = JY_YC[N - 1], dq = 0.0;
+#pragma unroll
+  for (int k = N - 2; k >= 0; --k) {
+    dp = fma(dp, w, p);
+    p = fma(p, w, JY_JA[k]);
+    dq = fma(dq, w, q);
+    q = fma(q, w, JY_YC[k]);
+  }
+  const double hx = 0.5 * x, lg = log(hx) + kEuler;
+  const double j1 = -dp * hx;
+  const double y0 = (lg * p + q) * (2.0 / kPi);
+  const double y1 = -(2.0 / kPi) * (p / x - lg * j1 + dq * hx);
+  h0 = cd{p, y0};
+  h1 = cd{j1, y1};
 }
 
 FMM_HD void hank_laguerre(cd z, cd& h0, cd& h1) {
