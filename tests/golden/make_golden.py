@@ -1,6 +1,7 @@
 """Writes tests/golden/*.npz: inputs and outputs of the reference C
 translation (compiled from its own sources into oracle/_ref by
-oracle/Makefile) for small seeded cases.  The reference ships no stored
+oracle/Makefile) for small seeded cases: Laplace (cfmm2d/lfmm2d + trees),
+Helmholtz (hfmm2d) and the Hankel evaluator hank103 on a grid.  The reference ships no stored
 golden vectors (its tests compute expectations at run time), so these are
 the fixtures the oracle and the engine are pinned to where the reference
 library is absent.  Re-run after `make -C oracle ref`."""
@@ -62,6 +63,50 @@ def main():
                     out[f"lfmm_{tag}_{k}"] = r[k]
         np.savez_compressed(os.path.join(HERE, f"lap_{name}.npz"), **out)
         print("wrote", name)
+    helm(ref)
+    hank(ref)
+
+
+def helm(ref):
+    """hfmm2d_ / hfmm2d_ndiv_ outputs of the reference for small cases."""
+    rng = np.random.default_rng(2025)
+    for name, n, nt, nd, zk in [("unif_zk12", 600, 200, 1, 12.5),
+                                ("unif_nd2_zkc", 900, 150, 2, 3.0 + 0.5j)]:
+        src = rng.random((2, n))
+        tg = rng.random((2, nt))
+        ch = rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))
+        dp = rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))
+        th = rng.random((nd, n)) * 2 * np.pi
+        dv = np.stack([np.cos(th), np.sin(th)], axis=1)
+        out = {"src": src, "targ": tg, "charge": ch, "dipstr": dp, "dipvec": dv,
+               "zk": np.array(zk, complex)}
+        for eps in (1e-6, 1e-12):
+            tag = f"e{int(-np.log10(eps))}"
+            r = abi.hfmm2d_call(ref, "hfmm2d_ndiv_", eps, zk, src, ch, dp, dv, 3, tg, 3, nd, ndiv=15)
+            assert r.ier == 0
+            for k in ("pot", "grad", "hess", "targ_pot", "targ_grad", "targ_hess"):
+                out[f"hfmm_{tag}_{k}"] = r[k]
+        np.savez_compressed(os.path.join(HERE, f"helm_{name}.npz"), **out)
+        print("wrote helm", name)
+
+
+def hank(ref):
+    """hank103_ (H0, H1) of the reference on a grid covering its regimes."""
+    import ctypes as C
+    re = np.concatenate([-np.geomspace(60, 1e-3, 25), np.geomspace(1e-3, 60, 25)])
+    im = np.concatenate([-np.geomspace(8, 1e-3, 9), [0.0], np.geomspace(1e-3, 30, 12)])
+    z = (re[:, None] + 1j * im[None, :]).ravel()
+    h = np.zeros((z.size, 2), complex)
+    zz = (C.c_double * 2)()
+    h0 = (C.c_double * 2)()
+    h1 = (C.c_double * 2)()
+    one = C.c_int32(1)
+    for i, v in enumerate(z):
+        zz[0], zz[1] = v.real, v.imag
+        ref.hank103_(zz, h0, h1, C.byref(one))
+        h[i] = (h0[0] + 1j * h0[1], h1[0] + 1j * h1[1])
+    np.savez_compressed(os.path.join(HERE, "hank103.npz"), z=z, h=h)
+    print("wrote hank103", z.size)
 
 
 if __name__ == "__main__":

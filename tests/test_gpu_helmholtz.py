@@ -3,6 +3,9 @@ restatement on the same seeded inputs, and against the compiled reference
 where present.  Same tree on both sides; tolerance 1e-11 relative l2 --
 the two sides differ in summation order and in the H0/H1 evaluator (ours vs
 the reference's hank103), both accurate to ~1e-15."""
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -70,3 +73,21 @@ def test_hfmm2d_large_arguments(eng, orc, zk):
     a = abi.hfmm2d_call(eng, "hfmm2d_", **kw)
     b = abi.hfmm2d_call(orc, "orc_hfmm2d_", **kw)
     _compare(a, b)
+
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "helm_*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[p.rsplit("/", 1)[-1] for p in GOLD])
+@pytest.mark.parametrize("eps", [1e-6, 1e-12])
+def test_hfmm2d_matches_reference_fixtures(eng, path, eps):
+    """Outputs of the reference's hfmm2d_ndiv_ written by tests/golden/make_golden.py."""
+    print(f"hfmm2d golden {path} eps={eps}", flush=True)
+    g = dict(np.load(path, allow_pickle=False))
+    tag = f"e{int(-np.log10(eps))}"
+    nd = g["charge"].shape[0]
+    a = abi.hfmm2d_call(eng, "hfmm2d_ndiv_", eps, complex(g["zk"]), g["src"], g["charge"],
+                        g["dipstr"], g["dipvec"], 3, g["targ"], 3, nd, ndiv=15)
+    assert a.ier == 0
+    for k in KEYS:
+        assert rel_l2(a[k], g[f"hfmm_{tag}_{k}"]) < TOL, k
