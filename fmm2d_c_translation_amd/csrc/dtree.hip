@@ -449,6 +449,16 @@ __global__ void k_wl_levels(Levels lv, const int* info, const int* out, int* lev
   lev[t] = lo;
 }
 
+// M2L list entries keyed by (level, child quadrant): a box of level l >= 1 is
+// child q = (id - loff[l]) & 3 of its parent
+__global__ void k_m2l_keys(Levels lv, const int* list, int n, unsigned char* key, int* val) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int b = list[e], l = level_of(lv, b);
+  key[e] = (unsigned char)(4 * l + ((b - lv.loff[l]) & 3));
+  val[e] = b;
+}
+
 // FMM2DHIP_DEBUG=1 in the environment prints the tree's shape per solve
 bool debug_enabled() {
   static const bool on = [] {
@@ -631,6 +641,24 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   plan.l2l_kids.n = 4 * plan.l2l.n;
   plan.l2l_kids.lev.resize(W);
   for (int l = 0; l < W; ++l) plan.l2l_kids.lev[l] = 4 * plan.l2l.lev[l];
+  // M2L targets of a level grouped by child quadrant (stable, so each group
+  // stays in Morton order): the 16 targets of an M2L tile then share their
+  // 27 list-2 offsets
+  if (plan.m2l.n > 0) {
+    const int n = plan.m2l.n;
+    unsigned char* k0 = S.get<unsigned char>(n);
+    unsigned char* k1 = S.get<unsigned char>(n);
+    int* v0 = S.get<int>(n);
+    if (S.err) return S.err;
+    k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, work + plan.m2l.off, n, k0, v0);
+    size_t tbs = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tbs, k0, k1, v0, work + plan.m2l.off, n,
+                                               0, 8, c.st));
+    void* tmps = S.get<char>(tbs);
+    if (S.err) return S.err;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n, 0,
+                                               8, c.st));
+  }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
   const unsigned long long* hm = reinterpret_cast<const unsigned long long*>(hs.data() + m2l_at);
   plan.m2l_pairs.assign(hm, hm + L + 1);

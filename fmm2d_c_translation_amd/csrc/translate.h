@@ -170,6 +170,112 @@ k_translate_direct(const int* __restrict__ work, const int* __restrict__ map, in
     }
 }
 
+// ---------------------------------------------------------------------
+// M2L on the FP64 matrix cores.  For one level, Y[work[w]] += sum_s T_s
+// X[map[w][s]] with T_s the P x P complex operator of list-2 offset s
+// (P <= 16 MT).  Complex products as four real 16x16x4 MFMAs:
+//   Yr += Tr Xr + Ti (-Xi),  Yi += Tr Xi + Ti Xr.
+// A wave owns 16 work items (the N columns) and all MT row tiles; lane l
+// holds A[l%16][l/16] = T_s[k0 + l/16][16t + l%16] and B[l/16][l%16] =
+// X[k0 + l/16] of column l%16, and accumulates D[4v + l/16][l%16]
+// (v = 0..3) -- the v_mfma_f64_16x16x4f64 operand layout.  A slot is
+// skipped when none of the wave's 16 columns has a source there, so tiles of
+// targets that share a child quadrant (same 27 offsets) waste no work.
+// lap_log as in k_translate.
+// ---------------------------------------------------------------------
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+inline hipError_t launch_translate(hipStream_t st, const int* work, int nwork, const int* map,
+                                   int nslot, const double2* T, int nin, int nout,
+                                   const double2* X, size_t xs, double2* Y, size_t ys, int nd,
+                                   double lap_log);
+
+template <int MT>
+__global__ void __launch_bounds__(256)
+k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
+           const double2* __restrict__ T, int P, const double2* __restrict__ X, size_t xstride,
+           double2* __restrict__ Y, size_t ystride, int nd, double lap_log) {
+  const int lane = threadIdx.x & 63;
+  const int n = lane & 15, kq = lane >> 4;
+  const int w = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + n;
+  const bool wok = w < nwork;
+  for (int d = 0; d < nd; ++d) {
+    mfma_d4 cr[MT], ci[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      cr[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      ci[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    }
+    double s0r = 0.0, s0i = 0.0;  // sum of x_0 over the sources (lanes kq == 0)
+    for (int s = 0; s < 49; ++s) {
+      const int src = wok ? map[(size_t)w * 49 + s] : -1;
+      if (!__any(src >= 0)) continue;
+      const double2* xs = X + (size_t)(src >= 0 ? src : 0) * xstride + (size_t)d * P;
+      const double2* Ts = T + (size_t)s * P * P;
+#pragma unroll 2
+      for (int k0 = 0; k0 < P; k0 += 4) {
+        const int j = k0 + kq;
+        const bool jok = j < P;
+        const double2 x = (src >= 0 && jok) ? xs[j] : make_double2(0.0, 0.0);
+        if (k0 == 0) {
+          s0r += x.x;
+          s0i += x.y;
+        }
+        const double xin = -x.y;
+        double ar[MT], ai[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          const int i = 16 * t + n;
+          const double2 a = (jok && i < P) ? Ts[(size_t)j * P + i] : make_double2(0.0, 0.0);
+          ar[t] = a.x;
+          ai[t] = a.y;
+        }
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], x.x, cr[t], 0, 0, 0);
+          ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], x.y, ci[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], xin, cr[t], 0, 0, 0);
+          ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], x.x, ci[t], 0, 0, 0);
+        }
+      }
+    }
+    if (!wok) continue;
+    double2* y = Y + (size_t)work[w] * ystride + (size_t)d * P;
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * t + 4 * v + kq;
+        if (i >= P) continue;
+        double re = cr[t][v], im = ci[t][v];
+        if (i == 0 && lap_log != 0.0) {
+          re += s0r * lap_log;
+          im += s0i * lap_log;
+        }
+        y[i] = make_double2(y[i].x + re, y[i].y + im);
+      }
+  }
+}
+
+// M2L through k_m2l_mfma when the order fits (P <= 64), else k_translate.
+inline hipError_t launch_m2l(hipStream_t st, const int* work, int nwork, const int* map,
+                             const double2* T, int P, const double2* X, size_t xs, double2* Y,
+                             size_t ys, int nd, double lap_log) {
+  if (nwork <= 0) return hipSuccess;
+  const int nb = (nwork + 63) / 64;
+  switch ((P + 15) / 16) {
+    case 1: k_m2l_mfma<1><<<nb, 256, 0, st>>>(work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log); break;
+    case 2: k_m2l_mfma<2><<<nb, 256, 0, st>>>(work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log); break;
+    case 3: k_m2l_mfma<3><<<nb, 256, 0, st>>>(work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log); break;
+    case 4: k_m2l_mfma<4><<<nb, 256, 0, st>>>(work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log); break;
+    default: return launch_translate(st, work, nwork, map, 49, T, P, P, X, xs, Y, ys, nd, lap_log);
+  }
+  return hipGetLastError();
+}
+
 constexpr int XT_TB = 16;
 constexpr size_t XT_LDS_MAX = 150 * 1024;
 
