@@ -724,7 +724,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, bs[l], map);
-    HIP_TRY(launch_m2l(c.st, W + sp.off, sp.n, map, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
+    HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
                        per, lo, per, nd, 0.0));
   }
   tm->mark(c.st);

@@ -480,7 +480,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
     k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, plan.bs[l], map);
-    HIP_TRY(launch_m2l(c.st, W + sp.off, sp.n, map, TM, P, mp, per, lo, per, nd,
+    HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, TM, P, mp, per, lo, per, nd,
                        std::log(plan.bs[l])));
   }
   tm->mark(c.st);
