@@ -155,9 +155,10 @@ struct NearAcc {
 };
 
 // contribution of order n (with value v = scaled H or J at the point,
-// pos = e^{i n theta}, neg = (-e^{-i theta})^n)
-template <int PG>
-__device__ inline void eval_term(const Ex& E, int n, cd v, cd pos, cd neg, EvalAcc& A) {
+// complex or, for J of a real argument, real; pos = e^{i n theta},
+// neg = (-e^{-i theta})^n)
+template <int PG, class V>
+__device__ inline void eval_term(const Ex& E, int n, V v, cd pos, cd neg, EvalAcc& A) {
   if (n == 0) {
     A.P += v * E.c(0);
     if (PG >= 2) { A.Gx += v * E.dx(0); A.Gy += v * E.dy(0); }
@@ -247,15 +248,99 @@ __device__ void eval_lo_at(const Ex& E, cd zk, double rs, double r, double th, E
   A.Hxx += B.Hxx * sc; A.Hxy += B.Hxy * sc; A.Hyy += B.Hyy * sc;
 }
 
-// wave per box, lane per point; MPK: list-3 multipoles, else own local
-template <bool MPK, int PG>
+// Real positive wavenumber, x = k r: the same two evaluations with real
+// Bessel arithmetic.  Multipole: H0, H1 from hank_series_real<NR> (the
+// launch guarantees x <= JY_XMAX for NR) and the upward recurrence with real
+// multipliers.  Local: J_n / rs^n by Miller's downward recurrence, normalised
+// with the Neumann sum J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.46).
+template <int PG, int NR>
+__device__ void eval_mp_at_real(const Ex& E, double k, double rs, double r, double th,
+                                EvalAcc& A) {
+  const double x = k * r;
+  if (x < 1e-200) return;
+  cd h0, h1;
+  hank_series_real<NR>(x, h0, h1);
+  cd e1 = mk(cos(th), sin(th)), pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  const double zi = rs / x, s2 = rs * rs;
+  cd vm = h0, v = h1 * rs;
+  eval_term<PG>(E, 0, h0, pos, neg, A);
+  const int top = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
+  for (int n = 1; n <= top; ++n) {
+    pos = pos * e1;
+    neg = neg * mi;
+    eval_term<PG>(E, n, v, pos, neg, A);
+    cd vn = v * (zi * (2.0 * n)) - vm * s2;
+    vm = v;
+    v = vn;
+  }
+}
+
+__device__ inline int miller_top_real(int nterms, double x) {
+  const double xi = 1.0 / x;
+  double fn = 1.0, fm = 0.0;
+  for (int i = nterms; i <= nterms + 10000; ++i) {
+    const double fp = xi * fn * (2.0 * i) - fm;
+    if (fp * fp > 1e40) return i + 1;
+    fm = fn;
+    fn = fp;
+  }
+  return nterms + 10000;
+}
+
+template <int PG>
+__device__ void eval_lo_at_real(const Ex& E, double k, double rs, double r, double th,
+                                EvalAcc& A) {
+  const double x = k * r;
+  const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
+  if (x < 1e-200) {  // J_0 = 1, others 0
+    eval_term<PG>(E, 0, 1.0, mk(1, 0), mk(1, 0), A);
+    return;
+  }
+  const int top = miller_top_real(top_n, x);
+  const double zr = rs / x, r2 = rs * rs;
+  cd e1 = mk(cos(th), sin(th));
+  cd pos = mk(1, 0);
+  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  const cd ie1 = conj(e1);
+  EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double gp = 0.0, g = 1.0, H = 0.0;  // H = sum over even n >= 2 of rs^n g_n (Horner)
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= top_n) {
+      cd neg = (n & 1) ? -conj(pos) : conj(pos);
+      eval_term<PG>(E, n, g, pos, neg, B);
+      pos = pos * ie1;
+    }
+    H = rs * ((n & 1) ? H : g + H);
+    const double gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (g * g > 1e40) {
+      g *= 1e-40; gp *= 1e-40; H *= 1e-40;
+      B.P = B.P * 1e-40; B.Gx = B.Gx * 1e-40; B.Gy = B.Gy * 1e-40;
+      B.Hxx = B.Hxx * 1e-40; B.Hxy = B.Hxy * 1e-40; B.Hyy = B.Hyy * 1e-40;
+    }
+  }
+  eval_term<PG>(E, 0, g, mk(1, 0), mk(1, 0), B);
+  const double sc = 1.0 / (g + 2.0 * H);
+  A.P += B.P * sc; A.Gx += B.Gx * sc; A.Gy += B.Gy * sc;
+  A.Hxx += B.Hxx * sc; A.Hxy += B.Hxy * sc; A.Hyy += B.Hyy * sc;
+}
+
+// Four boxes per wave, 16 lanes per box, lane per point (leaves hold about
+// ndiv points); MPK: list-3 multipoles, else the box's own local expansion.
+// NR > 0: real positive wavenumber (for MPK also the series length of
+// hank_series_real, valid over the launch's k r); NR = 0: complex path.
+template <bool MPK, int PG, int NR>
 __global__ void __launch_bounds__(WAVE)
-k_heval(const int* work, DTree D, const HLev* lev, cd zk, const int2* rng, const double2* pts,
-        int N, int nd, size_t per, const double2* ex, double2* pot, double2* grad, double2* hess) {
-  const int b = work[blockIdx.x];
+k_heval(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int2* rng,
+        const double2* pts, int N, int nd, size_t per, const double2* ex, double2* pot,
+        double2* grad, double2* hess) {
+  const int w = blockIdx.x * 4 + ((int)threadIdx.x >> 4);
+  if (w >= nwork) return;
+  const int b = work[w];
   const int2 rg = rng[b];
   const cd i4 = mk(0, 0.25);
-  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += WAVE) {
+  for (int i = rg.x + ((int)threadIdx.x & 15); i < rg.y; i += 16) {
     double2 x = pts[i];
     const int ns = MPK ? D.o3[b + 1] - D.o3[b] : 1;
     for (int d = 0; d < nd; ++d) {
@@ -270,8 +355,13 @@ k_heval(const int* work, DTree D, const HLev* lev, cd zk, const int2* rng, const
         double th = (ddx == 0 && ddy == 0) ? 0.0 : atan2(ddy, ddx);
         double rsd = MPK ? lv.rs : 1.0 / lv.rs;
         Ex E{ex + (size_t)s * per + (size_t)d * (2 * lv.p + 1), lv.p, zk * (0.5 / rsd), zk * (0.5 * rsd)};
-        if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
-        else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+        if (NR > 0) {
+          if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, th, A);
+          else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, th, A);
+        } else {
+          if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
+          else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+        }
       }
       addc(pot + (size_t)d * N + i, A.P * i4);
       if (PG >= 2) {
@@ -517,13 +607,35 @@ void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& 
 #undef FMM_HP2P
 }
 
+// xmax: bound on k r over the launch (multipoles), 0 when zk is not real
+// and positive
 template <bool MPK>
-void launch_heval(int pg, int nbx, hipStream_t st, const int* work, const DTree& D,
+void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& D,
                   const HLev* lev, cd zk, const int2* rng, const double2* pts, int N, int nd,
-                  size_t per, const double2* ex, double2* pot, double2* grad, double2* hess) {
-  if (pg == 1) k_heval<MPK, 1><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
-  if (pg == 2) k_heval<MPK, 2><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
-  if (pg == 3) k_heval<MPK, 3><<<nbx, WAVE, 0, st>>>(work, D, lev, zk, rng, pts, N, nd, per, ex, pot, grad, hess);
+                  size_t per, const double2* ex, double xmax, double2* pot, double2* grad,
+                  double2* hess) {
+  const bool rz = zk.y == 0.0 && zk.x > 0.0;
+  int nr = 0;
+  if (rz) {
+    if (!MPK) nr = 1;
+    else nr = xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
+  }
+  const int nbx = (nw + 3) / 4;
+#define FMM_HEVAL(PGV, NRV)                                                                   \
+  k_heval<MPK, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, nw, D, lev, zk, rng, pts, N, nd, per, ex, \
+                                               pot, grad, hess)
+#define FMM_HEVAL_NR(PGV)                                    \
+  do {                                                       \
+    if (!MPK && nr) FMM_HEVAL(PGV, 1);                       \
+    else if (nr == JY_NTERMS0) FMM_HEVAL(PGV, JY_NTERMS0);   \
+    else if (nr == JY_NTERMS1) FMM_HEVAL(PGV, JY_NTERMS1);   \
+    else FMM_HEVAL(PGV, 0);                                  \
+  } while (0)
+  if (pg == 1) FMM_HEVAL_NR(1);
+  if (pg == 2) FMM_HEVAL_NR(2);
+  if (pg == 3) FMM_HEVAL_NR(3);
+#undef FMM_HEVAL_NR
+#undef FMM_HEVAL
 }
 
 }  // namespace
@@ -750,14 +862,23 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       if (plan.p2p_s.at(l).n || plan.p2p_t.at(l).n) lmin = l;
     if (lmin <= L) xmax = a.zk.real() * 4.25 * bs[lmin];
   }
+  // k r over list 3: a point of a leaf of size s to the centre of a child
+  // of a colleague is at most 1.75 s away per coordinate, r <= 2.48 s
+  double xmax3 = 0.0;
+  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) {
+    int lmin = L + 1;
+    for (int l = 0; l <= L && lmin > L; ++l)
+      if (plan.list3_s.at(l).n || plan.list3_t.at(l).n) lmin = l;
+    if (lmin <= L) xmax3 = a.zk.real() * 2.48 * bs[lmin];
+  }
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.l3.n)
-      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, s.po, s.gr, s.he);
+      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.lv.n)
-      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, s.po, s.gr, s.he);
+      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, 0.0, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
