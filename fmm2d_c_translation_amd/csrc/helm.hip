@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -36,10 +37,26 @@ struct HLev {   // per-level parameters on the device
 // formation: multipoles of leaves (scaled J) and locals from list 4
 // (scaled H); charge part helmrouts2d.c:43-64, dipole part :67-105
 // ---------------------------------------------------------------------
-template <bool MP>
+// Bessel tables of one point, kept out of line: their constants then stay
+// inside the call instead of being held in registers across the kernel
+template <int N>
+__device__ __noinline__ void hvals_real_point(int nmax, double x, double rs, cd* out, int stride) {
+  hvals_real<N>(nmax, x, rs, out, stride);
+}
+__device__ __noinline__ void jvals_real_point(int nmax, double x, double rs, double* out,
+                                              int stride) {
+  jvals_real(nmax, x, rs, out, stride);
+}
+
+// NR > 0: real positive wavenumber (for list 4 also the length of the
+// H0/H1 series, valid over the launch's k r): J tables are real (VT =
+// double) and every Bessel value comes from real arithmetic; NR = 0:
+// complex path.
+template <bool MP, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, const double2* chg,
         const double2* dip, const double* dv, int N, int nd, int CH, size_t per, double2* out) {
+  using VT = typename std::conditional<MP && (NR > 0), double, cd>::type;
   extern __shared__ double2 sm[];
   const int b = work[blockIdx.x];
   const int lane = threadIdx.x;
@@ -47,8 +64,8 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
   const int p = lev[L].p;
   const double rs = lev[L].rs;
   const int Q = p + 2;                   // v_0 .. v_{p+1} (J) / .. v_{p+2} (H)
-  cd* V = reinterpret_cast<cd*>(sm);     // [Q+1][CH]
-  cd* Z = V + (size_t)(Q + 1) * CH;      // [Q+1][CH] powers of zmul
+  cd* Z = reinterpret_cast<cd*>(sm);     // [Q+1][CH] powers of e^{-i theta}
+  VT* V = reinterpret_cast<VT*>(Z + (size_t)(Q + 1) * CH);  // [Q+1][CH]
   const double2 ctr = D.ctr[b];
   const double rsd = MP ? rs : 1.0 / rs;  // dipole scale (locals use 1/rs)
   const int nsb = MP ? 1 : D.o4[b + 1] - D.o4[b];
@@ -62,9 +79,15 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
         double dx = s.x - ctr.x, dy = s.y - ctr.y;
         double r = sqrt(dx * dx + dy * dy);
         double th = (dx == 0 && dy == 0) ? 0.0 : atan2(dy, dx);
-        cd z = zk * r;
-        if (MP) jvals(p + 1, z, rs, V + j, CH);
-        else hvals(p + 2, z, rs, V + j, CH);
+        if constexpr (NR > 0) {
+          const double x = zk.x * r;
+          if constexpr (MP) jvals_real_point(p + 1, x, rs, reinterpret_cast<double*>(V) + j, CH);
+          else hvals_real_point<(NR > 0 ? NR : 1)>(p + 2, x, rs, reinterpret_cast<cd*>(V) + j, CH);
+        } else {
+          cd z = zk * r;
+          if (MP) jvals(p + 1, z, rs, reinterpret_cast<cd*>(V) + j, CH);
+          else hvals(p + 2, z, rs, reinterpret_cast<cd*>(V) + j, CH);
+        }
         cd zm = cd{cos(th), -sin(th)}, pw = mk(1, 0);   // e^{-i theta}
         for (int n = 0; n <= Q; ++n) {
           Z[(size_t)n * CH + j] = pw;
@@ -79,13 +102,13 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
           for (int j = 0; j < cnt; ++j) {
             cd zn = Z[(size_t)n * CH + j];
             cd ph = m >= 0 ? zn : ((n & 1) ? -conj(zn) : conj(zn));  // (-zinv)^n
-            if (chg) acc += ld(chg + (size_t)d * N + c0 + j) * V[(size_t)n * CH + j] * ph;
+            if (chg) acc += ld(chg + (size_t)d * N + c0 + j) * (V[(size_t)n * CH + j] * ph);
             if (dip) {
               cd q = ld(dip + (size_t)d * N + c0 + j);
               double v1 = dv[(size_t)d * N + c0 + j], v2 = dv[(size_t)(nd + d) * N + c0 + j];
               cd zm = Z[(size_t)CH + j], zi = conj(zm);
               if (m == 0) {
-                acc += -(q * V[(size_t)CH + j] * zk * 0.5 * rsd) *
+                acc += -(q * (V[(size_t)CH + j] * (zk * (0.5 * rsd)))) *
                        ((zm + zi) * v1 + mul_i(zm - zi) * v2);
               } else if (m > 0) {
                 cd t7 = (zi / rsd) * mk(-v1, v2), t8 = (zm * rsd) * mk(v1, v2);
@@ -253,13 +276,25 @@ __device__ void eval_lo_at(const Ex& E, cd zk, double rs, double r, double th, E
 // launch guarantees x <= JY_XMAX for NR) and the upward recurrence with real
 // multipliers.  Local: J_n / rs^n by Miller's downward recurrence, normalised
 // with the Neumann sum J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.46).
+struct H01 {
+  cd h0, h1;
+};
+// out of line, so the series constants are not held in registers across the
+// caller's loops
+template <int N>
+__device__ __noinline__ H01 hank_real_point(double x) {
+  H01 h;
+  hank_series_real<N>(x, h.h0, h.h1);
+  return h;
+}
+
 template <int PG, int NR>
 __device__ void eval_mp_at_real(const Ex& E, double k, double rs, double r, double th,
                                 EvalAcc& A) {
   const double x = k * r;
   if (x < 1e-200) return;
-  cd h0, h1;
-  hank_series_real<NR>(x, h0, h1);
+  const H01 hh = hank_real_point<NR>(x);
+  const cd h0 = hh.h0, h1 = hh.h1;
   cd e1 = mk(cos(th), sin(th)), pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
   const double zi = rs / x, s2 = rs * rs;
   cd vm = h0, v = h1 * rs;
@@ -624,12 +659,16 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
 #define FMM_HEVAL(PGV, NRV)                                                                   \
   k_heval<MPK, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, nw, D, lev, zk, rng, pts, N, nd, per, ex, \
                                                pot, grad, hess)
-#define FMM_HEVAL_NR(PGV)                                    \
-  do {                                                       \
-    if (!MPK && nr) FMM_HEVAL(PGV, 1);                       \
-    else if (nr == JY_NTERMS0) FMM_HEVAL(PGV, JY_NTERMS0);   \
-    else if (nr == JY_NTERMS1) FMM_HEVAL(PGV, JY_NTERMS1);   \
-    else FMM_HEVAL(PGV, 0);                                  \
+#define FMM_HEVAL_NR(PGV)                                                \
+  do {                                                                   \
+    if constexpr (!MPK) {                                                \
+      if (nr) FMM_HEVAL(PGV, 1);                                         \
+      else FMM_HEVAL(PGV, 0);                                            \
+    } else {                                                             \
+      if (nr == JY_NTERMS0) FMM_HEVAL(PGV, JY_NTERMS0);                  \
+      else if (nr == JY_NTERMS1) FMM_HEVAL(PGV, JY_NTERMS1);             \
+      else FMM_HEVAL(PGV, 0);                                            \
+    }                                                                    \
   } while (0)
   if (pg == 1) FMM_HEVAL_NR(1);
   if (pg == 2) FMM_HEVAL_NR(2);
@@ -776,10 +815,13 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // addressed through a per-level array of 49 offsets
   const int* W = plan.dwork;
   const double* dv = a.dipvec;
-  int CH = 64;
-  auto shm_of = [&](int p) { return (size_t)2 * (p + 3) * CH * sizeof(double2); };
-  while (CH > 8 && shm_of(pmax) > 48 * 1024) CH /= 2;
-  size_t shm = shm_of(pmax);
+  // formation: points staged CH at a time, Z (complex) and V (J real for a
+  // real wavenumber, else complex) for orders 0..p+2; 16 points keep the
+  // footprint near 12-16 KB so several waves share a CU
+  const int CH = 16;
+  const bool rz = zk.y == 0.0 && zk.x > 0.0;
+  const size_t shm = (size_t)(pmax + 3) * CH * (sizeof(double2) + sizeof(double2));
+  const size_t shm_mp = (size_t)(pmax + 3) * CH * (sizeof(double2) + (rz ? sizeof(double) : sizeof(double2)));
 
   // levels with gate[l] false (complex zk, boxes far above the decay length)
   // form no expansions; gate[l] only switches on as boxes shrink, so the
@@ -793,6 +835,16 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   const Span s_lvs = plan.leaves_s.from(lg), s_lvt = plan.leaves_t.from(lg);
   auto wptr = [&](const Span& sp) { return W + sp.off; };
 
+  // k r over list 4: a source of a coarse leaf to the centre of a target of
+  // size s is at most 3.5 s away per coordinate, r <= 4.95 s
+  int nr4 = 0;
+  if (rz) {
+    int lmin = L + 1;
+    for (int l = 0; l <= L && lmin > L; ++l)
+      if (plan.list4.at(l).n) lmin = l;
+    const double xmax4 = lmin <= L ? zk.x * 4.95 * bs[lmin] : 0.0;
+    nr4 = xmax4 <= JY_XMAX0 ? JY_NTERMS0 : xmax4 <= JY_XMAX1 ? JY_NTERMS1 : 0;
+  }
   StepTimer* tm = step_timer();
   {
     SolveStats& st = last_stats();
@@ -809,7 +861,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   // step 1: leaf multipoles
   if (s_form.n)
-    k_hform<true><<<s_form.n, WAVE, shm, c.st>>>(wptr(s_form), D, dlev, zk, Pt.src, chg, dip, dv,
+    (rz ? k_hform<true, 1> : k_hform<true, 0>)<<<s_form.n, WAVE, shm_mp, c.st>>>(wptr(s_form), D, dlev, zk, Pt.src, chg, dip, dv,
                                                 Pt.ns, nd, CH, per, mp);
   tm->mark(c.st);
   // step 3: M2M
@@ -828,7 +880,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   // step 2: list 4
   if (s_l4.n)
-    k_hform<false><<<s_l4.n, WAVE, shm, c.st>>>(wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv,
+    (nr4 == JY_NTERMS0 ? k_hform<false, JY_NTERMS0> : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1> : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv,
                                                Pt.ns, nd, CH, per, lo);
   tm->mark(c.st);
   // step 4: M2L (49 slot matrices per level, contiguous)

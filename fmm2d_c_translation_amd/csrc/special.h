@@ -217,6 +217,65 @@ FMM_HD void jvals(int nmax, cd z, double rs, cd* out, int stride) {
   for (int n = 1; n <= nmax && n <= top - 1; ++n) out[n * stride] = out[n * stride] * sc;
 }
 
+// J_n(x)/rs^n, n = 0..nmax, of a real argument x > 0: Miller's downward
+// recurrence as in jvals, in real arithmetic, normalised with the Neumann
+// sum J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.46).
+FMM_HD void jvals_real(int nmax, double x, double rs, double* out, int stride) {
+  if (x < 1e-200) {
+    out[0] = 1.0;
+    for (int n = 1; n <= nmax; ++n) out[n * stride] = 0.0;
+    return;
+  }
+  const double xi = 1.0 / x;
+  int top = nmax + 10000;
+  {
+    double fn = 1.0, fm = 0.0;
+    for (int i = nmax; i <= nmax + 10000; ++i) {
+      const double fp = xi * fn * (2.0 * i) - fm;
+      if (fp * fp > 1e40) {
+        top = i + 1;
+        break;
+      }
+      fm = fn;
+      fn = fp;
+    }
+  }
+  const double zr = xi * rs, r2 = rs * rs;
+  double gp = 0.0, g = 1.0, H = 0.0;  // H = sum over even n >= 2 of rs^n g_n (Horner)
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= nmax) out[n * stride] = g;
+    H = rs * ((n & 1) ? H : g + H);
+    const double gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (g * g > 1e40) {
+      g *= 1e-40; gp *= 1e-40; H *= 1e-40;
+      for (int k = n; k <= nmax && k <= top - 1; ++k) out[k * stride] *= 1e-40;
+    }
+  }
+  for (int n = top; n <= nmax; ++n) out[n * stride] = 0.0;
+  const double sc = 1.0 / (g + 2.0 * H);
+  out[0] = g * sc;
+  for (int n = 1; n <= nmax && n <= top - 1; ++n) out[n * stride] *= sc;
+}
+
+// Scaled H_n(x) rs^n of a real argument 0 < x <= JY_XMAX for N: H0, H1 from
+// hank_series_real<N>, then the upward recurrence with real multipliers.
+template <int N>
+FMM_HD void hvals_real(int nmax, double x, double rs, cd* out, int stride) {
+  if (x < 1e-200) {
+    for (int n = 0; n <= nmax; ++n) out[n * stride] = mk(0, 0);
+    return;
+  }
+  cd h0, h1;
+  hank_series_real<N>(x, h0, h1);
+  out[0] = h0;
+  if (nmax >= 1) out[stride] = h1 * rs;
+  const double zi = rs / x, s2 = rs * rs;
+  for (int i = 1; i <= nmax - 1; ++i)
+    out[(i + 1) * stride] = out[i * stride] * (zi * (2.0 * i)) - out[(i - 1) * stride] * s2;
+}
+
 // Scaled H_n(z) rs^n, n = 0..nmax (h2dcommon.c:35-90)
 FMM_HD void hvals(int nmax, cd z, double rs, cd* out, int stride) {
   if (cabs_(z) < 1e-200) {
