@@ -421,14 +421,21 @@ k_wl_fill(WlArgs a, const int* boff, int nblocks, const int* info, int* out) {
   }
 }
 
-// children of the boxes of the L2L list, four per box, after all lists
-__global__ void k_wl_kids(const int* info, const int4* child, int* out) {
+// children of the boxes of the L2L list, after all lists: for the parents
+// of level l (entries lev[l] .. lev[l+1] of the L2L list) the children take
+// positions 4 lev[l] .. 4 lev[l+1], grouped by quadrant, so the targets of
+// a translation tile share their one operator
+__global__ void k_wl_kids(const int* info, const int* lev, int W, const int4* child, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int n = info[WL_N + 1 + WL_L2L];
   if (i >= n) return;
+  const int* lv = lev + WL_L2L * W;
+  int l = 0;
+  while (l + 1 < W && lv[l + 1] <= i) ++l;
+  const int a = lv[l], nl = lv[l + 1] - a;
   int4 c = child[out[info[WL_L2L] + i]];
-  int* k = out + info[WL_N] + 4 * i;
-  k[0] = c.x; k[1] = c.y; k[2] = c.z; k[3] = c.w;
+  int* k = out + info[WL_N] + 4 * a + (i - a);
+  k[0] = c.x; k[nl] = c.y; k[2 * nl] = c.z; k[3 * nl] = c.w;
 }
 
 // lev[j*(L+2) + l] = first entry of list j whose box is at level >= l
@@ -616,8 +623,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   int* work = S.get<int>((size_t)WL_N * nb + 4 * (size_t)nb + 1);
   if (S.err) return S.err;
   k_wl_fill<<<nbk, WLB, 0, c.st>>>(wa, bcnt, nbk, info, work);
-  k_wl_kids<<<nblk(nb, 256), 256, 0, c.st>>>(info, child, work);
   k_wl_levels<<<nblk(WL_N * W, 64), 64, 0, c.st>>>(lv, info, work, lev);
+  k_wl_kids<<<nblk(nb, 256), 256, 0, c.st>>>(info, lev, W, child, work);
   HIP_TRY(hipGetLastError());
   std::vector<int> hs(nsmall + 2);
   e = sync_read(c, small, nsmall + 2, hs.data());

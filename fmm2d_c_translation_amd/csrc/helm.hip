@@ -874,7 +874,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2m.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
-    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
+    HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
                              2 * lev[l].p + 1, mp, per, mp, per, nd, 0.0));
   }
   tm->mark(c.st);
@@ -897,7 +897,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.l2l_kids.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
-    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
+    HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 1, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
                              2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
   }
   // steps 6-8 at sources and targets

@@ -467,7 +467,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     const Span sp = plan.m2m.at(l);
     if (!sp.n) continue;
     k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
-    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, T2, P, P, mp, per, mp, per, nd, 0.0));
+    HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 4, T2, P, P, mp, per, mp, per, nd, 0.0));
   }
   tm->mark(c.st);
   // step 2: list 4
@@ -489,7 +489,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     const Span sp = plan.l2l_kids.at(l);
     if (!sp.n) continue;
     k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
-    HIP_TRY(launch_translate(c.st, W + sp.off, sp.n, map, 4, TL, P, P, lo, per, lo, per, nd, 0.0));
+    HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 1, TL, P, P, lo, per, lo, per, nd, 0.0));
   }
   // steps 6 + 7: list 3 and local expansions at sources / targets
   struct Set { int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };

@@ -174,21 +174,22 @@ k_translate_direct(const int* __restrict__ work, const int* __restrict__ map, in
 }
 
 // ---------------------------------------------------------------------
-// M2L on the FP64 matrix cores.  For one level, Y[work[w]] += sum_s T_s
-// X[map[w][s]] with T_s the P x P complex operator of list-2 offset s.
-// Complex products as four real 16x16x4 MFMAs:
-//   Yr += Tr Xr + Ti (-Xi),  Yi += Tr Xi + Ti Xr.
+// Translations on the FP64 matrix cores.  For one level,
+//   Y[work[w]] += sum_s T_s X[map[w][s]],
+// T_s the nin x nout complex operator of slot s (list-2 offset or child
+// quadrant), stored T[s][j][i] (j input, i output).  Complex products as
+// four real 16x16x4 MFMAs: Yr += Tr Xr + Ti (-Xi), Yi += Tr Xi + Ti Xr.
 // A wave owns 16 work items (the N columns) and MW row tiles of 16 outputs;
 // lane l holds A[l%16][l/16] = T_s[k0 + l/16][16t + l%16] and
 // B[l/16][l%16] = X[k0 + l/16] of column l%16, and accumulates
 // D[4v + l/16][l%16] (v = 0..3) -- the v_mfma_f64_16x16x4f64 operand layout.
 // A slot is skipped when none of the wave's 16 columns has a source there,
-// so tiles of targets that share a child quadrant (same 27 offsets) waste
-// no work.  Grid: x = column tiles (4 per block), y = row-tile groups,
-// z = slot groups (slots s = z, z + G, ...).  With G > 1 each slot group
-// writes its partial sums to part[(g nd + d) nwork + w][i] and
-// k_m2l_reduce adds them up in a fixed order; with G = 1 the wave adds into
-// Y directly.  lap_log as in k_translate.
+// so tiles of work items that share a child quadrant waste no work.  Grid:
+// x = column tiles (4 per block), y = row-tile groups, z = slot groups
+// (slots s = z, z + G, ...).  With G > 1 each slot group writes its partial
+// sums to part[(g nd + d) nwork + w][i] and k_xlate_reduce adds them up in
+// a fixed order; with G = 1 the wave adds into Y directly.  lap_log as in
+// k_translate (nin == nout).
 // ---------------------------------------------------------------------
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
@@ -199,10 +200,10 @@ inline hipError_t launch_translate(hipStream_t st, const int* work, int nwork, c
 
 template <int MW>
 __global__ void __launch_bounds__(256)
-k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
-           const double2* __restrict__ T, int P, const double2* __restrict__ X, size_t xstride,
-           double2* __restrict__ Y, size_t ystride, int nd, double lap_log, int G,
-           double2* __restrict__ part) {
+k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
+             const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+             size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
+             int G, double2* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const int n = lane & 15, kq = lane >> 4;
   const int w = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + n;
@@ -216,15 +217,15 @@ k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
       ci[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
     }
     double s0r = 0.0, s0i = 0.0;  // sum of x_0 over the sources (lanes kq == 0)
-    for (int s = g; s < 49; s += G) {
-      const int src = wok ? map[(size_t)w * 49 + s] : -1;
+    for (int s = g; s < nslot; s += G) {
+      const int src = wok ? map[(size_t)w * nslot + s] : -1;
       if (!__any(src >= 0)) continue;
-      const double2* xs = X + (size_t)(src >= 0 ? src : 0) * xstride + (size_t)d * P;
-      const double2* Ts = T + (size_t)s * P * P;
+      const double2* xs = X + (size_t)(src >= 0 ? src : 0) * xstride + (size_t)d * nin;
+      const double2* Ts = T + (size_t)s * nin * nout;
 #pragma unroll 2
-      for (int k0 = 0; k0 < P; k0 += 4) {
+      for (int k0 = 0; k0 < nin; k0 += 4) {
         const int j = k0 + kq;
-        const bool jok = j < P;
+        const bool jok = j < nin;
         const double2 x = (src >= 0 && jok) ? xs[j] : make_double2(0.0, 0.0);
         if (k0 == 0) {
           s0r += x.x;
@@ -235,7 +236,7 @@ k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
           const int i = 16 * (rt0 + t) + n;
-          const double2 a = (jok && i < P) ? Ts[(size_t)j * P + i] : make_double2(0.0, 0.0);
+          const double2 a = (jok && i < nout) ? Ts[(size_t)j * nout + i] : make_double2(0.0, 0.0);
           ar[t] = a.x;
           ai[t] = a.y;
         }
@@ -252,14 +253,14 @@ k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
       }
     }
     if (!wok) continue;
-    double2* y = G == 1 ? Y + (size_t)work[w] * ystride + (size_t)d * P
-                        : part + ((size_t)(g * nd + d) * nwork + w) * P;
+    double2* y = G == 1 ? Y + (size_t)work[w] * ystride + (size_t)d * nout
+                        : part + ((size_t)(g * nd + d) * nwork + w) * nout;
 #pragma unroll
     for (int t = 0; t < MW; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = 16 * (rt0 + t) + 4 * v + kq;
-        if (i >= P) continue;
+        if (i >= nout) continue;
         double re = cr[t][v], im = ci[t][v];
         if (i == 0 && lap_log != 0.0) {
           re += s0r * lap_log;
@@ -272,64 +273,76 @@ k_m2l_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map,
 }
 
 // Y[work[w]] += sum over slot groups of the partials, group 0 first
-__global__ void k_m2l_reduce(const int* __restrict__ work, int nwork, int P, int nd, int G,
-                             const double2* __restrict__ part, double2* __restrict__ Y,
-                             size_t ystride) {
+__global__ void k_xlate_reduce(const int* __restrict__ work, int nwork, int nout, int nd, int G,
+                               const double2* __restrict__ part, double2* __restrict__ Y,
+                               size_t ystride) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per = (size_t)nwork * P;
+  const size_t per = (size_t)nwork * nout;
   if (e >= per * nd) return;
   const int d = (int)(e / per);
   const size_t r = e - (size_t)d * per;
-  const int w = (int)(r / P), i = (int)(r - (size_t)w * P);
+  const int w = (int)(r / nout), i = (int)(r - (size_t)w * nout);
   double re = 0.0, im = 0.0;
   for (int g = 0; g < G; ++g) {
-    const double2 v = part[((size_t)(g * nd + d) * nwork + w) * P + i];
+    const double2 v = part[((size_t)(g * nd + d) * nwork + w) * nout + i];
     re += v.x;
     im += v.y;
   }
-  double2* y = Y + (size_t)work[w] * ystride + (size_t)d * P + i;
+  double2* y = Y + (size_t)work[w] * ystride + (size_t)d * nout + i;
   *y = make_double2(y->x + re, y->y + im);
 }
 
-// M2L through k_m2l_mfma when the order fits (P <= 64), else k_translate.
-// The split of rows and slots over waves keeps about 4096 waves in flight
-// whatever the level's size; S provides the partial-sum buffer.
-inline hipError_t launch_m2l(hipStream_t st, Scratch& S, const int* work, int nwork,
-                             const int* map, const double2* T, int P, const double2* X, size_t xs,
-                             double2* Y, size_t ys, int nd, double lap_log) {
+// Translation through k_xlate_mfma when the output order fits (nout <= 64),
+// else k_translate.  Rows and slots are split over waves to keep about 4096
+// waves in flight whatever the level's size; maxg bounds the slot groups
+// (the number of slots a work item can use: 27 for M2L, 4 for M2M, 1 for
+// L2L).  S provides the partial-sum buffer.
+inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int nwork,
+                               const int* map, int nslot, int maxg, const double2* T, int nin,
+                               int nout, const double2* X, size_t xs, double2* Y, size_t ys,
+                               int nd, double lap_log) {
   if (nwork <= 0) return hipSuccess;
-  const int MT = (P + 15) / 16;
-  if (MT > 4) return launch_translate(st, work, nwork, map, 49, T, P, P, X, xs, Y, ys, nd, lap_log);
+  const int MT = (nout + 15) / 16;
+  if (MT > 4)
+    return launch_translate(st, work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, nd, lap_log);
   const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
   constexpr int kWaves = 4096;
   int mw = MT, G = 1;
-  if (ntile * MT / 1 < kWaves) {  // small level: one row tile per wave ...
+  if (ntile * MT < kWaves) {  // small level: one row tile per wave and split the slots
     mw = 1;
     const int per_g = ntile * MT;
-    G = std::min(27, std::max(1, (kWaves + per_g - 1) / per_g));  // ... and split the slots
+    G = std::min(std::min(maxg, nslot), std::max(1, (kWaves + per_g - 1) / per_g));
   } else if (ntile < kWaves) {
     mw = MT >= 2 ? 2 : 1;
   }
   double2* part = nullptr;
   if (G > 1) {
-    part = S.get<double2>((size_t)G * nd * nwork * P);
+    part = S.get<double2>((size_t)G * nd * nwork * nout);
     if (!part) return hipErrorOutOfMemory;
   }
   const dim3 grid(nb, (MT + mw - 1) / mw, G);
-#define FMM_M2L(MWV) \
-  k_m2l_mfma<MWV><<<grid, 256, 0, st>>>(work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log, G, part)
+#define FMM_XL(MWV)                                                                       \
+  k_xlate_mfma<MWV><<<grid, 256, 0, st>>>(work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, \
+                                          nd, lap_log, G, part)
   switch (mw) {
-    case 1: FMM_M2L(1); break;
-    case 2: FMM_M2L(2); break;
-    case 3: FMM_M2L(3); break;
-    default: FMM_M2L(4); break;
+    case 1: FMM_XL(1); break;
+    case 2: FMM_XL(2); break;
+    case 3: FMM_XL(3); break;
+    default: FMM_XL(4); break;
   }
-#undef FMM_M2L
+#undef FMM_XL
   if (G > 1) {
-    const size_t ne = (size_t)nd * nwork * P;
-    k_m2l_reduce<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(work, nwork, P, nd, G, part, Y, ys);
+    const size_t ne = (size_t)nd * nwork * nout;
+    k_xlate_reduce<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(work, nwork, nout, nd, G, part, Y,
+                                                                ys);
   }
   return hipGetLastError();
+}
+
+inline hipError_t launch_m2l(hipStream_t st, Scratch& S, const int* work, int nwork,
+                             const int* map, const double2* T, int P, const double2* X, size_t xs,
+                             double2* Y, size_t ys, int nd, double lap_log) {
+  return launch_xlate(st, S, work, nwork, map, 49, 27, T, P, P, X, xs, Y, ys, nd, lap_log);
 }
 
 constexpr int XT_TB = 16;
