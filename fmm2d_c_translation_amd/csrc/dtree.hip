@@ -120,6 +120,104 @@ __global__ void k_balance_cand(const uint64_t* R, int n, int k, uint64_t* out) {
   }
 }
 
+// Deepest split level: points i .. i+ndiv of the sorted keys share the
+// level-c box, c = the number of leading key digits the two ends agree on,
+// so that box holds more than ndiv points and is split; no deeper box is.
+__global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + ndiv >= n) return;
+  const uint64_t x = k[i] ^ k[i + ndiv];
+  const int c = x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
+  atomicMax(depth, c);
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// split flags of the cnt (on the device) boxes of one level
+__global__ void k_split_flags(const uint64_t* K, const int* cnt, int cap, SplitPred pred,
+                              int* flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  flag[i] = (i < *cnt && pred(K[i])) ? 1 : 0;
+}
+
+// split keys (sorted) and the next level's boxes; nR[l] and cnt[l+1] from
+// the last box; a split count above capR sets *overflow
+__global__ void k_split_scatter(const uint64_t* K, const int* cnt, const int* flag,
+                                const int* pos, int capR, uint64_t* R, uint64_t* Knext,
+                                int* nR, int* cnt_next, int* overflow) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *cnt;
+  if (i >= n) return;
+  const int p = pos[i];
+  if (flag[i] && p < capR) {
+    const uint64_t key = K[i];
+    R[p] = key;
+    for (int q = 0; q < 4; ++q) Knext[4 * p + q] = 4 * key + (uint64_t)q;
+  }
+  if (i == n - 1) {
+    const int nr = p + flag[i];
+    *nR = min(nr, capR);
+    *cnt_next = 4 * min(nr, capR);
+    if (nr > capR) *overflow = 1;
+  }
+}
+
+// Level-restriction closure in one pass.  A split box r of level k forces
+// the parents of its in-domain neighbours to be split at level k-1, whose
+// neighbours' parents must then be split at k-2, and so on: at each coarser
+// level the forced boxes form a rectangle of at most 3 x 3 cells.  Written
+// as (level << 58 | key), 9 slots per level below k (short rectangles repeat
+// their first cell); the adaptive split boxes themselves come first.
+constexpr int kLevShift = 2 * kMaxDepth;
+__global__ void k_closure_cand(const uint64_t* R, const int* roff, const long* cbase, int L,
+                               uint64_t* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= roff[L + 1]) return;
+  int k = 0;
+  while (k < L && r >= roff[k + 1]) ++k;
+  const uint64_t key = R[r];
+  out[r] = ((uint64_t)k << kLevShift) | key;
+  if (k < 2) return;
+  uint64_t* o = out + cbase[k] + (long)(r - roff[k]) * 9 * (k - 1);
+  int ix, iy;
+  key_cell(key, ix, iy);
+  int side = 1 << k;
+  int ax = max(ix - 1, 0), bx = min(ix + 1, side - 1);
+  int ay = max(iy - 1, 0), by = min(iy + 1, side - 1);
+  for (int kk = k - 1; kk >= 1; --kk) {
+    ax >>= 1; bx >>= 1; ay >>= 1; by >>= 1;
+    const uint64_t lev = (uint64_t)kk << kLevShift;
+    int m = 0;
+    for (int y = ay; y <= by && y <= ay + 2; ++y)
+      for (int x = ax; x <= bx && x <= ax + 2; ++x) o[m++] = lev | cell_key(x, y);
+    for (; m < 9; ++m) o[m] = lev | cell_key(ax, ay);
+    o += 9;
+    side = 1 << kk;
+    ax = max(ax - 1, 0); bx = min(bx + 1, side - 1);
+    ay = max(ay - 1, 0); by = min(by + 1, side - 1);
+  }
+}
+
+// per-level offsets of the sorted (level, key) split set, then the keys
+// without their level bits
+__global__ void k_split_levels(const uint64_t* U, const int* n, int L, int* off) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > L + 1) return;
+  const uint64_t v = (uint64_t)l << kLevShift;
+  int lo = 0, hi = *n;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (U[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  off[l] = lo;
+}
+__global__ void k_strip_level(uint64_t* U, const int* n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < *n) U[i] &= (uint64_t(1) << kLevShift) - 1;
+}
+
 struct Levels {
   int L;
   int loff[kMaxDepth + 2];  // first box id of level l; loff[L+1] = nb
@@ -490,58 +588,115 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                 double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
                 int ifpghtarg, Plan& plan, double* host_s) {
   double t0 = wall();
-  // ---- 1. adaptive refinement, one level per pass
-  std::vector<uint64_t*> R;
-  std::vector<int> nR;
+  // ---- 0. depth of the adaptive tree (one read-back)
+  int* small0 = S.get<int>(2 * (kMaxDepth + 2) + 2);
+  if (S.err) return S.err;
+  int* d_depth = small0;                    // [0]: deepest split level, -1 none
+  int* d_over = small0 + 1;                 // [1]: overflow flag
+  int* d_cnt = small0 + 2;                  // [2 ..]: boxes per level
+  int* d_nR = small0 + 2 + (kMaxDepth + 2); // split boxes per level
+  HIP_TRY(hipMemsetAsync(small0, 0, sizeof(int) * (2 * (kMaxDepth + 2) + 2), c.st));
+  HIP_TRY(hipMemsetAsync(d_depth, 0xff, sizeof(int), c.st));
+  k_set_int<<<1, 1, 0, c.st>>>(d_cnt, 1);
+  if (idivflag != 1 && ns > ndiv)
+    k_depth<<<nblk(ns - ndiv, 256), 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
+  if ((idivflag == 1 || idivflag == 2) && nt > ndiv)
+    k_depth<<<nblk(nt - ndiv, 256), 256, 0, c.st>>>(kt, nt, ndiv, d_depth);
+  int Ls = -1;
+  int e = sync_read(c, d_depth, 1, &Ls);
+  if (e) return e;
+  if (idivflag < 0 || idivflag > 2) Ls = -1;
+  if (Ls >= kMaxDepth) return IER_DEPTH;
+  const int L = Ls + 1;
+
+  // ---- 1. adaptive refinement, level by level, counts kept on the device
+  const long npe = idivflag == 0 ? ns / (ndiv + 1)
+                 : idivflag == 1 ? nt / (ndiv + 1)
+                                 : ns / (ndiv + 1) + nt / (ndiv + 1);
+  std::vector<uint64_t*> R(L + 1, nullptr);
+  std::vector<int> capR(L + 1, 0);
   uint64_t* K = S.get<uint64_t>(1);
-  int* d_n = S.get<int>(1);
   if (S.err) return S.err;
   HIP_TRY(hipMemsetAsync(K, 0, sizeof(uint64_t), c.st));
-  int nK = 1, L = 0;
-  for (int l = 0;; ++l) {
-    uint64_t* Rl = S.get<uint64_t>(nK);
+  long capK = 1;
+  for (int l = 0; l <= Ls; ++l) {
+    capR[l] = (int)std::max<long>(1, std::min<long>(capK, npe));
+    int* flag = S.get<int>(capK);
+    int* pos = S.get<int>(capK);
+    R[l] = S.get<uint64_t>(capR[l]);
+    uint64_t* Kn = S.get<uint64_t>(4L * capR[l]);
     if (S.err) return S.err;
     SplitPred pred{ks, kt, ns, nt, l, idivflag, ndiv};
+    k_split_flags<<<nblk(capK, 256), 256, 0, c.st>>>(K, d_cnt + l, (int)capK, pred, flag);
     size_t tb = 0;
-    HIP_TRY(hipcub::DeviceSelect::If(nullptr, tb, K, Rl, d_n, nK, pred, c.st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, pos, (int)capK, c.st));
     void* tmp = S.get<char>(tb);
     if (S.err) return S.err;
-    HIP_TRY(hipcub::DeviceSelect::If(tmp, tb, K, Rl, d_n, nK, pred, c.st));
-    int nr = 0;
-    int e = sync_read(c, d_n, 1, &nr);
-    if (e) return e;
-    R.push_back(Rl);
-    nR.push_back(nr);
-    if (nr == 0) {
-      L = l;
-      break;
-    }
-    if (l + 1 > kMaxDepth) return IER_DEPTH;
-    K = S.get<uint64_t>((size_t)4 * nr);
-    if (S.err) return S.err;
-    k_children<<<nblk(4L * nr, 256), 256, 0, c.st>>>(Rl, nr, K);
-    nK = 4 * nr;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)capK, c.st));
+    k_split_scatter<<<nblk(capK, 256), 256, 0, c.st>>>(K, d_cnt + l, flag, pos, capR[l], R[l], Kn,
+                                                      d_nR + l, d_cnt + l + 1, d_over);
+    K = Kn;
+    capK = 4L * capR[l];
   }
+  std::vector<int> hsm(2 * (kMaxDepth + 2) + 2);
+  e = sync_read(c, small0, (int)hsm.size(), hsm.data());
+  if (e) return e;
+  if (hsm[1]) return IER_ARGS;  // more split boxes than points allow: cannot happen
+  std::vector<int> nR(L + 1, 0);
+  for (int l = 0; l <= Ls; ++l) nR[l] = hsm[2 + (kMaxDepth + 2) + l];
+  if (Ls >= 0 && nR[Ls] == 0) return IER_ARGS;  // the depth and the refinement disagree
   double t1 = wall();
-  // ---- 2. level restriction: closure from the finest split level upward
-  for (int k = L - 1; k >= 2; --k) {
-    size_t nc = (size_t)nR[k - 1] + 9 * (size_t)nR[k];
-    uint64_t* cand = S.get<uint64_t>(nc);
-    uint64_t* sorted = S.get<uint64_t>(nc);
-    uint64_t* Rk1 = S.get<uint64_t>(nc);
+
+  // ---- 2. level restriction: closure of the split sets in one sort
+  uint64_t* Rall = nullptr;
+  {
+    std::vector<int> roff(L + 2, 0);
+    std::vector<long> cbase(L + 1, 0);
+    long nc = 0;
+    for (int l = 0; l <= L; ++l) roff[l + 1] = roff[l] + nR[l];
+    nc = roff[L + 1];
+    for (int l = 0; l <= L; ++l) {
+      cbase[l] = nc;
+      if (l >= 2) nc += 9L * (l - 1) * nR[l];
+    }
+    int* d_roff = S.get<int>(L + 2);
+    long* d_cbase = S.get<long>(L + 1);
+    uint64_t* cand = S.get<uint64_t>(std::max<long>(nc, 1));
+    uint64_t* sorted = S.get<uint64_t>(std::max<long>(nc, 1));
+    Rall = S.get<uint64_t>(std::max<long>(nc, 1));
+    int* d_nu = S.get<int>(1);
+    int* d_loff = S.get<int>(L + 2);
     if (S.err) return S.err;
-    HIP_TRY(hipMemcpyAsync(cand, R[k - 1], sizeof(uint64_t) * nR[k - 1], hipMemcpyDeviceToDevice, c.st));
-    k_balance_cand<<<nblk(nR[k], 256), 256, 0, c.st>>>(R[k], nR[k], k, cand + nR[k - 1]);
+    int* hro = static_cast<int*>(pinned(sizeof(int) * (L + 2) + sizeof(long) * (L + 1) + 16, 5));
+    if (!hro) return IER_ALLOC;
+    std::memcpy(hro, roff.data(), sizeof(int) * (L + 2));
+    long* hcb = reinterpret_cast<long*>(reinterpret_cast<char*>(hro) + ((sizeof(int) * (L + 2) + 7) & ~size_t(7)));
+    std::memcpy(hcb, cbase.data(), sizeof(long) * (L + 1));
+    HIP_TRY(hipMemcpyAsync(d_roff, hro, sizeof(int) * (L + 2), hipMemcpyHostToDevice, c.st));
+    HIP_TRY(hipMemcpyAsync(d_cbase, hcb, sizeof(long) * (L + 1), hipMemcpyHostToDevice, c.st));
+    for (int l = 0; l < L; ++l)
+      HIP_TRY(hipMemcpyAsync(cand + roff[l], R[l], sizeof(uint64_t) * nR[l], hipMemcpyDeviceToDevice, c.st));
+    // cand[0 .. roff[L]) holds the split keys without level bits; the kernel
+    // rewrites them with their level and appends the forced boxes
+    if (roff[L] > 0)
+      k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(cand, d_roff, d_cbase, L, cand);
     size_t tb1 = 0, tb2 = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, 2 * (k - 1), c.st));
-    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rk1, d_n, (int)nc, c.st));
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, 64, c.st));
+    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     void* tmp = S.get<char>(std::max(tb1, tb2));
     if (S.err) return S.err;
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, 2 * (k - 1), c.st));
-    HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rk1, d_n, (int)nc, c.st));
-    int e = sync_read(c, d_n, 1, &nR[k - 1]);
+    if (nc > 0) {
+      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, 64, c.st));
+      HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rall, d_nu, (int)nc, c.st));
+    } else {
+      HIP_TRY(hipMemsetAsync(d_nu, 0, sizeof(int), c.st));
+    }
+    k_split_levels<<<1, 64, 0, c.st>>>(Rall, d_nu, L, d_loff);
+    k_strip_level<<<nblk(std::max<long>(nc, 1), 256), 256, 0, c.st>>>(Rall, d_nu);
+    std::vector<int> lo(L + 2);
+    e = sync_read(c, d_loff, L + 2, lo.data());
     if (e) return e;
-    R[k - 1] = Rk1;
+    for (int l = 0; l <= L; ++l) nR[l] = lo[l + 1] - lo[l];
   }
   double t2 = wall();
   // ---- 3. boxes
@@ -559,10 +714,6 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     lv.roff[l + 1] = lv.roff[l] + nR[l];
   }
   const int nb = lv.loff[L + 1];
-  uint64_t* Rall = S.get<uint64_t>(std::max(lv.roff[L + 1], 1));
-  if (S.err) return S.err;
-  for (int l = 0; l < L; ++l)
-    HIP_TRY(hipMemcpyAsync(Rall + lv.roff[l], R[l], sizeof(uint64_t) * nR[l], hipMemcpyDeviceToDevice, c.st));
   uint64_t* key = S.get<uint64_t>(nb);
   int* level = S.get<int>(nb);
   double2* ctr = S.get<double2>(nb);
@@ -592,7 +743,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   HIP_TRY(hipcub::DeviceScan::ExclusiveScan(tmp, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
   int tot[4];
   HIP_TRY(hipMemcpyAsync(tot4, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
-  int e = sync_read(c, tot4, 4, tot);
+  e = sync_read(c, tot4, 4, tot);
   if (e) return e;
   lo.l1 = S.get<int>(std::max(tot[0], 1));
   lo.l2 = S.get<int>(std::max(tot[1], 1));
