@@ -455,6 +455,7 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
   const int2 tr = trng[b];
   const cd ik4 = zk * mk(0, 0.25);
   const bool cp_mode = C && !Q && PG == 1;
+  const double izk = NR > 0 ? 1.0 / zk.x : 0.0;
   for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
     const int Tc = min(WAVE, tr.y - t0);
     const int K = WAVE / Tc;
@@ -480,20 +481,26 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
         if (act) {
           for (int m = k; m < cnt; m += K) {
             double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
-            double rr = dx * dx + dy * dy, r = sqrt(rr);
+            double rr = dx * dx + dy * dy, r, ri;
             cd z, h0, h1;
             double az;
             if (NR > 0) {
+              // r and 1/r from one reciprocal square root (r = 0 for a
+              // coincident pair, which the threshold then skips)
+              ri = rr > 0.0 ? rsqrt(rr) : 0.0;
+              r = rr * ri;
               az = zk.x * r;
               z = cd{az, 0.0};
             } else {
+              r = sqrt(rr);
+              ri = 1.0 / r;
               z = zk * r;
               az = cabs_(z);
             }
             if (cp_mode ? az <= thresh : az < thresh) continue;
-            if (NR > 0) hank_series_real<(NR > 0 ? NR : 1)>(az, h0, h1);
+            const double azi = ri * izk;  // 1/(k r) for a real wavenumber
+            if (NR > 0) hank_series_real<(NR > 0 ? NR : 1)>(az, azi, h0, h1);
             else hank01(z, h0, h1);
-            const double ri = 1.0 / r;
             if (C) {
               cd c = ld(&sc[m]);
               A.Pc += h0 * c;
@@ -515,7 +522,7 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
               double ct = dx * ri, st = dy * ri;
               A.Pd += h1 * ri * q * (dx * v1 + dy * v2);
               if (PG >= 2) {
-                cd zi = NR > 0 ? cd{1.0 / az, 0.0} : inv(z);
+                cd zi = NR > 0 ? cd{azi, 0.0} : inv(z);
                 cd h2 = h1 * zi * 2.0 - h0;
                 cd f1 = h2 * (ct * ct - 0.5) - h0 * 0.5, f2 = h2 * (ct * st);
                 cd f3 = h2 * (st * st - 0.5) - h0 * 0.5;

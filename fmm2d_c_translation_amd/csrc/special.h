@@ -68,13 +68,13 @@ FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
 }
 
 // H0 = J0 + i Y0 and H1 = J1 + i Y1 of a real argument 0 < x <= JY_XMAX
-// of the chosen N (jy_series.h): the ascending series of hank_series in
+// of the chosen N (jy_series.h), given xi = 1/x: the ascending series of hank_series in
 // real arithmetic, J0 = P(w) and the Y0 remainder Q(w) in w = x^2/4 by
 // Horner with their derivatives, J1 = -P'(w) x/2 and
 // Y1 = -(2/pi) [J0/x - (ln(x/2) + gamma) J1 + Q'(w) x/2]
 // (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
 template <int N>
-FMM_HD void hank_series_real(double x, cd& h0, cd& h1) {
+FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
   const double w = x * x * 0.25;
   double p = JY_JA[N - 1], dp = 0.0, q = JY_YC[N - 1], dq = 0.0;
 #pragma unroll
@@ -87,9 +87,13 @@ FMM_HD void hank_series_real(double x, cd& h0, cd& h1) {
   const double hx = 0.5 * x, lg = log(hx) + kEuler;
   const double j1 = -dp * hx;
   const double y0 = (lg * p + q) * (2.0 / kPi);
-  const double y1 = -(2.0 / kPi) * (p / x - lg * j1 + dq * hx);
+  const double y1 = -(2.0 / kPi) * (p * xi - lg * j1 + dq * hx);
   h0 = cd{p, y0};
   h1 = cd{j1, y1};
+}
+template <int N>
+FMM_HD void hank_series_real(double x, cd& h0, cd& h1) {
+  hank_series_real<N>(x, 1.0 / x, h0, h1);
 }
 
 FMM_HD void hank_laguerre(cd z, cd& h0, cd& h1) {
