@@ -31,6 +31,11 @@ sys.path.insert(0, HERE)
 P2P_FLOPS_PER_PAIR = 180.0
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per near-field launch at the default workload (ns = nt = 1e6):
+# rocprofv3 FETCH_SIZE and WRITE_SIZE of k_hp2p, each scaled by what the same
+# counter reads for a 1 GiB device copy (x2.0 and x1.0); the commands and
+# numbers are in profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md
+P2P_TRAFFIC_BYTES_1M = 747.2e6
 
 
 def make_inputs(n, nt, seed=2):
@@ -66,7 +71,19 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (rocprof counter "
+                         "calibration: FETCH_SIZE/WRITE_SIZE of a kernel of known traffic)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; the multi-GPU path is a "
                                    "design only (see DESIGN.md), so --gpus must be 1"}))
@@ -154,7 +171,8 @@ def main():
              "leaf_order_gather", "operator_setup", "call_total"], stats[15:23])},
         "roofline": {"kernel": "k_hp2p (list-1 near field)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": P2P_TRAFFIC_BYTES_1M if n == 1_000_000 else None,
                      "pairs_per_launch_avg": pairs / 2, "flops_per_pair": P2P_FLOPS_PER_PAIR,
                      "m2l_tflops": None if m2l_tf is None else round(m2l_tf, 3)},
         "cpu_baseline": None,
