@@ -1,9 +1,12 @@
 // direct.hip -- O(N*M) direct sums on the GPU behind the reference's
-// c2d_direct* entry points (c_translation/src/cauchykernels2d.c:31-442).
-// One thread per target; sources staged through LDS in tiles of 256.
-// Outputs are incremented, as in the reference.
+// c2d_direct* (c_translation/src/cauchykernels2d.c:31-442), l2d_direct*
+// (lapkernels2d.c:19-494) and h2d_direct* (helmkernels2d.c:21-520) entry
+// points.  One thread per target; sources staged through LDS in tiles of
+// 256.  Outputs are incremented, as in the reference; each routine keeps
+// its own skip test (rr vs thresh^2 or |z| vs thresh, <= or <).
 #include "../../include/fmm2d_hip.h"
 #include "engine.h"
+#include "special.h"
 
 namespace fmm {
 namespace {
@@ -67,6 +70,259 @@ __global__ void __launch_bounds__(TILE) k_c2d(const double2* src, int ns, const 
       if (PG >= 3) hess[o] = make_double2(hess[o].x + H.x, hess[o].y + H.y);
     }
   }
+}
+
+// log kernel with complex densities (lapkernels2d.c): charge c log r,
+// dipole d (v . grad_y log|x - y|); gradient (nd,2,nt), Hessian (nd,3,nt)
+template <bool C, bool Q, int PG>
+__global__ void __launch_bounds__(TILE) k_l2d(const double2* src, int ns, const double2* chg,
+                                              const double2* dip, const double* dv,
+                                              const double2* trg, int nt, int nd, bool le,
+                                              double th2, double2* pot, double2* grad,
+                                              double2* hess) {
+  __shared__ double2 sx[TILE];
+  __shared__ double2 sc[TILE];
+  __shared__ double2 sq[TILE];
+  __shared__ double2 sv[TILE];
+  const int j = blockIdx.x * TILE + threadIdx.x;
+  const bool act = j < nt;
+  const double2 x = act ? trg[j] : make_double2(0, 0);
+  for (int d = 0; d < nd; ++d) {
+    cd P = mk(0, 0), Gx = mk(0, 0), Gy = mk(0, 0), Hxx = mk(0, 0), Hxy = mk(0, 0), Hyy = mk(0, 0);
+    for (int s0 = 0; s0 < ns; s0 += TILE) {
+      int cnt = min(TILE, ns - s0);
+      if ((int)threadIdx.x < cnt) {
+        const size_t i = s0 + threadIdx.x;
+        sx[threadIdx.x] = src[i];
+        if (C) sc[threadIdx.x] = chg[i * nd + d];
+        if (Q) {
+          sq[threadIdx.x] = dip[i * nd + d];
+          sv[threadIdx.x] = make_double2(dv[(i * 2 + 0) * nd + d], dv[(i * 2 + 1) * nd + d]);
+        }
+      }
+      __syncthreads();
+      if (act)
+        for (int m = 0; m < cnt; ++m) {
+          const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
+          const double dx2 = dx * dx, dy2 = dy * dy, rr = dx2 + dy2;
+          if (le ? rr <= th2 : rr < th2) continue;
+          const double rr2 = rr * rr;
+          if (C) {
+            const cd c = mk(sc[m].x, sc[m].y);
+            P += c * (log(rr) / 2);
+            if (PG >= 2) { Gx += c * (dx / rr); Gy += c * (dy / rr); }
+            if (PG >= 3) {
+              Hxx += c * ((rr - 2 * dx2) / rr2);
+              Hxy += c * (-2 * dx * dy / rr2);
+              Hyy += c * ((rr - 2 * dy2) / rr2);
+            }
+          }
+          if (Q) {
+            const cd q = mk(sq[m].x, sq[m].y), d1 = q * sv[m].x, d2 = q * sv[m].y;
+            P += d1 * (-dx / rr) + d2 * (-dy / rr);
+            if (PG >= 2) {
+              const double a1 = -(rr - 2 * dx2) / rr2, a2 = 2 * dx * dy / rr2;
+              const double b2 = -(rr - 2 * dy2) / rr2;
+              Gx += d1 * a1 + d2 * a2;
+              Gy += d1 * a2 + d2 * b2;
+            }
+            if (PG >= 3) {
+              const double rr3 = rr2 * rr;
+              const double e1 = -2 * dx * (dx2 - 3 * dy2) / rr3, e2 = 2 * dy * (dy2 - 3 * dx2) / rr3;
+              Hxx += d1 * e1 + d2 * e2;
+              Hxy += d1 * e2 + d2 * (-e1);
+              Hyy += d1 * (-e1) + d2 * (-e2);
+            }
+          }
+        }
+      __syncthreads();
+    }
+    if (act) {
+      auto add = [](double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); };
+      add(pot + (size_t)j * nd + d, P);
+      if (PG >= 2) {
+        add(grad + ((size_t)j * 2 + 0) * nd + d, Gx);
+        add(grad + ((size_t)j * 2 + 1) * nd + d, Gy);
+      }
+      if (PG >= 3) {
+        add(hess + ((size_t)j * 3 + 0) * nd + d, Hxx);
+        add(hess + ((size_t)j * 3 + 1) * nd + d, Hxy);
+        add(hess + ((size_t)j * 3 + 2) * nd + d, Hyy);
+      }
+    }
+  }
+}
+
+// Helmholtz, (i/4) H0(k r) kernel (helmkernels2d.c): charge (i/4) c H0,
+// dipole (i/4) d k H1(kr) (x - y).v / r; the potential-only charge routine
+// skips on |z| <= thresh, every other routine on <
+template <bool C, bool Q, int PG>
+__global__ void __launch_bounds__(TILE) k_h2d(cd zk, const double2* src, int ns,
+                                              const double2* chg, const double2* dip,
+                                              const double* dv, const double2* trg, int nt,
+                                              int nd, bool le, double th, double2* pot,
+                                              double2* grad, double2* hess) {
+  __shared__ double2 sx[TILE];
+  __shared__ double2 sc[TILE];
+  __shared__ double2 sq[TILE];
+  __shared__ double2 sv[TILE];
+  const int j = blockIdx.x * TILE + threadIdx.x;
+  const bool act = j < nt;
+  const double2 x = act ? trg[j] : make_double2(0, 0);
+  const cd i4 = mk(0, 0.25), ik4 = zk * mk(0, 0.25);
+  for (int d = 0; d < nd; ++d) {
+    cd Pc = mk(0, 0), Pd = mk(0, 0), Gx = mk(0, 0), Gy = mk(0, 0);
+    cd Hxx = mk(0, 0), Hxy = mk(0, 0), Hyy = mk(0, 0);
+    for (int s0 = 0; s0 < ns; s0 += TILE) {
+      int cnt = min(TILE, ns - s0);
+      if ((int)threadIdx.x < cnt) {
+        const size_t i = s0 + threadIdx.x;
+        sx[threadIdx.x] = src[i];
+        if (C) sc[threadIdx.x] = chg[i * nd + d];
+        if (Q) {
+          sq[threadIdx.x] = dip[i * nd + d];
+          sv[threadIdx.x] = make_double2(dv[(i * 2 + 0) * nd + d], dv[(i * 2 + 1) * nd + d]);
+        }
+      }
+      __syncthreads();
+      if (act)
+        for (int m = 0; m < cnt; ++m) {
+          const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
+          const double rr = dx * dx + dy * dy, r = sqrt(rr);
+          const cd z = zk * r;
+          const double az = cabs_(z);
+          if (le ? az <= th : az < th) continue;
+          cd h0, h1;
+          hank01(z, h0, h1);
+          if (C) {
+            const cd c = mk(sc[m].x, sc[m].y);
+            Pc += h0 * c;
+            if (PG >= 2) {
+              const cd g = -(h1 * c * (1.0 / r));
+              Gx += g * dx;
+              Gy += g * dy;
+            }
+            if (PG >= 3) {
+              const cd f = c * (1.0 / (r * rr)), h2z = -(z * h0) + h1 * 2.0;
+              Hxx += f * (h2z * (dx * dx) - h1 * rr);
+              Hxy += f * (h2z * (dx * dy));
+              Hyy += f * (h2z * (dy * dy) - h1 * rr);
+            }
+          }
+          if (Q) {
+            const cd q = mk(sq[m].x, sq[m].y);
+            const double v1 = sv[m].x, v2 = sv[m].y, ct = dx / r, st = dy / r;
+            Pd += h1 * (1.0 / r) * q * (dx * v1 + dy * v2);
+            if (PG >= 2) {
+              const cd h2 = h1 * inv(z) * 2.0 - h0;
+              const cd f1 = h2 * (ct * ct - 0.5) - h0 * 0.5, f2 = h2 * (ct * st);
+              const cd f3 = h2 * (st * st - 0.5) - h0 * 0.5, qk = -(q * zk);
+              Gx += qk * (f1 * v1 + f2 * v2);
+              Gy += qk * (f2 * v1 + f3 * v2);
+              if (PG >= 3) {
+                const cd h3 = h2 * inv(z) * 4.0 - h1;
+                const cd a = (h1 * 0.75 - h3 * (0.5 * (ct * ct - 0.5 - st * st))) * ct;
+                const cd bb = (h1 * 0.25 - h3 * (0.5 * (1.5 * ct * ct - 0.5 * st * st))) * st;
+                const cd c2 = (h1 * 0.25 - h3 * (0.5 * (1.5 * st * st - 0.5 * ct * ct))) * ct;
+                const cd e = (h1 * 0.75 - h3 * (0.5 * (st * st - 0.5 - ct * ct))) * st;
+                const cd qk2 = -(q * zk * zk);
+                Hxx += qk2 * (a * v1 + bb * v2);
+                Hxy += qk2 * (bb * v1 + c2 * v2);
+                Hyy += qk2 * (c2 * v1 + e * v2);
+              }
+            }
+          }
+        }
+      __syncthreads();
+    }
+    if (act) {
+      auto add = [](double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); };
+      add(pot + (size_t)j * nd + d, Pc * i4 + Pd * ik4);
+      if (PG >= 2) {
+        add(grad + ((size_t)j * 2 + 0) * nd + d, Gx * ik4);
+        add(grad + ((size_t)j * 2 + 1) * nd + d, Gy * ik4);
+      }
+      if (PG >= 3) {
+        add(hess + ((size_t)j * 3 + 0) * nd + d, Hxx * ik4);
+        add(hess + ((size_t)j * 3 + 1) * nd + d, Hxy * ik4);
+        add(hess + ((size_t)j * 3 + 2) * nd + d, Hyy * ik4);
+      }
+    }
+  }
+}
+
+// host staging for the direct sums: copy in, run, copy the outputs back
+struct DirectIO {
+  int nd, ns, nt, ncg, nch;  // gradient / Hessian components per output
+  const double *sources, *targ, *dipvec;
+  const void *charge, *dipstr;
+  void *pot, *grad, *hess;
+};
+
+template <class Launch>
+int run_direct(const DirectIO& io, bool C, bool Q, int PG, Launch&& launch) {
+  if (io.nd < 1 || io.ns < 0 || io.nt < 0) return IER_ARGS;
+  if (io.ns == 0 || io.nt == 0) return IER_OK;
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  Scratch S(c.st);
+  const int nd = io.nd, ns = io.ns, nt = io.nt;
+  double2* s = S.get<double2>(ns);
+  double2* t = S.get<double2>(nt);
+  double2* cc = C ? S.get<double2>((size_t)nd * ns) : nullptr;
+  double2* qq = Q ? S.get<double2>((size_t)nd * ns) : nullptr;
+  double* vv = Q ? S.get<double>((size_t)2 * nd * ns) : nullptr;
+  const size_t no = (size_t)nd * nt;
+  double2* po = S.get<double2>(no);
+  double2* go = PG >= 2 ? S.get<double2>(no * io.ncg) : nullptr;
+  double2* ho = PG >= 3 ? S.get<double2>(no * io.nch) : nullptr;
+  if (S.err) return S.err;
+  HIP_TRY(hipMemcpyAsync(s, io.sources, 16 * (size_t)ns, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(t, io.targ, 16 * (size_t)nt, hipMemcpyHostToDevice, c.st));
+  if (C) HIP_TRY(hipMemcpyAsync(cc, io.charge, 16 * (size_t)nd * ns, hipMemcpyHostToDevice, c.st));
+  if (Q) {
+    HIP_TRY(hipMemcpyAsync(qq, io.dipstr, 16 * (size_t)nd * ns, hipMemcpyHostToDevice, c.st));
+    HIP_TRY(hipMemcpyAsync(vv, io.dipvec, 16 * (size_t)nd * ns, hipMemcpyHostToDevice, c.st));
+  }
+  HIP_TRY(hipMemcpyAsync(po, io.pot, 16 * no, hipMemcpyHostToDevice, c.st));
+  if (go) HIP_TRY(hipMemcpyAsync(go, io.grad, 16 * no * io.ncg, hipMemcpyHostToDevice, c.st));
+  if (ho) HIP_TRY(hipMemcpyAsync(ho, io.hess, 16 * no * io.nch, hipMemcpyHostToDevice, c.st));
+  launch(c.st, s, cc, qq, vv, t, po, go, ho);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(io.pot, po, 16 * no, hipMemcpyDeviceToHost, c.st));
+  if (go) HIP_TRY(hipMemcpyAsync(io.grad, go, 16 * no * io.ncg, hipMemcpyDeviceToHost, c.st));
+  if (ho) HIP_TRY(hipMemcpyAsync(io.hess, ho, 16 * no * io.nch, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
+}
+
+template <bool C, bool Q, int PG>
+int l2d(int nd, const double* sources, int ns, const void* charge, const void* dipstr,
+        const double* dipvec, const double* targ, int nt, void* pot, void* grad, void* hess,
+        double th, bool le) {
+  DirectIO io{nd, ns, nt, 2, 3, sources, targ, dipvec, charge, dipstr, pot, grad, hess};
+  return run_direct(io, C, Q, PG, [&](hipStream_t st, const double2* s, const double2* cc,
+                                      const double2* qq, const double* vv, const double2* t,
+                                      double2* po, double2* go, double2* ho) {
+    k_l2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, st>>>(s, ns, cc, qq, vv, t, nt, nd, le,
+                                                             th * th, po, go, ho);
+  });
+}
+
+template <bool C, bool Q, int PG>
+int h2d(int nd, const fmm2d_cplx* zk, const double* sources, int ns, const void* charge,
+        const void* dipstr, const double* dipvec, const double* targ, int nt, void* pot,
+        void* grad, void* hess, double th, bool le) {
+  const double* z = reinterpret_cast<const double*>(zk);
+  const cd k = mk(z[0], z[1]);
+  DirectIO io{nd, ns, nt, 2, 3, sources, targ, dipvec, charge, dipstr, pot, grad, hess};
+  return run_direct(io, C, Q, PG, [&](hipStream_t st, const double2* s, const double2* cc,
+                                      const double2* qq, const double* vv, const double2* t,
+                                      double2* po, double2* go, double2* ho) {
+    k_h2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, st>>>(k, s, ns, cc, qq, vv, t, nt, nd, le,
+                                                             th, po, go, ho);
+  });
 }
 
 template <bool C, bool Q, int PG>
@@ -151,5 +407,59 @@ void c2d_directcdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns,
                     const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess,
                     const double* th) {
   fmm::c2d<true, true, 3>(*nd, s, *ns, c, q, t, *nt, pot, grad, hess, *th, 1);
+}
+void l2d_directcp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::l2d<true, false, 1>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void h2d_directcp_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::h2d<true, false, 1>(*nd, zk, s, *ns, c, nullptr, nullptr, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void l2d_directcg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::l2d<true, false, 2>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, nullptr, *th, false);
+}
+void h2d_directcg_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::h2d<true, false, 2>(*nd, zk, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, nullptr, *th, false);
+}
+void l2d_directch_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::l2d<true, false, 3>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, hess, *th, false);
+}
+void h2d_directch_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::h2d<true, false, 3>(*nd, zk, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, hess, *th, false);
+}
+void l2d_directdp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::l2d<false, true, 1>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void h2d_directdp_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::h2d<false, true, 1>(*nd, zk, s, *ns, nullptr, q, v, t, *nt, pot, nullptr, nullptr, *th, false);
+}
+void l2d_directdg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::l2d<false, true, 2>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, grad, nullptr, *th, true);
+}
+void h2d_directdg_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::h2d<false, true, 2>(*nd, zk, s, *ns, nullptr, q, v, t, *nt, pot, grad, nullptr, *th, false);
+}
+void l2d_directdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::l2d<false, true, 3>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, grad, hess, *th, true);
+}
+void h2d_directdh_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::h2d<false, true, 3>(*nd, zk, s, *ns, nullptr, q, v, t, *nt, pot, grad, hess, *th, false);
+}
+void l2d_directcdp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::l2d<true, true, 1>(*nd, s, *ns, c, q, v, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void h2d_directcdp_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
+  fmm::h2d<true, true, 1>(*nd, zk, s, *ns, c, q, v, t, *nt, pot, nullptr, nullptr, *th, false);
+}
+void l2d_directcdg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::l2d<true, true, 2>(*nd, s, *ns, c, q, v, t, *nt, pot, grad, nullptr, *th, false);
+}
+void h2d_directcdg_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, const double* th) {
+  fmm::h2d<true, true, 2>(*nd, zk, s, *ns, c, q, v, t, *nt, pot, grad, nullptr, *th, false);
+}
+void l2d_directcdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::l2d<true, true, 3>(*nd, s, *ns, c, q, v, t, *nt, pot, grad, hess, *th, false);
+}
+void h2d_directcdh_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
+  fmm::h2d<true, true, 3>(*nd, zk, s, *ns, c, q, v, t, *nt, pot, grad, hess, *th, false);
 }
 }

@@ -149,6 +149,32 @@ void c2d_directcdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int 
                     const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess,
                     const double *thresh);
 
+
+/* ---- direct sums, log kernel with complex densities: replace l2d_direct{c,d,cd}{p,g,h}_
+ * (c_translation/src/lapkernels2d.c:19-494); outputs incremented, grad (nd,2,nt),
+ * hess (nd,3,nt) */
+void l2d_directcp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void l2d_directcg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void l2d_directch_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void l2d_directdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void l2d_directdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void l2d_directdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void l2d_directcdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void l2d_directcdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void l2d_directcdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+
+/* ---- direct sums, Helmholtz (i/4) H0(k r): replace h2d_direct{c,d,cd}{p,g,h}_
+ * (c_translation/src/helmkernels2d.c:21-520); outputs incremented */
+void h2d_directcp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void h2d_directcg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void h2d_directch_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void h2d_directdp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void h2d_directdg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void h2d_directdh_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+void h2d_directcdp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
+void h2d_directcdg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
+void h2d_directcdh_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+
 /* ---- device-resident variants (no counterpart in the reference).
  * Same argument meaning as cfmm2d_ndiv_ / lfmm2d_ndiv_, but every array is a
  * device pointer, and `stream` (a hipStream_t, or NULL for the library's
