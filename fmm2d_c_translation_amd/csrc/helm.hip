@@ -180,15 +180,15 @@ struct NearAcc {
 // contribution of order n (with value v = scaled H or J at the point,
 // complex or, for J of a real argument, real; pos = e^{i n theta},
 // neg = (-e^{-i theta})^n)
-template <int PG, class V>
-__device__ inline void eval_term(const Ex& E, int n, V v, cd pos, cd neg, EvalAcc& A) {
+template <int PG, class EA, class V>
+__device__ inline void eval_term(const EA& E, int n, V v, cd pos, cd neg, EvalAcc& A) {
   if (n == 0) {
     A.P += v * E.c(0);
     if (PG >= 2) { A.Gx += v * E.dx(0); A.Gy += v * E.dy(0); }
     if (PG >= 3) {
-      A.Hxx += v * E.d2<false, false>(0);
-      A.Hxy += v * E.d2<false, true>(0);
-      A.Hyy += v * E.d2<true, true>(0);
+      A.Hxx += v * E.template d2<false, false>(0);
+      A.Hxy += v * E.template d2<false, true>(0);
+      A.Hyy += v * E.template d2<true, true>(0);
     }
     return;
   }
@@ -199,15 +199,15 @@ __device__ inline void eval_term(const Ex& E, int n, V v, cd pos, cd neg, EvalAc
     A.Gy += v * (E.dy(n) * pos + E.dy(-n) * neg);
   }
   if (PG >= 3 && n <= p + 2) {
-    A.Hxx += v * (E.d2<false, false>(n) * pos + E.d2<false, false>(-n) * neg);
-    A.Hxy += v * (E.d2<false, true>(n) * pos + E.d2<false, true>(-n) * neg);
-    A.Hyy += v * (E.d2<true, true>(n) * pos + E.d2<true, true>(-n) * neg);
+    A.Hxx += v * (E.template d2<false, false>(n) * pos + E.template d2<false, false>(-n) * neg);
+    A.Hxy += v * (E.template d2<false, true>(n) * pos + E.template d2<false, true>(-n) * neg);
+    A.Hyy += v * (E.template d2<true, true>(n) * pos + E.template d2<true, true>(-n) * neg);
   }
 }
 
 // multipole at a point: v_n = H_n rs^n by upward recurrence, n = 0..p+2
-template <int PG>
-__device__ void eval_mp_at(const Ex& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+template <int PG, class EA>
+__device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, double th, EvalAcc& A) {
   cd z = zk * r;
   if (cabs_(z) < 1e-200) return;
   cd h0, h1;
@@ -230,8 +230,8 @@ __device__ void eval_mp_at(const Ex& E, cd zk, double rs, double r, double th, E
 
 // local at a point: v_n = J_n / rs^n by Miller's downward recurrence with
 // the generating-function normalisation applied at the end
-template <int PG>
-__device__ void eval_lo_at(const Ex& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+template <int PG, class EA>
+__device__ void eval_lo_at(const EA& E, cd zk, double rs, double r, double th, EvalAcc& A) {
   cd z = zk * r;
   const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
   if (cabs_(z) < 1e-200) {  // J_0 = 1, others 0
@@ -288,8 +288,8 @@ __device__ __noinline__ H01 hank_real_point(double x) {
   return h;
 }
 
-template <int PG, int NR>
-__device__ void eval_mp_at_real(const Ex& E, double k, double rs, double r, double th,
+template <int PG, int NR, class EA>
+__device__ void eval_mp_at_real(const EA& E, double k, double rs, double r, double th,
                                 EvalAcc& A) {
   const double x = k * r;
   if (x < 1e-200) return;
@@ -322,8 +322,8 @@ __device__ inline int miller_top_real(int nterms, double x) {
   return nterms + 10000;
 }
 
-template <int PG>
-__device__ void eval_lo_at_real(const Ex& E, double k, double rs, double r, double th,
+template <int PG, class EA>
+__device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, double th,
                                 EvalAcc& A) {
   const double x = k * r;
   const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
@@ -359,6 +359,102 @@ __device__ void eval_lo_at_real(const Ex& E, double k, double rs, double r, doub
   const double sc = 1.0 / (g + 2.0 * H);
   A.P += B.P * sc; A.Gx += B.Gx * sc; A.Gy += B.Gy * sc;
   A.Hxx += B.Hxx * sc; A.Hxy += B.Hxy * sc; A.Hyy += B.Hyy * sc;
+}
+
+// An expansion staged in LDS with its x and y derivative expansions
+// (value and first derivatives only): c(k) for |k| <= p+2 (zero beyond p),
+// dx(k), dy(k) for |k| <= p+1, each array addressed at its k = 0 entry.
+struct ExS {
+  const cd *e0, *ex, *ey;
+  int p;
+  __device__ cd c(int k) const { return e0[k]; }
+  __device__ cd dx(int k) const { return ex[k]; }
+  __device__ cd dy(int k) const { return ey[k]; }
+  template <bool Y1, bool Y2>
+  __device__ cd d2(int) const { return mk(0, 0); }  // not staged (PG <= 2 only)
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Four boxes per wave, 16 lanes per box, lane per point; for each source
+// expansion the box's 16 lanes stage it in LDS with its derivative
+// expansions (computed once per box instead of once per point and order),
+// then evaluate.  PG <= 2.  W >= 2 pmax + 5 entries per staged array.
+template <bool MPK, int PG, int NR>
+__global__ void __launch_bounds__(WAVE)
+k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int2* rng,
+          const double2* pts, int N, int nd, size_t per, const double2* ex, int W,
+          double2* pot, double2* grad) {
+  extern __shared__ double2 shx[];
+  const int grp = (int)threadIdx.x >> 4, gl = (int)threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + grp;
+  if (w >= nwork) return;  // the whole group leaves; no block barriers below
+  cd* E0 = reinterpret_cast<cd*>(shx) + (size_t)grp * 3 * W;
+  cd* EX = E0 + W;
+  cd* EY = EX + W;
+  const int b = work[w];
+  const int2 rg = rng[b];
+  const cd i4 = mk(0, 0.25);
+  const int nsrc = MPK ? D.o3[b + 1] - D.o3[b] : 1;
+  for (int i0 = rg.x; i0 < rg.y; i0 += 16) {
+    const int i = i0 + gl;
+    const bool act = i < rg.y;
+    const double2 x = act ? pts[i] : make_double2(0.0, 0.0);
+    for (int d = 0; d < nd; ++d) {
+      EvalAcc A = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+      for (int k = 0; k < nsrc; ++k) {
+        const int s = MPK ? D.l3[D.o3[b] + k] : b;
+        const HLev lv = lev[D.level[s]];
+        const int p = lv.p;
+        const double rsd = MPK ? lv.rs : 1.0 / lv.rs;
+        const cd z1 = zk * (0.5 / rsd), z2 = zk * (0.5 * rsd);
+        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1);
+        wave_sync();  // the group's previous readers are done
+        for (int kk = gl; kk < 2 * p + 5; kk += 16) {
+          const int q = kk - (p + 2);
+          E0[kk] = (q < -p || q > p) ? mk(0, 0) : ld(e + q + p);
+        }
+        wave_sync();
+        const cd* c0 = E0 + (p + 2);
+        for (int kk = gl; kk < 2 * p + 3; kk += 16) {
+          const int q = kk - (p + 1);
+          const cd a = c0[q + 1], bm = c0[q - 1];
+          cd gx, gy;
+          if (q > 0) { gx = z2 * a - z1 * bm; gy = mul_i(z2 * a + z1 * bm); }
+          else if (q < 0) { gx = z1 * a - z2 * bm; gy = mul_i(z1 * a + z2 * bm); }
+          else { gx = z2 * (a - bm); gy = mul_i(z2 * (a + bm)); }
+          EX[kk] = gx;
+          EY[kk] = gy;
+        }
+        wave_sync();
+        if (act) {
+          const ExS E{c0, EX + (p + 1), EY + (p + 1), p};
+          const double2 cc = D.ctr[s];
+          const double ddx = x.x - cc.x, ddy = x.y - cc.y;
+          const double r = sqrt(ddx * ddx + ddy * ddy);
+          const double th = (ddx == 0 && ddy == 0) ? 0.0 : atan2(ddy, ddx);
+          if (NR > 0) {
+            if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, th, A);
+            else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, th, A);
+          } else {
+            if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
+            else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+          }
+        }
+      }
+      if (act) {
+        addc(pot + (size_t)d * N + i, A.P * i4);
+        if (PG >= 2) {
+          addc(grad + (size_t)d * N + i, A.Gx * i4);
+          addc(grad + (size_t)(nd + d) * N + i, A.Gy * i4);
+        }
+      }
+    }
+  }
 }
 
 // Four boxes per wave, 16 lanes per box, lane per point (leaves hold about
@@ -654,8 +750,8 @@ void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& 
 template <bool MPK>
 void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& D,
                   const HLev* lev, cd zk, const int2* rng, const double2* pts, int N, int nd,
-                  size_t per, const double2* ex, double xmax, double2* pot, double2* grad,
-                  double2* hess) {
+                  size_t per, const double2* ex, double xmax, int pmax, double2* pot,
+                  double2* grad, double2* hess) {
   const bool rz = zk.y == 0.0 && zk.x > 0.0;
   int nr = 0;
   if (rz) {
@@ -663,9 +759,20 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
     else nr = xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
   }
   const int nbx = (nw + 3) / 4;
-#define FMM_HEVAL(PGV, NRV)                                                                   \
-  k_heval<MPK, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, nw, D, lev, zk, rng, pts, N, nd, per, ex, \
-                                               pot, grad, hess)
+  // staged expansions (value and gradient) while three arrays of 2 pmax + 5
+  // entries per group fit comfortably in LDS
+  const int W = 2 * pmax + 5;
+  const bool staged = pg <= 2 && pmax <= 77;
+  const size_t shs = (size_t)4 * 3 * W * sizeof(double2);
+#define FMM_HEVAL(PGV, NRV)                                                                      \
+  do {                                                                                           \
+    if (staged && PGV <= 2)                                                                      \
+      k_heval_s<MPK, (PGV <= 2 ? PGV : 2), NRV><<<nbx, WAVE, shs, st>>>(                         \
+          work, nw, D, lev, zk, rng, pts, N, nd, per, ex, W, pot, grad);                         \
+    else                                                                                         \
+      k_heval<MPK, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, nw, D, lev, zk, rng, pts, N, nd, per,   \
+                                                   ex, pot, grad, hess);                         \
+  } while (0)
 #define FMM_HEVAL_NR(PGV)                                                \
   do {                                                                   \
     if constexpr (!MPK) {                                                \
@@ -933,11 +1040,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.l3.n)
-      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, s.po, s.gr, s.he);
+      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, pmax, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.lv.n)
-      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, 0.0, s.po, s.gr, s.he);
+      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, 0.0, pmax, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
