@@ -65,7 +65,8 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
   const double rs = lev[L].rs;
   const int Q = p + 2;                   // v_0 .. v_{p+1} (J) / .. v_{p+2} (H)
   cd* Z = reinterpret_cast<cd*>(sm);     // [Q+1][CH] powers of e^{-i theta}
-  VT* V = reinterpret_cast<VT*>(Z + (size_t)(Q + 1) * CH);  // [Q+1][CH]
+  cd* F = Z + (size_t)(Q + 1) * CH;      // [6][CH] per-point factors of one channel
+  VT* V = reinterpret_cast<VT*>(F + (size_t)6 * CH);  // [Q+1][CH]
   const double2 ctr = D.ctr[b];
   const double rsd = MP ? rs : 1.0 / rs;  // dipole scale (locals use 1/rs)
   const int nsb = MP ? 1 : D.o4[b + 1] - D.o4[b];
@@ -96,35 +97,43 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
       }
       __syncthreads();
       for (int d = 0; d < nd; ++d) {
+        // per-point factors of this channel: charge, and the dipole terms of
+        // orders m > 0 (TA, TB), m < 0 (TC, TD) and m = 0 (K0), so that the
+        // coefficient loop below is the same for every lane
+        for (int j = lane; j < cnt; j += WAVE) {
+          const size_t g = (size_t)d * N + c0 + j;
+          F[j] = chg ? ld(chg + g) : mk(0, 0);
+          if (dip) {
+            const cd qf = ld(dip + g) * (zk * -0.5);
+            const double v1 = dv[g], v2 = dv[(size_t)nd * N + g];
+            const cd zm = Z[(size_t)CH + j], zi = conj(zm);
+            F[CH + j] = ((zi / rsd) * mk(-v1, v2)) * qf;
+            F[2 * CH + j] = ((zm * rsd) * mk(v1, v2)) * qf;
+            F[3 * CH + j] = ((zm / rsd) * mk(-v1, -v2)) * qf;
+            F[4 * CH + j] = ((zi * rsd) * mk(v1, -v2)) * qf;
+            F[5 * CH + j] = (qf * rsd) * ((zm + zi) * v1 + mul_i(zm - zi) * v2);
+          }
+        }
+        __syncthreads();
         for (int m = lane - p; m <= p; m += WAVE) {
           const int n = m < 0 ? -m : m;
+          const bool neg = m < 0;
+          const int ia = m == 0 ? 1 : n - 1, ib = m == 0 ? 1 : n + 1;
+          const cd* FA = F + (m > 0 ? CH : neg ? 3 * CH : 5 * CH);
+          const cd* FB = F + (m > 0 ? 2 * CH : 4 * CH);
+          const double sb = m == 0 ? 0.0 : 1.0;  // m = 0 has one dipole term
           cd acc = mk(0, 0);
           for (int j = 0; j < cnt; ++j) {
-            cd zn = Z[(size_t)n * CH + j];
-            cd ph = m >= 0 ? zn : ((n & 1) ? -conj(zn) : conj(zn));  // (-zinv)^n
-            if (chg) acc += ld(chg + (size_t)d * N + c0 + j) * (V[(size_t)n * CH + j] * ph);
-            if (dip) {
-              cd q = ld(dip + (size_t)d * N + c0 + j);
-              double v1 = dv[(size_t)d * N + c0 + j], v2 = dv[(size_t)(nd + d) * N + c0 + j];
-              cd zm = Z[(size_t)CH + j], zi = conj(zm);
-              if (m == 0) {
-                acc += -(q * (V[(size_t)CH + j] * (zk * (0.5 * rsd)))) *
-                       ((zm + zi) * v1 + mul_i(zm - zi) * v2);
-              } else if (m > 0) {
-                cd t7 = (zi / rsd) * mk(-v1, v2), t8 = (zm * rsd) * mk(v1, v2);
-                acc += (V[(size_t)(n - 1) * CH + j] * t7 + V[(size_t)(n + 1) * CH + j] * t8) *
-                       (-(zk * 0.5)) * ph * q;
-              } else {
-                cd t9 = (zm / rsd) * mk(-v1, -v2), t10 = (zi * rsd) * mk(v1, -v2);
-                acc += (V[(size_t)(n - 1) * CH + j] * t9 + V[(size_t)(n + 1) * CH + j] * t10) *
-                       (-(zk * 0.5)) * ph * q;
-              }
-            }
+            const cd zn = Z[(size_t)n * CH + j];
+            const cd ph = neg ? ((n & 1) ? -conj(zn) : conj(zn)) : zn;  // (-zinv)^n for m < 0
+            cd t = F[j] * V[(size_t)n * CH + j];
+            if (dip) t += FA[j] * V[(size_t)ia * CH + j] + FB[j] * (V[(size_t)ib * CH + j] * sb);
+            acc += t * ph;
           }
           addc(out + (size_t)b * per + (size_t)d * (2 * p + 1) + (m + p), acc);
         }
+        __syncthreads();  // F is rewritten for the next channel
       }
-      __syncthreads();
     }
   }
 }
@@ -934,8 +943,10 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // footprint near 12-16 KB so several waves share a CU
   const int CH = 16;
   const bool rz = zk.y == 0.0 && zk.x > 0.0;
-  const size_t shm = (size_t)(pmax + 3) * CH * (sizeof(double2) + sizeof(double2));
-  const size_t shm_mp = (size_t)(pmax + 3) * CH * (sizeof(double2) + (rz ? sizeof(double) : sizeof(double2)));
+  const size_t shm = (size_t)(pmax + 3) * CH * (sizeof(double2) + sizeof(double2)) +
+                     (size_t)6 * CH * sizeof(double2);
+  const size_t shm_mp = (size_t)(pmax + 3) * CH * (sizeof(double2) + (rz ? sizeof(double) : sizeof(double2))) +
+                        (size_t)6 * CH * sizeof(double2);
 
   // levels with gate[l] false (complex zk, boxes far above the decay length)
   // form no expansions; gate[l] only switches on as boxes shrink, so the
