@@ -125,10 +125,13 @@ __global__ void k_balance_cand(const uint64_t* R, int n, int k, uint64_t* out) {
 // so that box holds more than ndiv points and is split; no deeper box is.
 __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + ndiv >= n) return;
-  const uint64_t x = k[i] ^ k[i + ndiv];
-  const int c = x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
-  atomicMax(depth, c);
+  int c = -1;
+  if (i + ndiv < n) {
+    const uint64_t x = k[i] ^ k[i + ndiv];
+    c = x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
+  }
+  for (int o = 32; o >= 1; o >>= 1) c = max(c, __shfl_xor(c, o));  // one atomic per wave
+  if ((threadIdx.x & 63) == 0 && c >= 0) atomicMax(depth, c);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -167,11 +170,12 @@ __global__ void k_split_scatter(const uint64_t* K, const int* cnt, const int* fl
 // the parents of its in-domain neighbours to be split at level k-1, whose
 // neighbours' parents must then be split at k-2, and so on: at each coarser
 // level the forced boxes form a rectangle of at most 3 x 3 cells.  Written
-// as (level << 58 | key), 9 slots per level below k (short rectangles repeat
-// their first cell); the adaptive split boxes themselves come first.
-constexpr int kLevShift = 2 * kMaxDepth;
+// as (level << 2L | key) for a tree of depth L, 9 slots per level below k
+// (short rectangles repeat their first cell); the adaptive split boxes
+// themselves come first.
 __global__ void k_closure_cand(const uint64_t* R, const int* roff, const long* cbase, int L,
                                uint64_t* out) {
+  const int kLevShift = 2 * L;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= roff[L + 1]) return;
   int k = 0;
@@ -204,7 +208,7 @@ __global__ void k_closure_cand(const uint64_t* R, const int* roff, const long* c
 __global__ void k_split_levels(const uint64_t* U, const int* n, int L, int* off) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l > L + 1) return;
-  const uint64_t v = (uint64_t)l << kLevShift;
+  const uint64_t v = (uint64_t)l << (2 * L);
   int lo = 0, hi = *n;
   while (lo < hi) {
     int m = (lo + hi) >> 1;
@@ -213,9 +217,9 @@ __global__ void k_split_levels(const uint64_t* U, const int* n, int L, int* off)
   }
   off[l] = lo;
 }
-__global__ void k_strip_level(uint64_t* U, const int* n) {
+__global__ void k_strip_level(uint64_t* U, const int* n, int L) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < *n) U[i] &= (uint64_t(1) << kLevShift) - 1;
+  if (i < *n) U[i] &= (uint64_t(1) << (2 * L)) - 1;
 }
 
 struct Levels {
@@ -681,18 +685,19 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     if (roff[L] > 0)
       k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(cand, d_roff, d_cbase, L, cand);
     size_t tb1 = 0, tb2 = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, 64, c.st));
+    const int kbits = 2 * L + 5;  // level (< 32) above 2L key bits
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     void* tmp = S.get<char>(std::max(tb1, tb2));
     if (S.err) return S.err;
     if (nc > 0) {
-      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, 64, c.st));
+      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     } else {
       HIP_TRY(hipMemsetAsync(d_nu, 0, sizeof(int), c.st));
     }
     k_split_levels<<<1, 64, 0, c.st>>>(Rall, d_nu, L, d_loff);
-    k_strip_level<<<nblk(std::max<long>(nc, 1), 256), 256, 0, c.st>>>(Rall, d_nu);
+    k_strip_level<<<nblk(std::max<long>(nc, 1), 256), 256, 0, c.st>>>(Rall, d_nu, L);
     std::vector<int> lo(L + 2);
     e = sync_read(c, d_loff, L + 2, lo.data());
     if (e) return e;
