@@ -220,25 +220,33 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
     for (int s = g; s < nslot; s += G) {
       const int src = wok ? map[(size_t)w * nslot + s] : -1;
       if (!__any(src >= 0)) continue;
-      const double2* xs = X + (size_t)(src >= 0 ? src : 0) * xstride + (size_t)d * nin;
+      const bool sok = src >= 0;
+      const double2* xs = X + (size_t)(sok ? src : 0) * xstride + (size_t)d * nin;
       const double2* Ts = T + (size_t)s * nin * nout;
-#pragma unroll 2
+      if (sok && kq == 0) {
+        const double2 x0 = xs[0];
+        s0r += x0.x;
+        s0i += x0.y;
+      }
+      // Branch-free body: indexes are clamped into range and out-of-range
+      // operands zeroed by selects, so the accumulators stay in one register
+      // class across the loop.
       for (int k0 = 0; k0 < nin; k0 += 4) {
         const int j = k0 + kq;
         const bool jok = j < nin;
-        const double2 x = (src >= 0 && jok) ? xs[j] : make_double2(0.0, 0.0);
-        if (k0 == 0) {
-          s0r += x.x;
-          s0i += x.y;
-        }
+        const int jc = jok ? j : nin - 1;
+        const double2 xl = xs[jc];
+        const bool xok = sok && jok;
+        const double2 x = make_double2(xok ? xl.x : 0.0, xok ? xl.y : 0.0);
         const double xin = -x.y;
         double ar[MW], ai[MW];
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
           const int i = 16 * (rt0 + t) + n;
-          const double2 a = (jok && i < nout) ? Ts[(size_t)j * nout + i] : make_double2(0.0, 0.0);
-          ar[t] = a.x;
-          ai[t] = a.y;
+          const bool aok = jok && i < nout;
+          const double2 a = Ts[(size_t)jc * nout + (i < nout ? i : nout - 1)];
+          ar[t] = aok ? a.x : 0.0;
+          ai[t] = aok ? a.y : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
