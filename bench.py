@@ -63,6 +63,114 @@ def cpu_baseline(n_sample, zk, eps, threads):
     return 2 * n_sample / dt / 1e6, dt
 
 
+class DeviceProblem:
+    """Inputs resident in HBM in the reference layouts (points (2, n) as an
+    (n, 2) row-major array, strengths (nd, n), dipole vectors (nd, 2, n)) and
+    output buffers, for the device entry points of the library."""
+
+    def __init__(self, lib, src, trg, ch, dp, dv):
+        import torch
+        dev = torch.device("cuda:0")
+        self.lib, self.n, self.nt = lib, src.shape[1], trg.shape[1]
+        self.src = torch.from_numpy(np.ascontiguousarray(src.T)).to(dev)
+        self.trg = torch.from_numpy(np.ascontiguousarray(trg.T)).to(dev)
+        self.ch = torch.from_numpy(ch.astype(np.complex128)).to(dev)
+        self.dp = torch.from_numpy(dp.astype(np.complex128)).to(dev)
+        self.dv = torch.from_numpy(np.ascontiguousarray(np.transpose(dv, (2, 1, 0)))).to(dev)
+        self.pot = torch.zeros(self.n, dtype=torch.complex128, device=dev)
+        self.grad = torch.zeros(2 * self.n, dtype=torch.complex128, device=dev)
+        self.pott = torch.zeros(self.nt, dtype=torch.complex128, device=dev)
+        self.gradt = torch.zeros(2 * self.nt, dtype=torch.complex128, device=dev)
+        self.stream = torch.cuda.current_stream(dev)
+
+    def solve(self, kernel, eps, zk, ndiv, ti):
+        """kernel "h": hfmm2d, "l": lfmm2d; charges + dipoles, pot + grad at
+        sources and targets."""
+        vp, ier = C.c_void_p, C.c_int(0)
+        d = lambda t: vp(t.data_ptr())
+        if kernel == "h":
+            self.lib.fmm2dhip_hfmm2d_dev(
+                1, C.c_double(eps), C.c_double(zk), C.c_double(0.0), self.n, d(self.src), 1,
+                d(self.ch), 1, d(self.dp), d(self.dv), 2, d(self.pot), d(self.grad), None,
+                self.nt, d(self.trg), 2, d(self.pott), d(self.gradt), None, ndiv, 0, 1,
+                ti.ctypes.data_as(vp), vp(self.stream.cuda_stream), C.byref(ier))
+        else:
+            self.lib.fmm2dhip_lfmm2d_dev(
+                1, C.c_double(eps), self.n, d(self.src), 1, d(self.ch), 1, d(self.dp),
+                d(self.dv), 2, d(self.pot), d(self.grad), None, self.nt, d(self.trg), 2,
+                d(self.pott), d(self.gradt), None, ndiv, 0, 1, ti.ctypes.data_as(vp),
+                vp(self.stream.cuda_stream), C.byref(ier))
+        if ier.value != 0:
+            raise RuntimeError(f"{kernel}fmm2d returned ier={ier.value}")
+
+    def outputs(self):
+        return (self.pot.cpu().numpy(), self.grad.cpu().numpy().reshape(self.n, 2).T,
+                self.pott.cpu().numpy(), self.gradt.cpu().numpy().reshape(self.nt, 2).T)
+
+
+def direct_subsample(lib, kernel, zk, src, ch, dp, dv, pts):
+    """FP64 direct sum (the library's GPU l2d/h2d_directcdg_) of every source
+    at the points pts (2, m): pot (m,), grad (2, m)."""
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    ns, m = src.shape[1], pts.shape[1]
+    s = np.asfortranarray(src)
+    t = np.asfortranarray(pts)
+    c = np.asfortranarray(ch.reshape(1, ns))
+    q = np.asfortranarray(dp.reshape(1, ns))
+    v = np.asfortranarray(dv)
+    pot = np.zeros((1, m), complex, order="F")
+    grad = np.zeros((1, 2, m), complex, order="F")
+    nd = C.byref(C.c_int32(1))
+    if kernel == "h":
+        lib.h2d_directcdg_(nd, (C.c_double * 2)(zk, 0.0), P(s), C.byref(C.c_int32(ns)), P(c),
+                           P(q), P(v), P(t), C.byref(C.c_int32(m)), P(pot), P(grad),
+                           C.byref(C.c_double(1e-14)))
+    else:
+        lib.l2d_directcdg_(nd, P(s), C.byref(C.c_int32(ns)), P(c), P(q), P(v), P(t),
+                           C.byref(C.c_int32(m)), P(pot), P(grad), C.byref(C.c_double(1e-14)))
+    return pot[0], grad[0]
+
+
+def mixed_vs_fp64(lib, prob, src, trg, ch, dp, dv, zk, eps=1e-6, steps=5, nsub=10_000):
+    """The same points at eps = 1e-6 with the near field and M2L in FP64 and in
+    mixed precision: Mpts/s of each and relative l2 errors against FP64 direct
+    sums at nsub sources and nsub targets (fixed random subsample)."""
+    import torch
+    import fmm2d_c_translation_amd as F
+    rng = np.random.default_rng(99)
+    i_s = np.sort(rng.choice(prob.n, nsub, replace=False))
+    i_t = np.sort(rng.choice(prob.nt, nsub, replace=False))
+    out = {"eps": eps, "steps": steps,
+           "error_reference": f"FP64 direct sum at {nsub} sources + {nsub} targets"}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    for kernel, name, ndiv in (("h", "hfmm2d zk=40", 15), ("l", "lfmm2d", 15)):
+        ds, dgs = direct_subsample(lib, kernel, zk, src, ch, dp, dv, src[:, i_s])
+        dt, dgt = direct_subsample(lib, kernel, zk, src, ch, dp, dv, trg[:, i_t])
+        res = {}
+        for mode, label in ((F.FP64, "fp64"), (F.MIXED, "mixed")):
+            ti = np.zeros(8)
+            with F.precision(mode):
+                prob.solve(kernel, eps, zk, ndiv, ti)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    prob.solve(kernel, eps, zk, ndiv, ti)
+                torch.cuda.synchronize()
+                dt_s = (time.perf_counter() - t0) / steps
+            p, g, pt, gt = prob.outputs()
+            res[label] = {
+                "mpts_s": round((prob.n + prob.nt) / dt_s / 1e6, 3),
+                "ms_per_solve": round(dt_s * 1e3, 3),
+                "step_ms": {k: round(float(v) * 1e3, 3) for k, v in zip(
+                    ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"], ti)},
+                "rel_l2_err": {"pot": float(f"{max(rel(p[i_s], ds), rel(pt[i_t], dt)):.3e}"),
+                               "grad": float(f"{max(rel(g[:, i_s], dgs), rel(gt[:, i_t], dgt)):.3e}")},
+            }
+        res["speedup"] = round(res["mixed"]["mpts_s"] / res["fp64"]["mpts_s"], 3)
+        out[name] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +179,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the eps = 1e-6 mixed-precision vs FP64 comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter "
                          "calibration: FETCH_SIZE/WRITE_SIZE of a kernel of known traffic)")
@@ -99,32 +209,12 @@ def main():
     zk, eps = 40.0, 1e-9
     n = nt = args.n
     src, trg, ch, dp, dv = make_inputs(n, nt)
-    dev = torch.device("cuda:0")
-    # device-resident inputs, reference layouts (points (2,n), strengths (nd,n))
-    d_src = torch.from_numpy(np.ascontiguousarray(src.T)).to(dev)
-    d_trg = torch.from_numpy(np.ascontiguousarray(trg.T)).to(dev)
-    d_ch = torch.from_numpy(ch.astype(np.complex128)).to(dev)
-    d_dp = torch.from_numpy(dp.astype(np.complex128)).to(dev)
-    d_dv = torch.from_numpy(np.ascontiguousarray(np.transpose(dv, (2, 1, 0)))).to(dev)
-    d_pot = torch.zeros(n, dtype=torch.complex128, device=dev)
-    d_grad = torch.zeros(2 * n, dtype=torch.complex128, device=dev)
-    d_pott = torch.zeros(nt, dtype=torch.complex128, device=dev)
-    d_gradt = torch.zeros(2 * nt, dtype=torch.complex128, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    prob = DeviceProblem(lib, src, trg, ch, dp, dv)
     ndiv = 20  # hndiv2d for eps = 1e-9 (c_translation/src/hndiv2d.c:31-47)
     ti = np.zeros(8)
-    ier = C.c_int(0)
-    vp = C.c_void_p
 
     def solve():
-        lib.fmm2dhip_hfmm2d_dev(1, C.c_double(eps), C.c_double(zk), C.c_double(0.0), n,
-                                vp(d_src.data_ptr()), 1, vp(d_ch.data_ptr()), 1,
-                                vp(d_dp.data_ptr()), vp(d_dv.data_ptr()), 2, vp(d_pot.data_ptr()),
-                                vp(d_grad.data_ptr()), None, nt, vp(d_trg.data_ptr()), 2,
-                                vp(d_pott.data_ptr()), vp(d_gradt.data_ptr()), None, ndiv, 0, 1,
-                                ti.ctypes.data_as(vp), vp(stream.cuda_stream), C.byref(ier))
-        if ier.value != 0:
-            raise RuntimeError(f"hfmm2d returned ier={ier.value}")
+        prob.solve("h", eps, zk, ndiv, ti)
 
     for _ in range(args.warmup):
         solve()
@@ -140,6 +230,7 @@ def main():
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     stats = np.zeros(23)
+    vp = C.c_void_p
     lib.fmm2dhip_last_stats(stats.ctypes.data_as(vp), 23)
     ms = total / args.steps * 1e3
     value = (n + nt) / (total / args.steps) / 1e6
@@ -177,6 +268,8 @@ def main():
                      "m2l_tflops": None if m2l_tf is None else round(m2l_tf, 3)},
         "cpu_baseline": None,
     }
+    if not args.no_mixed:
+        out["mixed_vs_fp64"] = mixed_vs_fp64(lib, prob, src, trg, ch, dp, dv, zk)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v, dt = cpu_baseline(args.cpu_sample, zk, eps, threads)

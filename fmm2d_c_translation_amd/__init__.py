@@ -46,6 +46,27 @@ def lib():
     return _lib
 
 
+FP64, MIXED = 0, 1
+
+
+class precision:
+    """Context manager: arithmetic of the near-field and M2L steps for the
+    solves this thread makes inside the block.  ``FP64`` (the default) or
+    ``MIXED`` (FP32 near field and M2L when eps >= 1e-6, results within eps
+    of FP64 direct sums; tighter eps stays FP64)."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = lib().fmm2dhip_set_precision(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().fmm2dhip_set_precision(self.prev)
+        return False
+
+
 def device_name():
     buf = _C.create_string_buffer(64)
     rc = lib().fmm2dhip_device_info(buf, 64)

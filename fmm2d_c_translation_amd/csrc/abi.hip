@@ -118,6 +118,7 @@ int run_lap(const LapReq& r) {
   a.ifpgh = r.pg;
   a.ifpghtarg = r.pgt;
   a.ifnear = r.ifnear;
+  a.mixed = mixed_for(r.eps);
   double2* cs = r.ifc ? S.get<double2>((size_t)ndc * ns) : nullptr;
   double2* qs = r.ifd ? S.get<double2>((size_t)ndc * ns) : nullptr;
   if (S.err) return S.err;
@@ -233,6 +234,7 @@ int run_helm(const HelmReq& r) {
   HelmArgs a;
   a.nd = nd; a.eps = r.eps; a.zk = r.zk;
   a.ifcharge = r.ifc; a.ifdipole = r.ifd; a.ifpgh = r.pg; a.ifpghtarg = r.pgt; a.ifnear = r.ifnear;
+  a.mixed = mixed_for(r.eps);
   double2* cs = r.ifc ? S.get<double2>((size_t)nd * ns) : nullptr;
   double2* qs = r.ifd ? S.get<double2>((size_t)nd * ns) : nullptr;
   double* vs = r.ifd ? S.get<double>(2 * (size_t)nd * ns) : nullptr;
@@ -649,6 +651,15 @@ int fmm2dhip_test_devtree(const double* src, int ns, const double* targ, int nt,
 }
 
 void fmm2dhip_release_workspace(void) { fmm::release_workspace(); }
+
+int fmm2dhip_set_precision(int mode) {
+  int& m = fmm::thread_precision();
+  const int old = m;
+  if (mode == fmm::PREC_FP64 || mode == fmm::PREC_MIXED) m = mode;
+  return old;
+}
+
+int fmm2dhip_get_precision(void) { return fmm::thread_precision(); }
 
 int fmm2dhip_last_stats(double* out, int n) {
   const fmm::SolveStats& s = fmm::last_stats();

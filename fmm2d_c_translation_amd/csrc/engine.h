@@ -133,6 +133,16 @@ StepTimer* step_timer();       // per thread, reset
 StepTimer* step_timer_peek();  // per thread, as left by the last solve
 void finish_timer(double* timeinfo);  // after the stream is synchronised
 
+// Arithmetic of the list-1 and M2L steps.  PREC_FP64 keeps every step in
+// FP64.  PREC_MIXED runs those two steps in FP32 whenever eps >= 1e-6
+// (operands rounded from FP64 after the point differences are taken in FP64,
+// sums carried into FP64 accumulators); tighter eps stays FP64.  The mode is
+// per thread; its initial value is PREC_MIXED when the environment variable
+// FMM2DHIP_PRECISION is "mixed", else PREC_FP64.
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+int& thread_precision();
+inline bool mixed_for(double eps) { return thread_precision() == PREC_MIXED && eps >= 1e-6; }
+
 // Work counts of the last solve on this thread (for roofline reporting).
 struct SolveStats {
   double p2p_pairs_src = 0, p2p_pairs_trg = 0;  // sum over targets of list-1 sources
@@ -168,6 +178,7 @@ struct LapArgs {
   int nd = 1;
   double eps = 1e-6;
   int ifcharge = 0, ifdipole = 0, ifpgh = 0, ifpghtarg = 0, ifnear = 1;
+  bool mixed = false;                                 // FP32 list 1 and M2L
   const double2 *chg = nullptr, *dip = nullptr;      // sorted, channel-major
   double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources
   double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
@@ -181,6 +192,7 @@ struct HelmArgs {
   double eps = 1e-6;
   std::complex<double> zk = 1.0;
   int ifcharge = 0, ifdipole = 0, ifpgh = 0, ifpghtarg = 0, ifnear = 1;
+  bool mixed = false;                                 // FP32 M2L
   const double2 *chg = nullptr, *dip = nullptr;
   const double* dipvec = nullptr;                            // [2][nd][ns]
   double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources

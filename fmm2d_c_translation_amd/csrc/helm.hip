@@ -934,6 +934,13 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     HIP_TRY(hipMemcpyAsync(djobs, jobs.data(), sizeof(OpJob) * jobs.size(), hipMemcpyHostToDevice, c.st));
     k_hops<<<(int)jobs.size(), 256, sizeof(double2) * (vmax + 8), c.st>>>(djobs, zk, dT);
   }
+  // FP32 copy of the operators for the mixed-precision M2L
+  float2* dT32 = nullptr;
+  if (a.mixed && toff) {
+    dT32 = S.get<float2>(toff);
+    if (S.err) return S.err;
+    k_to_f32<<<(unsigned)((toff + 255) / 256), 256, 0, c.st>>>(dT, dT32, toff);
+  }
   // M2M/L2L slot tables are the 4 quadrant matrices in a row; M2L slots are
   // addressed through a per-level array of 49 offsets
   const int* W = plan.dwork;
@@ -1013,8 +1020,12 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
     k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, bs[l], map);
-    HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
-                       per, lo, per, nd, 0.0));
+    if (dT32)
+      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0],
+                               dT32 + tab[l].m2l[0], 2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
+    else
+      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
+                         per, lo, per, nd, 0.0));
   }
   tm->mark(c.st);
   // step 5: L2L

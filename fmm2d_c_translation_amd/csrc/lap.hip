@@ -51,6 +51,7 @@ struct LapTables {
   cd* m2m = nullptr;  // 4 child quadrants
   cd* l2l = nullptr;  // 4 child quadrants
   cd* m2l = nullptr;  // 49 offsets (dy+3)*7 + (dx+3), |dx|,|dy| <= 3
+  float2* m2l32 = nullptr;  // the M2L operators rounded to FP32
 };
 
 std::vector<double> binomials(int n) {
@@ -116,200 +117,251 @@ void make_tables(int p, std::vector<cd>& h) {
 // device-resident tables, built once per (device, order)
 int get_tables(Ctx& c, int p, LapTables& t) {
   static std::mutex mu;
-  static std::map<std::pair<int, int>, cd*> cache;
+  static std::map<std::pair<int, int>, std::pair<cd*, float2*>> cache;
   std::lock_guard<std::mutex> g(mu);
   auto key = std::make_pair(c.device, p);
   auto it = cache.find(key);
   cd* d = nullptr;
+  float2* d32 = nullptr;
   const size_t N2 = (size_t)(p + 1) * (p + 1);
   if (it == cache.end()) {
     std::vector<cd> h;
     make_tables(p, h);
+    std::vector<float2> h32(49 * N2);
+    for (size_t e = 0; e < h32.size(); ++e)
+      h32[e] = make_float2((float)h[8 * N2 + e].x, (float)h[8 * N2 + e].y);
     if (hipMalloc(&d, sizeof(cd) * h.size()) != hipSuccess) return IER_ALLOC;
-    if (hipMemcpy(d, h.data(), sizeof(cd) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMalloc(&d32, sizeof(float2) * h32.size()) != hipSuccess) {
+      (void)hipFree(d);
+      return IER_ALLOC;
+    }
+    if (hipMemcpy(d, h.data(), sizeof(cd) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d32, h32.data(), sizeof(float2) * h32.size(), hipMemcpyHostToDevice) != hipSuccess)
       return IER_HIP;
-    cache[key] = d;
+    cache[key] = {d, d32};
   } else {
-    d = it->second;
+    d = it->second.first;
+    d32 = it->second.second;
   }
   t.m2m = d;
   t.l2l = d + 4 * N2;
   t.m2l = d + 8 * N2;
+  t.m2l32 = d32;
   return IER_OK;
 }
 
 // ---------------------------------------------------------------------
-// form multipole (leaf sources) / local (list-4 sources)
-// Wave per box, sources staged in LDS in chunks of CH; lane n owns
-// coefficient n (looping when p >= 64).
+// Per-point kernels work in 16-lane groups, four boxes per wave: lane u of a
+// group takes points u, u + 16, ... of its box.  Expansion coefficients of
+// a box are read by every lane of the group (one cache line per request).
 // ---------------------------------------------------------------------
+__device__ inline void addc(double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); }
+__device__ inline cd ldc(const double2* p) { const double2 v = *p; return mk(v.x, v.y); }
+
+// sum over the 16 lanes of a group; every lane receives the sum
+__device__ inline cd gsum16(cd v) {
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 16);
+    v.y += __shfl_xor(v.y, o, 16);
+  }
+  return v;
+}
+
+// Leaf multipoles (MP) from the leaf's sources, or local expansions from
+// the sources of the list-4 boxes (!MP).  Lane u forms its point's term of
+// coefficient n, the group sums it, and lane n % 16 keeps it (p <= 63):
+//   multipole (laprouts2d.c:41-182):  mp_0 += c,
+//     mp_n += -c (z/r)^n / n + d z^(n-1) / r^n
+//   local (laprouts2d.c:358-500):     lo_0 += c log|z| - d / z,
+//     lo_n += -c (r/z)^n / n - d (r/z)^n / z
+// with z = source - centre and r the box size.
 template <bool MP>
-__global__ void __launch_bounds__(WAVE) k_form(const int* work, DTree D, const double2* pts,
-                                               const double2* chg, const double2* dip, int N,
-                                               int nd, int p, int CH, double2* out) {
-  extern __shared__ double2 sm[];
-  const int b = work[blockIdx.x];
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree D,
+                                               const double2* pts, const double2* chg,
+                                               const double2* dip, int N, int nd, int p,
+                                               double2* out) {
+  const int u = threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 4);
+  if (w >= nwork) return;  // whole groups leave; sums never cross groups
+  const int b = work[w];
   const int P = p + 1;
-  double2* Wc = sm;                    // [P][CH]
-  double2* Wd = sm + (size_t)P * CH;   // [P][CH]
   const double2 ctr = D.ctr[b];
-  const double r = D.bs[D.level[b]];
-  // source boxes: the box itself (multipole) or its list-4 entries (local)
-  int nsrcbox = MP ? 1 : D.o4[b + 1] - D.o4[b];
-  for (int sb = 0; sb < nsrcbox; ++sb) {
-    int jb = MP ? b : D.l4[D.o4[b] + sb];
-    int2 rg = D.srng[jb];
-    for (int c0 = rg.x; c0 < rg.y; c0 += CH) {
-      int cnt = min(CH, rg.y - c0);
-      for (int j = lane; j < cnt; j += WAVE) {
-        double2 s = pts[c0 + j];
-        cd z = mk(s.x - ctr.x, s.y - ctr.y);
+  const double r = D.bs[D.level[b]], ri = 1.0 / r;
+  const int nsb = MP ? 1 : D.o4[b + 1] - D.o4[b];
+  const int l4 = MP ? 0 : D.o4[b];
+  for (int d = 0; d < nd; ++d) {
+    cd acc[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+    for (int sb = 0; sb < nsb; ++sb) {
+      const int jb = MP ? b : D.l4[l4 + sb];
+      const int2 rg = D.srng[jb];
+      for (int c0 = rg.x; c0 < rg.y; c0 += 16) {
+        const int j = c0 + u;
+        const bool ok = j < rg.y;
+        const double2 s = pts[ok ? j : rg.x];
+        const cd c = ok && chg ? ldc(&chg[(size_t)d * N + j]) : mk(0, 0);
+        const cd q = ok && dip ? ldc(&dip[(size_t)d * N + j]) : mk(0, 0);
+        const cd z = mk(s.x - ctr.x, s.y - ctr.y);
+        // running powers: MP w = (z/r)^n, qw = d z^(n-1) / r^n;
+        // local w = (r/z)^n, qw = -d (r/z)^n / z
+        cd w0, step, qw;
+        cd term0;
         if (MP) {
-          // -(z/r)^n / n for charges, z^(n-1)/r^n for dipoles (laprouts2d.c:41-182)
-          cd zr = z / r, pc = mk(-1.0, 0.0), pd = mk(1.0 / r, 0.0);
-          Wc[j] = make_double2(1.0, 0.0);
-          Wd[j] = make_double2(0.0, 0.0);
-          for (int n = 1; n <= p; ++n) {
-            pc = pc * zr;
-            cd v = pc / (double)n;
-            Wc[(size_t)n * CH + j] = make_double2(v.x, v.y);
-            Wd[(size_t)n * CH + j] = make_double2(pd.x, pd.y);
-            pd = pd * zr;
-          }
+          w0 = mk(1, 0);
+          step = z * ri;
+          qw = q * ri;
+          term0 = c;
         } else {
-          // log|z0|, -(r/z0)^n / n ; -r^n / z0^(n+1)  (laprouts2d.c:358-500)
-          cd t = r * inv(z), pc = mk(-1.0, 0.0), pd = -inv(z);
-          Wc[j] = make_double2(0.5 * log(norm2(z)), 0.0);
-          Wd[j] = make_double2(pd.x, pd.y);
-          for (int n = 1; n <= p; ++n) {
-            pc = pc * t;
-            cd v = pc / (double)n;
-            pd = pd * t;
-            Wc[(size_t)n * CH + j] = make_double2(v.x, v.y);
-            Wd[(size_t)n * CH + j] = make_double2(pd.x, pd.y);
+          const cd zi = inv(z);
+          w0 = mk(1, 0);
+          step = r * zi;
+          qw = -(q * zi);
+          term0 = c * (0.5 * log(norm2(z))) + qw;
+        }
+        cd pw = w0;
+#pragma unroll
+        for (int hi = 0; hi < 4; ++hi) {
+          if (16 * hi > p) break;
+          for (int lo = 0; lo < 16; ++lo) {
+            const int n = 16 * hi + lo;
+            if (n > p) break;
+            cd t;
+            if (n == 0) {
+              t = term0;
+            } else {
+              pw = pw * step;
+              const double inv_n = 1.0 / n;  // uniform over the wave
+              if (MP) {
+                t = -(c * pw) * inv_n + qw;
+                qw = qw * step;
+              } else {
+                qw = qw * step;
+                t = -(c * pw) * inv_n + qw;
+              }
+            }
+            t = gsum16(t);
+            if (lo == u) acc[hi] += t;
           }
         }
       }
-      __syncthreads();
-      for (int d = 0; d < nd; ++d) {
-        for (int n = lane; n <= p; n += WAVE) {
-          cd acc = mk(0, 0);
-          for (int j = 0; j < cnt; ++j) {
-            if (chg) {
-              double2 cc = chg[(size_t)d * N + c0 + j], w = Wc[(size_t)n * CH + j];
-              acc += mk(cc.x, cc.y) * mk(w.x, w.y);
-            }
-            if (dip && (n > 0 || !MP)) {
-              double2 qq = dip[(size_t)d * N + c0 + j], w = Wd[(size_t)n * CH + j];
-              acc += mk(qq.x, qq.y) * mk(w.x, w.y);
-            }
-          }
-          double2* o = out + ((size_t)b * nd + d) * P + n;
-          *o = make_double2(o->x + acc.x, o->y + acc.y);
-        }
-      }
-      __syncthreads();
+    }
+    double2* o = out + ((size_t)b * nd + d) * P;
+#pragma unroll
+    for (int hi = 0; hi < 4; ++hi) {
+      const int n = 16 * hi + u;
+      if (n <= p) addc(o + n, acc[hi]);
     }
   }
 }
 
-// ---------------------------------------------------------------------
-// evaluation at points: wave per box, lane per point
-// ---------------------------------------------------------------------
-__device__ inline void addc(double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); }
-
-// list 3: multipoles of finer far boxes at this box's points (laprouts2d.c:193-344)
-__global__ void __launch_bounds__(WAVE) k_eval_mp(const int* work, DTree D, const int2* rng,
-                                                  const double2* pts, int N, int nd, int p,
-                                                  int pg, const double2* mp, double2* pot,
+// list 3: multipoles of finer far boxes at this box's points
+// (laprouts2d.c:193-344): pot += m_0 log|z| + sum m_n t^n, t = r/z;
+// grad = (m_0 - sum n m_n t^n) / z; hess = (sum n(n+1) m_n t^n - m_0) / z^2
+template <int PG>
+__global__ void __launch_bounds__(WAVE) k_eval_mp(const int* work, int nwork, DTree D,
+                                                  const int2* rng, const double2* pts, int N,
+                                                  int nd, int p, const double2* mp, double2* pot,
                                                   double2* grad, double2* hess) {
-  const int b = work[blockIdx.x];
+  const int u = threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 4);
+  if (w >= nwork) return;
+  const int b = work[w];
   const int P = p + 1;
-  int2 rg = rng[b];
-  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += WAVE) {
-    double2 x = pts[i];
-    for (int k = D.o3[b]; k < D.o3[b + 1]; ++k) {
-      int s = D.l3[k];
-      double2 c = D.ctr[s];
-      double r = D.bs[D.level[s]];
-      cd z = mk(x.x - c.x, x.y - c.y);
-      cd zi = inv(z), t = r * zi;
-      double lz = 0.5 * log(norm2(z));
-      for (int d = 0; d < nd; ++d) {
+  const int2 rg = rng[b];
+  const int k0 = D.o3[b], k1 = D.o3[b + 1];
+  for (int i = rg.x + u; i < rg.y; i += 16) {
+    const double2 x = pts[i];
+    for (int d = 0; d < nd; ++d) {
+      cd sp = mk(0, 0), sg = mk(0, 0), sh = mk(0, 0);
+      for (int k = k0; k < k1; ++k) {
+        const int s = D.l3[k];
+        const double2 c = D.ctr[s];
+        const double r = D.bs[D.level[s]];
+        const cd z = mk(x.x - c.x, x.y - c.y);
+        const cd zi = inv(z), t = r * zi;
         const double2* m = mp + ((size_t)s * nd + d) * P;
-        cd m0 = mk(m[0].x, m[0].y);
+        const cd m0 = ldc(m);
         cd s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0), tn = mk(1, 0);
         for (int n = 1; n <= p; ++n) {
           tn = tn * t;
-          cd v = mk(m[n].x, m[n].y) * tn;
+          const cd v = ldc(m + n) * tn;
           s1 += v;
-          s2 += v * (double)n;
-          s3 += v * (double)(n * (n + 1));
+          if (PG >= 2) s2 += v * (double)n;
+          if (PG >= 3) s3 += v * (double)(n * (n + 1));
         }
-        addc(pot + (size_t)d * N + i, m0 * lz + s1);
-        if (pg >= 2) addc(grad + (size_t)d * N + i, (m0 - s2) * zi);
-        if (pg >= 3) addc(hess + (size_t)d * N + i, (s3 - m0) * (zi * zi));
+        sp += m0 * (0.5 * log(norm2(z))) + s1;
+        if (PG >= 2) sg += (m0 - s2) * zi;
+        if (PG >= 3) sh += (s3 - m0) * (zi * zi);
       }
+      addc(pot + (size_t)d * N + i, sp);
+      if (PG >= 2) addc(grad + (size_t)d * N + i, sg);
+      if (PG >= 3) addc(hess + (size_t)d * N + i, sh);
     }
   }
 }
 
-// leaf local expansions at the leaf's points (laprouts2d.c:508-642)
-__global__ void __launch_bounds__(WAVE) k_eval_lo(const int* work, DTree D, const int2* rng,
-                                                  const double2* pts, int N, int nd, int p,
-                                                  int pg, const double2* lo, double2* pot,
+// leaf local expansions at the leaf's points (laprouts2d.c:508-642), Horner
+// from the top: S = sum L_n w^n, G = sum n L_n w^(n-1), H = sum n(n-1) L_n
+// w^(n-2), w = (x - c)/r; grad = G / r, hess = H / r^2
+template <int PG>
+__global__ void __launch_bounds__(WAVE) k_eval_lo(const int* work, int nwork, DTree D,
+                                                  const int2* rng, const double2* pts, int N,
+                                                  int nd, int p, const double2* lo, double2* pot,
                                                   double2* grad, double2* hess) {
-  const int b = work[blockIdx.x];
+  const int u = threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 4);
+  if (w >= nwork) return;
+  const int b = work[w];
   const int P = p + 1;
-  int2 rg = rng[b];
+  const int2 rg = rng[b];
   const double2 c = D.ctr[b];
   const double r = D.bs[D.level[b]], ri = 1.0 / r;
-  for (int i = rg.x + (int)threadIdx.x; i < rg.y; i += WAVE) {
-    double2 x = pts[i];
-    cd z = mk((x.x - c.x) / r, (x.y - c.y) / r);
+  for (int i = rg.x + u; i < rg.y; i += 16) {
+    const double2 x = pts[i];
+    const cd z = mk((x.x - c.x) * ri, (x.y - c.y) * ri);
     for (int d = 0; d < nd; ++d) {
       const double2* L = lo + ((size_t)b * nd + d) * P;
-      // Horner from the top: S = sum L_n z^n, G = sum n L_n z^(n-1), H = sum n(n-1) L_n z^(n-2)
       cd S = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
       for (int n = p; n >= 0; --n) {
-        H = H * z + G * 2.0;
-        G = G * z + S;
-        S = S * z + mk(L[n].x, L[n].y);
+        if (PG >= 3) H = H * z + G * 2.0;
+        if (PG >= 2) G = G * z + S;
+        S = S * z + ldc(L + n);
       }
       addc(pot + (size_t)d * N + i, S);
-      if (pg >= 2) addc(grad + (size_t)d * N + i, G * ri);
-      if (pg >= 3) addc(hess + (size_t)d * N + i, H * (ri * ri));
+      if (PG >= 2) addc(grad + (size_t)d * N + i, G * ri);
+      if (PG >= 3) addc(hess + (size_t)d * N + i, H * (ri * ri));
     }
   }
 }
 
+template <int PG>
+void launch_eval(bool mpk, int nw, hipStream_t st, const int* work, const DTree& D,
+                 const int2* rng, const double2* pts, int N, int nd, int p, const double2* ex,
+                 double2* pot, double2* grad, double2* hess) {
+  const int nbx = (nw + 3) / 4;
+  if (mpk)
+    k_eval_mp<PG><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
+  else
+    k_eval_lo<PG><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
+}
+
 // ---------------------------------------------------------------------
-// list-1 near field.  Wave per target box; list-1 sources are staged in LDS
-// (coalesced loads), lanes are (target, source slice) pairs so small leaves
-// still fill the wave, slices are reduced through LDS at the end.
+// list-1 near field.  Wave per target box, one channel at a time; list-1
+// sources are staged in LDS LCAP at a time (coalesced loads), lanes are
+// (target, source slice) pairs so small leaves still fill the wave, and the
+// slices are reduced through LDS at the end.
 // MODE: skip test 0: rr <= th^2, 1: rr < th^2 (cauchykernels2d.c:10-14).
 // ---------------------------------------------------------------------
-constexpr int P2P_CAP = 256;
+constexpr int LCAP = 128;
 
-template <bool C, bool Q, int PG, int MODE>
-__global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const double2* src,
-                                              const double2* chg, const double2* dip, int NS,
-                                              const int2* trng, const double2* tpts, int NT,
-                                              int nd, double th2, double2* pot, double2* grad,
-                                              double2* hess) {
-  __shared__ double2 sx[P2P_CAP];
-  __shared__ double2 sc[2][P2P_CAP];
-  __shared__ double2 sq[2][P2P_CAP];
-  __shared__ int boff[14], bsrc[14];
-  const int b = work[blockIdx.x];
-  const int lane = threadIdx.x;
+// source boxes of list 1 as a concatenated range: boff[k] = first slot of
+// box k, bsrc[k] = its first sorted source; returns the total
+__device__ inline int list1_ranges(const DTree& D, int b, int* boff, int* bsrc) {
   const int n1 = D.o1[b + 1] - D.o1[b];
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     int acc = 0;
     for (int k = 0; k < n1; ++k) {
-      int jb = D.l1[D.o1[b] + k];
-      int2 r = D.srng[jb];
+      const int2 r = D.srng[D.l1[D.o1[b] + k]];
       boff[k] = acc;
       bsrc[k] = r.x;
       acc += r.y - r.x;
@@ -317,7 +369,31 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
     boff[n1] = acc;
   }
   __syncthreads();
-  const int S = boff[n1];
+  return boff[n1];
+}
+__device__ inline int list1_source(const int* boff, const int* bsrc, int g) {
+  int kb = 0;
+  while (boff[kb + 1] <= g) ++kb;
+  return bsrc[kb] + (g - boff[kb]);
+}
+
+template <bool C, bool Q, int PG, int MODE>
+__global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const double2* src,
+                                              const double2* chg, const double2* dip, int NS,
+                                              const int2* trng, const double2* tpts, int NT,
+                                              int nd, double th2, double2* pot, double2* grad,
+                                              double2* hess) {
+  // staging (x, charge, dipole) and the slice reduction share one buffer
+  __shared__ double2 lds[3 * LCAP];
+  double2* sx = lds;
+  double2* sc = lds + LCAP;
+  double2* sq = lds + 2 * LCAP;
+  double2* red = lds;
+  static_assert(3 * WAVE <= 3 * LCAP, "reduction buffer");
+  __shared__ int boff[14], bsrc[14];
+  const int b = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int S = list1_ranges(D, b, boff, bsrc);
   const int2 tr = trng[b];
   for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
     const int Tc = min(WAVE, tr.y - t0);
@@ -325,74 +401,190 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
     const int t = lane / K, k = lane % K;
     const bool act = t < Tc;
     const double2 x = act ? tpts[t0 + t] : make_double2(0, 0);
-    for (int d0 = 0; d0 < nd; d0 += 2) {
-      const int nch = min(2, nd - d0);
-      cd ap[2] = {mk(0, 0), mk(0, 0)}, ag[2] = {mk(0, 0), mk(0, 0)}, ah[2] = {mk(0, 0), mk(0, 0)};
-      for (int g0 = 0; g0 < S; g0 += P2P_CAP) {
-        const int cnt = min(P2P_CAP, S - g0);
+    for (int d = 0; d < nd; ++d) {
+      cd ap = mk(0, 0), ag = mk(0, 0), ah = mk(0, 0);
+      for (int g0 = 0; g0 < S; g0 += LCAP) {
+        const int cnt = min(LCAP, S - g0);
         for (int m = lane; m < cnt; m += WAVE) {
-          int g = g0 + m, kb = 0;
-          while (boff[kb + 1] <= g) ++kb;
-          int si = bsrc[kb] + (g - boff[kb]);
+          const int si = list1_source(boff, bsrc, g0 + m);
           sx[m] = src[si];
-          for (int e = 0; e < nch; ++e) {
-            if (C) sc[e][m] = chg[(size_t)(d0 + e) * NS + si];
-            if (Q) sq[e][m] = dip[(size_t)(d0 + e) * NS + si];
-          }
+          if (C) sc[m] = chg[(size_t)d * NS + si];
+          if (Q) sq[m] = dip[(size_t)d * NS + si];
         }
         __syncthreads();
         if (act) {
           for (int m = k; m < cnt; m += K) {
-            double2 s = sx[m];
-            double dx = x.x - s.x, dy = x.y - s.y;
-            double rr = dx * dx + dy * dy;
+            const double2 s = sx[m];
+            const double dx = x.x - s.x, dy = x.y - s.y;
+            const double rr = dx * dx + dy * dy;
             if (MODE == 0 ? (rr <= th2) : (rr < th2)) continue;
-            double rinv = 1.0 / rr;
-            cd zi = mk(dx * rinv, -dy * rinv);
-            cd z2 = -(zi * zi);
-            double lr = C ? 0.5 * log(rr) : 0.0;
-            for (int e = 0; e < nch; ++e) {
-              if (C) {
-                cd cc = mk(sc[e][m].x, sc[e][m].y);
-                ap[e] += cc * lr;
-                if (PG >= 2) ag[e] += cc * zi;
-                if (PG >= 3) ah[e] += cc * z2;
-              }
-              if (Q) {
-                cd qq = mk(sq[e][m].x, sq[e][m].y);
-                ap[e] += qq * zi;
-                if (PG >= 2) ag[e] += qq * z2;
-                if (PG >= 3) ah[e] += qq * ((z2 * zi) * -2.0);
-              }
+            const double rinv = 1.0 / rr;
+            const cd zi = mk(dx * rinv, -dy * rinv);
+            const cd z2 = -(zi * zi);
+            if (C) {
+              const cd cc = mk(sc[m].x, sc[m].y);
+              ap += cc * (0.5 * log(rr));
+              if (PG >= 2) ag += cc * zi;
+              if (PG >= 3) ah += cc * z2;
+            }
+            if (Q) {
+              const cd qq = mk(sq[m].x, sq[m].y);
+              ap += qq * zi;
+              if (PG >= 2) ag += qq * z2;
+              if (PG >= 3) ah += qq * ((z2 * zi) * -2.0);
             }
           }
         }
         __syncthreads();
       }
-      // reduce the K slices of each target through LDS (reusing sc/sq)
-      for (int e = 0; e < nch; ++e) {
-        double2* red = &sc[0][0];
-        red[lane] = make_double2(ap[e].x, ap[e].y);
-        red[WAVE + lane] = make_double2(ag[e].x, ag[e].y);
-        red[2 * WAVE + lane] = make_double2(ah[e].x, ah[e].y);
-        __syncthreads();
-        if (act && k == 0) {
-          cd P = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
-          for (int u = 0; u < K; ++u) {
-            int l = t * K + u;
-            P += mk(red[l].x, red[l].y);
-            G += mk(red[WAVE + l].x, red[WAVE + l].y);
-            H += mk(red[2 * WAVE + l].x, red[2 * WAVE + l].y);
+      red[lane] = make_double2(ap.x, ap.y);
+      if (PG >= 2) red[WAVE + lane] = make_double2(ag.x, ag.y);
+      if (PG >= 3) red[2 * WAVE + lane] = make_double2(ah.x, ah.y);
+      __syncthreads();
+      if (act && k == 0) {
+        cd P = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
+        for (int u = 0; u < K; ++u) {
+          const int l = t * K + u;
+          P += mk(red[l].x, red[l].y);
+          if (PG >= 2) G += mk(red[WAVE + l].x, red[WAVE + l].y);
+          if (PG >= 3) H += mk(red[2 * WAVE + l].x, red[2 * WAVE + l].y);
+        }
+        const size_t o = (size_t)d * NT + t0 + t;
+        addc(pot + o, P);
+        if (PG >= 2) addc(grad + o, G);
+        if (PG >= 3) addc(hess + o, H);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Mixed-precision list-1 kernel (potential and gradient).  Coordinate
+// differences and the skip test are FP64, as in k_p2p; the pair terms are
+// then formed in FP32 from z scaled by sc = 2^-e (2^e near the root box
+// size, so |z| sc stays far from the FP32 range limits), and summed in FP32
+// per staged chunk of at most LCAP sources before the chunk sums, rescaled,
+// join FP64 accumulators.  Terms of each power of sc are summed apart:
+//   log|z| = log|z sc| - log sc,  1/z = sc / (z sc),  1/z^2 = sc^2 / (z sc)^2.
+struct cf {
+  float x, y;
+};
+__device__ inline cf cfm(cf a, cf b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline cf cfa(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+
+template <bool C, bool Q, int PG, int MODE>
+__global__ void __launch_bounds__(WAVE) k_p2p32(const int* work, DTree D, const double2* src,
+                                                const double2* chg, const double2* dip, int NS,
+                                                const int2* trng, const double2* tpts, int NT,
+                                                int nd, double th2, double sc, double2* pot,
+                                                double2* grad) {
+  // staging: x (FP64), charge and dipole strengths (FP32); the slice
+  // reduction reuses the x buffer
+  __shared__ double2 lds[2 * LCAP];
+  double2* sx = lds;
+  float2* sc32 = reinterpret_cast<float2*>(lds + LCAP);
+  float2* sq32 = sc32 + LCAP;
+  double2* red = lds;
+  static_assert(2 * WAVE <= LCAP, "reduction buffer");
+  __shared__ int boff[14], bsrc[14];
+  const int b = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int S = list1_ranges(D, b, boff, bsrc);
+  const int2 tr = trng[b];
+  const float lsc = (float)-log(sc);
+  const double sc2 = sc * sc;
+  for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
+    const int Tc = min(WAVE, tr.y - t0);
+    const int K = WAVE / Tc;
+    const int t = lane / K, k = lane % K;
+    const bool act = t < Tc;
+    const double2 x = act ? tpts[t0 + t] : make_double2(0, 0);
+    for (int d = 0; d < nd; ++d) {
+      cd ap = mk(0, 0), ag = mk(0, 0);
+      for (int g0 = 0; g0 < S; g0 += LCAP) {
+        const int cnt = min(LCAP, S - g0);
+        for (int m = lane; m < cnt; m += WAVE) {
+          const int si = list1_source(boff, bsrc, g0 + m);
+          sx[m] = src[si];
+          if (C) {
+            const double2 v = chg[(size_t)d * NS + si];
+            sc32[m] = make_float2((float)v.x, (float)v.y);
           }
-          size_t o = (size_t)(d0 + e) * NT + t0 + t;
-          addc(pot + o, P);
-          if (PG >= 2) addc(grad + o, G);
-          if (PG >= 3) addc(hess + o, H);
+          if (Q) {
+            const double2 v = dip[(size_t)d * NS + si];
+            sq32[m] = make_float2((float)v.x, (float)v.y);
+          }
+        }
+        __syncthreads();
+        if (act) {
+          // chunk sums: p0 (scale-free) and p1 (x sc) of the potential,
+          // g1 (x sc) and g2 (x sc^2) of the gradient
+          cf p0 = {0, 0}, p1 = {0, 0}, g1 = {0, 0}, g2 = {0, 0};
+          for (int m = k; m < cnt; m += K) {
+            const double2 s = sx[m];
+            const double dx = x.x - s.x, dy = x.y - s.y;
+            const double rr = dx * dx + dy * dy;
+            if (MODE == 0 ? (rr <= th2) : (rr < th2)) continue;
+            const float fx = (float)(dx * sc), fy = (float)(dy * sc);
+            const float frr = fx * fx + fy * fy;
+            const float ri = __builtin_amdgcn_rcpf(frr);
+            const cf zi = {fx * ri, -fy * ri};
+            if (C) {
+              const float lr = 0.5f * 0.693147180559945309f * __builtin_amdgcn_logf(frr) + lsc;
+              const float2 cc = sc32[m];
+              p0 = cfa(p0, {cc.x * lr, cc.y * lr});
+              if (PG >= 2) g1 = cfa(g1, cfm({cc.x, cc.y}, zi));
+            }
+            if (Q) {
+              const float2 qq = sq32[m];
+              p1 = cfa(p1, cfm({qq.x, qq.y}, zi));
+              if (PG >= 2) {
+                const cf z2 = {zi.y * zi.y - zi.x * zi.x, -2.0f * zi.x * zi.y};  // -1/z^2
+                g2 = cfa(g2, cfm({qq.x, qq.y}, z2));
+              }
+            }
+          }
+          ap += mk((double)p0.x + (double)p1.x * sc, (double)p0.y + (double)p1.y * sc);
+          if (PG >= 2)
+            ag += mk((double)g1.x * sc + (double)g2.x * sc2, (double)g1.y * sc + (double)g2.y * sc2);
         }
         __syncthreads();
       }
+      red[lane] = make_double2(ap.x, ap.y);
+      if (PG >= 2) red[WAVE + lane] = make_double2(ag.x, ag.y);
+      __syncthreads();
+      if (act && k == 0) {
+        cd P = mk(0, 0), G = mk(0, 0);
+        for (int u = 0; u < K; ++u) {
+          const int l = t * K + u;
+          P += mk(red[l].x, red[l].y);
+          if (PG >= 2) G += mk(red[WAVE + l].x, red[WAVE + l].y);
+        }
+        const size_t o = (size_t)d * NT + t0 + t;
+        addc(pot + o, P);
+        if (PG >= 2) addc(grad + o, G);
+      }
+      __syncthreads();
     }
   }
+}
+
+template <bool C, bool Q>
+void launch_p2p32(int pg, int mode, int nbx, hipStream_t st, const int* work, const DTree& D,
+                  const double2* src, const double2* chg, const double2* dip, int NS,
+                  const int2* trng, const double2* tpts, int NT, int nd, double th2, double sc,
+                  double2* pot, double2* grad) {
+#define FMM_P2P32(PGV, MV)                                                                       \
+  k_p2p32<C, Q, PGV, MV><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, \
+                                               sc, pot, grad)
+  if (pg == 1) {
+    if (mode == 0) FMM_P2P32(1, 0);
+    else FMM_P2P32(1, 1);
+  } else {
+    if (mode == 0) FMM_P2P32(2, 0);
+    else FMM_P2P32(2, 1);
+  }
+#undef FMM_P2P32
 }
 
 template <bool C, bool Q, int PG>
@@ -441,9 +633,6 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   const double2* dip = a.ifdipole ? a.dip : nullptr;
   if (!chg && !dip) return IER_OK;  // zero strengths: outputs stay zero
 
-  int CH = 64;
-  while (CH > 8 && (size_t)2 * P * CH * sizeof(double2) > 40960) CH /= 2;
-  size_t shm = (size_t)2 * P * CH * sizeof(double2);
   StepTimer* tm = step_timer();
   {
     SolveStats& st = last_stats();
@@ -454,8 +643,8 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   tm->mark(c.st);
   // step 1: leaf multipoles
   if (plan.formmp.n)
-    k_form<true><<<plan.formmp.n, WAVE, shm, c.st>>>(W + plan.formmp.off, D, Pt.src, chg, dip,
-                                                    Pt.ns, nd, p, CH, mp);
+    k_form<true><<<nblk(plan.formmp.n, 4), WAVE, 0, c.st>>>(W + plan.formmp.off, plan.formmp.n, D,
+                                                           Pt.src, chg, dip, Pt.ns, nd, p, mp);
   tm->mark(c.st);
   // step 3: upward pass
   int maxw = 1;
@@ -472,16 +661,20 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   tm->mark(c.st);
   // step 2: list 4
   if (plan.list4.n)
-    k_form<false><<<plan.list4.n, WAVE, shm, c.st>>>(W + plan.list4.off, D, Pt.src, chg, dip,
-                                                    Pt.ns, nd, p, CH, lo);
+    k_form<false><<<nblk(plan.list4.n, 4), WAVE, 0, c.st>>>(W + plan.list4.off, plan.list4.n, D,
+                                                           Pt.src, chg, dip, Pt.ns, nd, p, lo);
   tm->mark(c.st);
   // step 4: M2L level by level (log of the box size enters entry (0,0) only)
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
     k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, plan.bs[l], map);
-    HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, TM, P, mp, per, lo, per, nd,
-                       std::log(plan.bs[l])));
+    if (a.mixed)
+      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, TM, tab.m2l32, P, mp, per, lo, per,
+                               nd, std::log(plan.bs[l])));
+    else
+      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, TM, P, mp, per, lo, per, nd,
+                         std::log(plan.bs[l])));
   }
   tm->mark(c.st);
   // step 5: L2L into the children of boxes that hold targets
@@ -499,19 +692,34 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   // '<=' for potential only and '<' otherwise; dipole-only routines compare |z|
   const double thresh = Pt.bs0 * 0x1p-51;
   tm->mark(c.st);
+  auto eval = [&](bool mpk, const Set& s, const Span& sp, const double2* ex) {
+    if (s.pg == 1) launch_eval<1>(mpk, sp.n, c.st, W + sp.off, D, s.rng, s.pts, s.N, nd, p, ex, s.po, s.gr, s.he);
+    if (s.pg == 2) launch_eval<2>(mpk, sp.n, c.st, W + sp.off, D, s.rng, s.pts, s.N, nd, p, ex, s.po, s.gr, s.he);
+    if (s.pg == 3) launch_eval<3>(mpk, sp.n, c.st, W + sp.off, D, s.rng, s.pts, s.N, nd, p, ex, s.po, s.gr, s.he);
+  };
   for (auto& s : sets)
-    if (s.pg > 0 && s.N > 0 && s.l3.n)
-      k_eval_mp<<<s.l3.n, WAVE, 0, c.st>>>(W + s.l3.off, D, s.rng, s.pts, s.N, nd, p, s.pg, mp, s.po, s.gr, s.he);
+    if (s.pg > 0 && s.N > 0 && s.l3.n) eval(true, s, s.l3, mp);
   tm->mark(c.st);
   for (auto& s : sets)
-    if (s.pg > 0 && s.N > 0 && s.lv.n)
-      k_eval_lo<<<s.lv.n, WAVE, 0, c.st>>>(W + s.lv.off, D, s.rng, s.pts, s.N, nd, p, s.pg, lo, s.po, s.gr, s.he);
+    if (s.pg > 0 && s.N > 0 && s.lv.n) eval(false, s, s.lv, lo);
   tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0) continue;
     if (a.ifnear && s.nn.n) {
       int mode = s.pg == 1 ? 0 : 1;
       double th2 = thresh * thresh;
+      if (a.mixed && s.pg <= 2) {
+        // FP32 pair terms (the Hessian's 1/z^3 can leave the FP32 range,
+        // so pg = 3 stays FP64)
+        const double sc = Pt.bs0 > 0.0 ? std::ldexp(1.0, -std::ilogb(Pt.bs0)) : 1.0;
+        if (chg && dip)
+          launch_p2p32<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+        else if (chg)
+          launch_p2p32<true, false>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+        else
+          launch_p2p32<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+        continue;
+      }
       if (chg && dip)
         launch_p2p_pg<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he);
       else if (chg)

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.h"
@@ -43,6 +44,15 @@ void finish_timer(double* timeinfo) {
 SolveStats& last_stats() {
   static thread_local SolveStats s;
   return s;
+}
+
+int& thread_precision() {
+  static const int initial = [] {
+    const char* v = std::getenv("FMM2DHIP_PRECISION");
+    return v && std::strcmp(v, "mixed") == 0 ? (int)PREC_MIXED : (int)PREC_FP64;
+  }();
+  static thread_local int mode = initial;
+  return mode;
 }
 
 int get_ctx(Ctx& c) {

@@ -300,6 +300,108 @@ __global__ void k_xlate_reduce(const int* __restrict__ work, int nwork, int nout
   *y = make_double2(y->x + re, y->y + im);
 }
 
+// ---------------------------------------------------------------------
+// FP32 variant of k_xlate_mfma for the mixed-precision M2L: the operator is
+// stored in FP32, the input expansion is rounded to FP32 as it is loaded,
+// each slot's contraction (at most nin products per output) runs on the
+// FP32 matrix cores, and the per-slot results are summed in FP64 before
+// they are added into the FP64 output.  v_mfma_f32_16x16x4f32 operand
+// layout: A[l%16][l/16], B[l/16][l%16], D[4(l/16) + v][l%16].
+// ---------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
+               const float2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+               size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
+               int G, double2* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int n = lane & 15, kq = lane >> 4;
+  const int w = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + n;
+  const bool wok = w < nwork;
+  const int rt0 = blockIdx.y * MW, g = blockIdx.z;
+  for (int d = 0; d < nd; ++d) {
+    double yr[MW][4], yi[MW][4];
+#pragma unroll
+    for (int t = 0; t < MW; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) yr[t][v] = yi[t][v] = 0.0;
+    double s0r = 0.0, s0i = 0.0;  // sum of x_0 over the sources (lanes kq == 0)
+    for (int s = g; s < nslot; s += G) {
+      const int src = wok ? map[(size_t)w * nslot + s] : -1;
+      if (!__any(src >= 0)) continue;
+      const bool sok = src >= 0;
+      const double2* xs = X + (size_t)(sok ? src : 0) * xstride + (size_t)d * nin;
+      const float2* Ts = T + (size_t)s * nin * nout;
+      if (sok && kq == 0) {
+        const double2 x0 = xs[0];
+        s0r += x0.x;
+        s0i += x0.y;
+      }
+      mfma_f4 cr[MW], ci[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) {
+        cr[t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        ci[t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+      for (int k0 = 0; k0 < nin; k0 += 4) {
+        const int j = k0 + kq;
+        const bool jok = j < nin;
+        const int jc = jok ? j : nin - 1;
+        const double2 xl = xs[jc];
+        const bool xok = sok && jok;
+        const float xr = xok ? (float)xl.x : 0.0f, xi = xok ? (float)xl.y : 0.0f;
+        const float xin = -xi;
+        float ar[MW], ai[MW];
+#pragma unroll
+        for (int t = 0; t < MW; ++t) {
+          const int i = 16 * (rt0 + t) + n;
+          const bool aok = jok && i < nout;
+          const float2 a = Ts[(size_t)jc * nout + (i < nout ? i : nout - 1)];
+          ar[t] = aok ? a.x : 0.0f;
+          ai[t] = aok ? a.y : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < MW; ++t) {
+          cr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[t], xr, cr[t], 0, 0, 0);
+          ci[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[t], xi, ci[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < MW; ++t) {
+          cr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[t], xin, cr[t], 0, 0, 0);
+          ci[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[t], xr, ci[t], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MW; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          yr[t][v] += (double)cr[t][v];
+          yi[t][v] += (double)ci[t][v];
+        }
+    }
+    if (!wok) continue;
+    double2* y = G == 1 ? Y + (size_t)work[w] * ystride + (size_t)d * nout
+                        : part + ((size_t)(g * nd + d) * nwork + w) * nout;
+#pragma unroll
+    for (int t = 0; t < MW; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * (rt0 + t) + 4 * kq + v;
+        if (i >= nout) continue;
+        double re = yr[t][v], im = yi[t][v];
+        if (i == 0 && lap_log != 0.0) {
+          // x_0 sums sit in the kq == 0 lanes, which own output row 0
+          re += s0r * lap_log;
+          im += s0i * lap_log;
+        }
+        if (G == 1) y[i] = make_double2(y[i].x + re, y[i].y + im);
+        else y[i] = make_double2(re, im);
+      }
+  }
+}
+
 // Translation through k_xlate_mfma when the output order fits (nout <= 64),
 // else k_translate.  Rows and slots are split over waves to keep about 4096
 // waves in flight whatever the level's size; maxg bounds the slot groups
@@ -351,6 +453,57 @@ inline hipError_t launch_m2l(hipStream_t st, Scratch& S, const int* work, int nw
                              const int* map, const double2* T, int P, const double2* X, size_t xs,
                              double2* Y, size_t ys, int nd, double lap_log) {
   return launch_xlate(st, S, work, nwork, map, 49, 27, T, P, P, X, xs, Y, ys, nd, lap_log);
+}
+
+// Mixed-precision M2L: k_xlate_mfma32 with the FP32 operators T32 when the
+// order fits its tiles (P <= 64), else the FP64 path with T.  Work is split
+// over waves as in launch_xlate.
+inline hipError_t launch_m2l_mixed(hipStream_t st, Scratch& S, const int* work, int nwork,
+                                   const int* map, const double2* T, const float2* T32, int P,
+                                   const double2* X, size_t xs, double2* Y, size_t ys, int nd,
+                                   double lap_log) {
+  if (nwork <= 0) return hipSuccess;
+  const int MT = (P + 15) / 16;
+  if (MT > 4 || !T32) return launch_m2l(st, S, work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log);
+  const int nslot = 49, maxg = 27;
+  const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
+  constexpr int kWaves = 4096;
+  int mw = MT, G = 1;
+  if (ntile * MT < kWaves) {
+    mw = 1;
+    const int per_g = ntile * MT;
+    G = std::min(std::min(maxg, nslot), std::max(1, (kWaves + per_g - 1) / per_g));
+  } else if (ntile < kWaves) {
+    mw = MT >= 2 ? 2 : 1;
+  }
+  double2* part = nullptr;
+  if (G > 1) {
+    part = S.get<double2>((size_t)G * nd * nwork * P);
+    if (!part) return hipErrorOutOfMemory;
+  }
+  const dim3 grid(nb, (MT + mw - 1) / mw, G);
+#define FMM_XL32(MWV)                                                                           \
+  k_xlate_mfma32<MWV><<<grid, 256, 0, st>>>(work, nwork, map, nslot, T32, P, P, X, xs, Y, ys, nd, \
+                                            lap_log, G, part)
+  switch (mw) {
+    case 1: FMM_XL32(1); break;
+    case 2: FMM_XL32(2); break;
+    case 3: FMM_XL32(3); break;
+    default: FMM_XL32(4); break;
+  }
+#undef FMM_XL32
+  if (G > 1) {
+    const size_t ne = (size_t)nd * nwork * P;
+    k_xlate_reduce<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(work, nwork, P, nd, G, part, Y,
+                                                                ys);
+  }
+  return hipGetLastError();
+}
+
+// FP64 -> FP32 copy of operator tables.
+__global__ void k_to_f32(const double2* __restrict__ in, float2* __restrict__ out, size_t n) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = make_float2((float)in[e].x, (float)in[e].y);
 }
 
 constexpr int XT_TB = 16;

@@ -244,6 +244,20 @@ int fmm2dhip_last_stats(double *out, int n);
  * this returns it to the system.  Safe to call between solves only. */
 void fmm2dhip_release_workspace(void);
 
+/* arithmetic of the list-1 (near field) and M2L steps for the calling
+ * thread's later solves (no counterpart in the reference, which is FP64
+ * throughout).  FMM2DHIP_PREC_FP64: every step in FP64 (the default, unless
+ * the environment variable FMM2DHIP_PRECISION is "mixed" when the library
+ * loads).  FMM2DHIP_PREC_MIXED: when eps >= 1e-6, the Laplace near field
+ * and the M2L translations of both kernels compute in FP32 (differences of
+ * coordinates in FP64, sums carried in FP64); tighter eps stays FP64.
+ * Results then agree with the FP64 solve to about eps rather than to
+ * rounding.  Returns the previous mode; other values leave it unchanged. */
+#define FMM2DHIP_PREC_FP64 0
+#define FMM2DHIP_PREC_MIXED 1
+int fmm2dhip_set_precision(int mode);
+int fmm2dhip_get_precision(void);
+
 /* library/device information for diagnostics: returns 0 when a gfx950
  * device is usable and writes its name (at most len bytes). */
 int fmm2dhip_device_info(char *name, int len);

@@ -29,6 +29,17 @@ def test_header_symbols_exported(eng):
         assert hasattr(eng, n), n
 
 
+def test_precision_mode_is_thread_state(eng):
+    prev = eng.fmm2dhip_set_precision(F.MIXED)
+    assert eng.fmm2dhip_get_precision() == F.MIXED
+    assert eng.fmm2dhip_set_precision(7) == F.MIXED  # invalid: unchanged
+    assert eng.fmm2dhip_get_precision() == F.MIXED
+    with F.precision(F.FP64):
+        assert eng.fmm2dhip_get_precision() == F.FP64
+    assert eng.fmm2dhip_get_precision() == F.MIXED
+    eng.fmm2dhip_set_precision(prev)
+
+
 def test_library_fails_loudly_when_missing(monkeypatch, tmp_path):
     monkeypatch.setattr(F, "LIB_PATH", str(tmp_path / "missing.so"))
     monkeypatch.setattr(F, "_lib", None)
