@@ -171,6 +171,133 @@ def mixed_vs_fp64(lib, prob, src, trg, ch, dp, dv, zk, eps=1e-6, steps=5, nsub=1
     return out
 
 
+def batched_c4(lib, nworkers=8, nprob=512, seed=4):
+    """512 independent problems (N lognormal, median 2e4, clipped to
+    [2e3, 2e5]; uniform or Gaussian-blob points; half Helmholtz with
+    zk ~ U[1, 50], half Cauchy; complex charges; eps = 1e-6; pot + grad at
+    sources) solved by fmm2dhip_batch in FP64 and in mixed precision, and one
+    at a time through the drop-in calls for comparison."""
+    import fmm2d_c_translation_amd as F
+    from fmm2d_c_translation_amd import serve
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(nprob):
+        n = int(np.clip(np.exp(rng.normal(np.log(2e4), 1.0)), 2e3, 2e5))
+        src = rng.random((2, n)) if rng.random() < 0.5 else 0.5 + 0.05 * rng.standard_normal((2, n))
+        helm = rng.random() < 0.5
+        specs.append(dict(kernel=serve.HELMHOLTZ if helm else serve.CAUCHY, eps=1e-6,
+                          zk=float(rng.uniform(1, 50)) if helm else 0.0, sources=src,
+                          charges=rng.standard_normal(n) + 1j * rng.standard_normal(n), pg=2))
+    npts = sum(s["sources"].shape[1] for s in specs)
+    out = {"problems": nprob, "points": npts, "workers": nworkers, "eps": 1e-6}
+    print(f"batched: {nprob} problems, {npts} points", file=sys.stderr, flush=True)
+    results = {}
+    for mode, label in ((F.FP64, "fp64"), (F.MIXED, "mixed")):
+        probs = [serve.Problem(precision=mode, **s) for s in specs]
+        serve.batch(lib, probs[:nworkers], nworkers)  # workers' streams and workspaces
+        probs = [serve.Problem(precision=mode, **s) for s in specs]
+        t0 = time.perf_counter()
+        nbad = serve.batch(lib, probs, nworkers)
+        dt = time.perf_counter() - t0
+        if nbad:
+            raise RuntimeError(f"batched solve: {nbad} problems failed")
+        results[label] = probs
+        out[label] = {"mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4)}
+        print(f"batched {label}: {dt:.3f} s", file=sys.stderr, flush=True)
+    # largest relative difference of the mixed results from the FP64 ones
+    worst = 0.0
+    for a, b in zip(results["fp64"], results["mixed"]):
+        ra, rb = a.result(), b.result()
+        for k in ("pot", "grad"):
+            worst = max(worst, float(np.linalg.norm(rb[k] - ra[k]) / np.linalg.norm(ra[k])))
+    out["mixed_max_rel_l2_diff_vs_fp64"] = float(f"{worst:.3e}")
+    # the same problems one at a time on one thread (FP64, drop-in calls)
+    from fmm2d_c_translation_amd import abi
+    t0 = time.perf_counter()
+    for s in specs:
+        if s["kernel"] == serve.HELMHOLTZ:
+            abi.hfmm2d_call(lib, "hfmm2d_", 1e-6, s["zk"], s["sources"], charges=s["charges"], pg=2)
+        else:
+            abi.cfmm2d_call(lib, "cfmm2d_", 1e-6, s["sources"], charges=s["charges"], pg=2)
+    dt = time.perf_counter() - t0
+    print(f"sequential fp64: {dt:.3f} s", file=sys.stderr, flush=True)
+    out["sequential_fp64"] = {"mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4)}
+    out["data"] = "host arrays (PCIe transfers inside the timed region)"
+    return out
+
+
+def served_c5(lib, seconds, rate=200.0, nworkers=8, budget=200e9, seed=5):
+    """Poisson arrivals at `rate` requests/s for `seconds`: 60% lfmm2d, 40%
+    hfmm2d (zk ~ U[1, 100]); N log-uniform in [1e3, 2e6] uniform in [0,1]^2;
+    charges + dipoles N(0,1); eps from {1e-4, 1e-6, 1e-10}; pot + grad at
+    sources; HBM budget 200 GB.  Inputs are slices of pre-drawn pools."""
+    import threading
+    from fmm2d_c_translation_amd import serve
+    rng = np.random.default_rng(seed)
+    npool, M = 4, 2_000_000
+    pools = []
+    for _ in range(npool):
+        a = rng.random(M) * 2 * np.pi
+        pools.append((np.asfortranarray(rng.random((2, M))),
+                      rng.standard_normal(M) + 1j * rng.standard_normal(M),
+                      rng.standard_normal(M) + 1j * rng.standard_normal(M),
+                      np.asfortranarray(np.stack([np.cos(a), np.sin(a)])[None])))
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 10))
+    t_arr = t_arr[t_arr < seconds]
+    done = []
+    pending = []
+    cv = threading.Condition()
+    with serve.Server(lib, nworkers=nworkers, hbm_budget_bytes=budget) as srv:
+        def collector():
+            while True:
+                with cv:
+                    while not pending:
+                        cv.wait()
+                    item = pending.pop(0)
+                if item is None:
+                    return
+                p, t = item
+                srv.wait(t, -1.0)
+                done.append((p.c.ier, p.c.latency))
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        print(f"served: {len(t_arr)} arrivals over {seconds} s", file=sys.stderr, flush=True)
+        next_report = 1.0
+        for ta in t_arr:
+            if ta >= next_report:
+                print(f"served: t={ta:.1f} s, {len(done)} done", file=sys.stderr, flush=True)
+                next_report += 1.0
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+            src, ch, dp, dv = pools[int(rng.integers(npool))]
+            off = int(rng.integers(0, M - n + 1))
+            helm = rng.random() < 0.4
+            eps = float(rng.choice([1e-4, 1e-6, 1e-10]))
+            p = serve.Problem(kernel=serve.HELMHOLTZ if helm else serve.LOG, eps=eps,
+                              zk=float(rng.uniform(1, 100)) if helm else 0.0,
+                              sources=src[:, off:off + n], charges=ch[off:off + n],
+                              dipstr=dp[off:off + n], dipvec=dv[:, :, off:off + n], pg=2)
+            wait = ta - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            t = srv.submit(p)
+            with cv:
+                pending.append((p, t))
+                cv.notify()
+        with cv:
+            pending.append(None)
+            cv.notify()
+        th.join()
+        st = srv.stats()
+    fails = sum(1 for ier, _ in done if ier != 0)
+    return {"seconds": seconds, "rate_req_s": rate, "requests": len(done), "failed": fails,
+            "workers": nworkers, "budget_bytes": budget,
+            "mpts_s": round(st["mpts_s"], 3), "p50_ms": round(st["p50_s"] * 1e3, 2),
+            "p99_ms": round(st["p99_s"] * 1e3, 2), "max_ms": round(st["max_s"] * 1e3, 2),
+            "peak_workspace_gb": round(st["peak_workspace_bytes"] / 1e9, 3),
+            "data": "host arrays, arrival to result written (PCIe transfers included)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +308,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the eps = 1e-6 mixed-precision vs FP64 comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-problem batch")
+    ap.add_argument("--served-seconds", type=float, default=10.0,
+                    help="length of the served request stream (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter "
                          "calibration: FETCH_SIZE/WRITE_SIZE of a kernel of known traffic)")
@@ -270,6 +400,10 @@ def main():
     }
     if not args.no_mixed:
         out["mixed_vs_fp64"] = mixed_vs_fp64(lib, prob, src, trg, ch, dp, dv, zk)
+    if not args.no_batch:
+        out["batched_c4"] = batched_c4(lib)
+    if args.served_seconds > 0:
+        out["served_c5"] = served_c5(lib, args.served_seconds)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v, dt = cpu_baseline(args.cpu_sample, zk, eps, threads)

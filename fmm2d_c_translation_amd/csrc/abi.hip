@@ -303,6 +303,45 @@ void zero8(double* t) {
 }
 
 }  // namespace
+
+// One problem of the batched/served interface on the calling thread, with
+// the problem's precision mode for the duration of the solve.
+int solve_problem(fmm2dhip_problem& p) {
+  const double t0 = wall();
+  int& mode = thread_precision();
+  const int saved = mode;
+  mode = p.precision == PREC_MIXED ? PREC_MIXED : PREC_FP64;
+  const int ndiv = p.ndiv > 0 ? p.ndiv : ndiv_rule(p.eps, p.ns, p.nt);
+  int e;
+  if (p.kernel == FMM2DHIP_HELMHOLTZ) {
+    HelmReq r;
+    r.nd = p.nd; r.eps = p.eps; r.zk = std::complex<double>(p.zk_re, p.zk_im);
+    r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
+    r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
+    r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
+    r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
+    r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
+    r.ndiv = ndiv; r.device = p.device_ptrs == 1;
+    e = guarded_h(r);
+  } else if (p.kernel == FMM2DHIP_CAUCHY || p.kernel == FMM2DHIP_LOG) {
+    LapReq r;
+    r.logk = p.kernel == FMM2DHIP_LOG;
+    r.nd = p.nd; r.eps = p.eps; r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
+    r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
+    r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
+    r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
+    r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
+    r.ndiv = ndiv; r.device = p.device_ptrs == 1;
+    e = guarded(r);
+  } else {
+    e = IER_ARGS;
+  }
+  mode = saved;
+  p.ier = e;
+  p.seconds = wall() - t0;
+  return e;
+}
+
 }  // namespace fmm
 
 using fmm::LapReq;
@@ -663,12 +702,13 @@ int fmm2dhip_get_precision(void) { return fmm::thread_precision(); }
 
 int fmm2dhip_last_stats(double* out, int n) {
   const fmm::SolveStats& s = fmm::last_stats();
-  double v[23] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
+  double v[24] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
                   s.nlevels, s.pmax};
   for (int k = 0; k < 8; ++k) v[7 + k] = s.step_s[k];
   for (int k = 0; k < 8; ++k) v[15 + k] = s.host_s[k];
-  for (int k = 0; k < n && k < 23; ++k) out[k] = v[k];
-  return 23;
+  v[23] = s.workspace_bytes;
+  for (int k = 0; k < n && k < 24; ++k) out[k] = v[k];
+  return 24;
 }
 
 int fmm2dhip_device_info(char* name, int len) {

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../include/fmm2d_hip.h"
 #include "common.h"
 #include "tree.h"
 
@@ -41,6 +42,11 @@ class Scratch {
 };
 // Frees the calling thread's workspace on the current device.
 void release_workspace();
+// Device bytes held by the calling thread's workspace on the current device.
+size_t workspace_capacity();
+// Frees everything the calling thread holds: its workspaces, pinned staging
+// and stream (for worker threads that are about to exit).
+void release_thread();
 
 struct Ctx {
   hipStream_t st = nullptr;
@@ -153,6 +159,7 @@ struct SolveStats {
   // refinement, 3 level restriction, 4 boxes+lists+work lists, 5 leaf
   // order+gather, 6 operator setup, 7 whole call
   double host_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double workspace_bytes = 0;  // device workspace the solve took
 };
 double wall();  // seconds, monotonic
 SolveStats& last_stats();
@@ -203,5 +210,9 @@ int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier);
 
 int l2dterms(double eps);
 int ndiv_rule(double eps, int ns, int nt);
+
+// One problem of the batched/served interface, solved on the calling
+// thread (abi.hip); writes p.ier and p.seconds, returns p.ier.
+int solve_problem(fmm2dhip_problem& p);
 
 }  // namespace fmm

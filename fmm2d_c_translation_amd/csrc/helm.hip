@@ -825,19 +825,28 @@ int h2dterms_cached(double bs, std::complex<double> zk, double eps, int* ier) {
   double rs = 1.0;
   cd zb = cd{zk.real() * bs, zk.imag() * bs};
   if (cabs_(zb) < 2 * kPi) rs = cabs_(zb);
-  std::vector<cd> hf(ntmax + 2), jf(ntmax + 2);
-  hvals(ntmax, zb * 1.5, rs, hf.data(), 1);
-  jvals(ntmax, zb * (std::sqrt(2.0) / 2.0), rs, jf.data(), 1);
   auto ab = [](cd a) { return cabs_(a); };
-  double x0 = ab(jf[0] * hf[0]) + ab(jf[1] * hf[1]);
+  // The rule scans j = 2..ntmax and stops at the first small term, almost
+  // always within a few hundred orders.  Orders 0..nlim are evaluated with
+  // the ntmax-term recurrences' own starting points (bit for bit the values
+  // of the full evaluation), widening nlim only when the scan gets past it.
   int nt = 10001, e = 13;
-  for (int j = 2; j <= ntmax; ++j) {
-    double x = (ab(jf[j] * hf[j]) + ab(jf[j - 1] * hf[j - 1])) * ab(hf[0]);
-    if (x < eps * x0) {
-      nt = j + 1;
-      e = 0;
-      break;
+  for (int nlim : {256, 4096, ntmax}) {
+    std::vector<cd> hf(nlim + 2), jf(nlim + 2);
+    hvals(nlim, zb * 1.5, rs, hf.data(), 1);
+    jvals_part(ntmax, nlim, zb * (std::sqrt(2.0) / 2.0), rs, jf.data(), 1);
+    double x0 = ab(jf[0] * hf[0]) + ab(jf[1] * hf[1]);
+    bool found = false;
+    for (int j = 2; j <= nlim; ++j) {
+      double x = (ab(jf[j] * hf[j]) + ab(jf[j - 1] * hf[j - 1])) * ab(hf[0]);
+      if (x < eps * x0) {
+        nt = j + 1;
+        e = 0;
+        found = true;
+        break;
+      }
     }
+    if (found) break;
   }
   std::lock_guard<std::mutex> g(mu);
   memo[key] = {nt, e};

@@ -55,16 +55,28 @@ int& thread_precision() {
   return mode;
 }
 
+namespace {
+struct ThreadStream {
+  hipStream_t st = nullptr;
+  int dev = -1;
+};
+thread_local ThreadStream g_stream;
+struct Pinned {
+  void* buf[8] = {};
+  size_t cap[8] = {};
+};
+thread_local Pinned g_pinned;
+}  // namespace
+
 int get_ctx(Ctx& c) {
-  static thread_local hipStream_t st = nullptr;
-  static thread_local int dev = -1;
+  ThreadStream& T = g_stream;
   int d = 0;
   if (hipGetDevice(&d) != hipSuccess) return IER_NOGPU;
-  if (!st || dev != d) {
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return IER_NOGPU;
-    dev = d;
+  if (!T.st || T.dev != d) {
+    if (hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking) != hipSuccess) return IER_NOGPU;
+    T.dev = d;
   }
-  c.st = st;
+  c.st = T.st;
   c.device = d;
   return IER_OK;
 }
@@ -114,8 +126,17 @@ Scratch::~Scratch() {
     char* p = nullptr;
     if (hipMalloc(&p, want) == hipSuccess) A.blocks.push_back({p, want});
   }
+  last_stats().workspace_bytes = (double)A.total;
   A.used = 0;
   A.total = 0;
+}
+
+size_t workspace_capacity() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) return 0;
+  size_t n = 0;
+  for (auto& b : g_arena[d].blocks) n += b.second;
+  return n;
 }
 
 void release_workspace() {
@@ -129,9 +150,27 @@ void release_workspace() {
   A.used = A.total = 0;
 }
 
+void release_thread() {
+  release_workspace();
+  if (g_timer.ok) {
+    for (auto& e : g_timer.ev) (void)hipEventDestroy(e);
+    g_timer = StepTimer{};
+  }
+  for (int k = 0; k < 8; ++k) {
+    if (g_pinned.buf[k]) (void)hipHostFree(g_pinned.buf[k]);
+    g_pinned.buf[k] = nullptr;
+    g_pinned.cap[k] = 0;
+  }
+  if (g_stream.st) {
+    (void)hipStreamSynchronize(g_stream.st);
+    (void)hipStreamDestroy(g_stream.st);
+  }
+  g_stream = ThreadStream{};
+}
+
 void* pinned(size_t bytes, int slot) {
-  static thread_local void* buf[8] = {};
-  static thread_local size_t cap[8] = {};
+  void** buf = g_pinned.buf;
+  size_t* cap = g_pinned.cap;
   if (slot < 0 || slot >= 8) return nullptr;
   if (bytes > cap[slot]) {
     if (buf[slot]) (void)hipHostFree(buf[slot]);

@@ -189,14 +189,16 @@ FMM_HD int miller_top(int nterms, cd z) {
   return nterms + 10000;
 }
 
-// Scaled J_n(z)/rs^n for n = 0..nmax into out[n*stride].
-FMM_HD void jvals(int nmax, cd z, double rs, cd* out, int stride) {
+// Scaled J_n(z)/rs^n into out[n*stride], n = 0..nmax, with the downward
+// recurrence started where an ntop-term evaluation starts (nmax <= ntop):
+// the stored values are bit for bit those of jvals(ntop, ...).
+FMM_HD void jvals_part(int ntop, int nmax, cd z, double rs, cd* out, int stride) {
   if (cabs_(z) < 1e-200) {
     out[0] = mk(1, 0);
     for (int n = 1; n <= nmax; ++n) out[n * stride] = mk(0, 0);
     return;
   }
-  int top = miller_top(nmax, z);
+  int top = miller_top(ntop, z);
   cd zr = inv(z) * rs;       // g_{n-1} = 2n (rs/z) g_n - rs^2 g_{n+1}
   double r2 = rs * rs;
   cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1);
@@ -219,6 +221,11 @@ FMM_HD void jvals(int nmax, cd z, double rs, cd* out, int stride) {
   cd sc = inv(lam);
   out[0] = g * sc;
   for (int n = 1; n <= nmax && n <= top - 1; ++n) out[n * stride] = out[n * stride] * sc;
+}
+
+// Scaled J_n(z)/rs^n for n = 0..nmax into out[n*stride].
+FMM_HD void jvals(int nmax, cd z, double rs, cd* out, int stride) {
+  jvals_part(nmax, nmax, z, rs, out, stride);
 }
 
 // J_n(x)/rs^n, n = 0..nmax, of a real argument x > 0: Miller's downward

@@ -235,7 +235,8 @@ int fmm2dhip_test_devtree(const double *src, int ns, const double *targ, int nt,
  * timeinfo order, [15..22] host wall times of the front end (bounding box,
  * keys and sort, adaptive refinement, level restriction, boxes + lists +
  * work lists, leaf order and gather, operator setup, whole call including
- * the end-of-solve synchronisation).  Returns the number of values available (23). */
+ * the end-of-solve synchronisation), [23] device workspace bytes the solve
+ * took.  Returns the number of values available (24). */
 int fmm2dhip_last_stats(double *out, int n);
 
 /* frees the calling thread's device workspace on the current device (no
@@ -261,6 +262,72 @@ int fmm2dhip_get_precision(void);
 /* library/device information for diagnostics: returns 0 when a gfx950
  * device is usable and writes its name (at most len bytes). */
 int fmm2dhip_device_info(char *name, int len);
+
+/* ---- batched and served solves (no counterpart in the reference, whose
+ * entry points solve one problem per call).
+ *
+ * A problem of any of the three kernels, by value.  Field meanings are those
+ * of the drop-in calls (cfmm2d_ndiv_, lfmm2d_ndiv_, hfmm2d_ndiv_); arrays
+ * are host memory, or device memory in the layouts of the _dev entry points
+ * when device_ptrs is 1.  Outputs are overwritten as by those calls.  The
+ * caller keeps the struct and its arrays alive until the problem is done. */
+#define FMM2DHIP_CAUCHY 0    /* cfmm2d */
+#define FMM2DHIP_LOG 1       /* lfmm2d */
+#define FMM2DHIP_HELMHOLTZ 2 /* hfmm2d */
+typedef struct fmm2dhip_problem {
+  int kernel;
+  int nd;
+  double eps;
+  double zk_re, zk_im; /* Helmholtz wavenumber */
+  int ns;
+  const double *sources;
+  int ifcharge;
+  const void *charge;
+  int ifdipole;
+  const void *dipstr;
+  const double *dipvec; /* log and Helmholtz kernels */
+  int ifpgh;
+  void *pot, *grad, *hess;
+  int nt;
+  const double *targ;
+  int ifpghtarg;
+  void *pottarg, *gradtarg, *hesstarg;
+  int ndiv;        /* <= 0: the leaf-size rule for eps (hndiv2d_) */
+  int precision;   /* FMM2DHIP_PREC_FP64 or FMM2DHIP_PREC_MIXED */
+  int device_ptrs; /* 1: every array is a device pointer */
+  /* written by the library */
+  int ier;
+  double seconds;  /* wall time of the solve on its worker */
+  double latency;  /* submission to completion (s), queueing included */
+} fmm2dhip_problem;
+
+/* Solves probs[0..n) on nworkers worker threads (1..32), each with its own
+ * HIP stream and resident device workspace, so the problems' kernels
+ * overlap on the device; larger problems start first.  The workers persist
+ * in the process and are reused by later calls.  Returns the number of
+ * problems whose ier is not 0. */
+int fmm2dhip_batch(int n, fmm2dhip_problem *probs, int nworkers);
+
+/* A persistent request loop.  Requests are admitted in arrival order as a
+ * worker and device memory free up: a worker keeps the workspace of its
+ * largest solve resident, and a request starts only while the workspaces of
+ * all workers, its own grown to the request's estimate, stay within
+ * hbm_budget_bytes (idle workers give theirs back when the head of the
+ * queue would not fit).  Work of concurrent requests overlaps on the
+ * device.  submit returns a ticket >= 0, or -1 when the request alone
+ * exceeds the budget or the server is shutting down; wait returns 0 when
+ * the ticket is done, 1 on time-out, -1 for an unknown ticket.  stats
+ * writes up to n of: requests done, failed, points solved, seconds from the
+ * first submission to the last completion, Mpts/s over that span, p50, p99
+ * and max latency (s), peak workspace bytes held, budget; returns the count
+ * available.  destroy finishes admitted and queued work, then stops the
+ * workers and frees their workspaces. */
+typedef struct fmm2dhip_server fmm2dhip_server;
+fmm2dhip_server *fmm2dhip_server_create(int nworkers, double hbm_budget_bytes);
+long long fmm2dhip_server_submit(fmm2dhip_server *s, fmm2dhip_problem *p);
+int fmm2dhip_server_wait(fmm2dhip_server *s, long long ticket, double timeout_s);
+int fmm2dhip_server_stats(fmm2dhip_server *s, double *out, int n);
+void fmm2dhip_server_destroy(fmm2dhip_server *s);
 
 #ifdef __cplusplus
 }

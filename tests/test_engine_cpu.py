@@ -19,12 +19,14 @@ HDR = os.path.join(ROOT, "include", "fmm2d_hip.h")
 def declared():
     txt = open(HDR).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^(?:void|int)\s+(\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:void|int|long long|fmm2dhip_server)\s*\**\s*(\w+)\s*\(",
+                                 txt, flags=re.M)))
 
 
 def test_header_symbols_exported(eng):
     names = declared()
     assert "cfmm2d_" in names and "lfmm2d_ndiv_" in names
+    assert "fmm2dhip_server_create" in names and "fmm2dhip_server_submit" in names
     for n in names:
         assert hasattr(eng, n), n
 
@@ -108,3 +110,24 @@ def test_engine_tree_matches_oracle(seed, n, ndiv, idiv, kind):
     assert (a.nlevels, a.nboxes) == (b.nlevels, b.nboxes)
     for k in ("itree", "centers", "boxsize", "isrc", "isrcse", "itarg", "itargse", "nlist", "lists"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_engine_h2dterms_matches_reference(eng, ref):
+    """The engine's expansion-order rule (host code, evaluated over the
+    orders it needs) against the compiled reference's h2dterms_ (h2dterms.c:22-78)."""
+    if ref is None:
+        pytest.skip("reference library not built here")
+    rng = np.random.default_rng(11)
+    cases = [(bs, zk, eps) for bs in (1.0, 0.25, 1 / 64) for zk in (40.0, 3.0 + 0.5j, 100.0)
+             for eps in (1e-3, 1e-6, 1e-10, 1e-14)]
+    cases += [(2.0 ** -int(rng.integers(0, 8)) * rng.uniform(0.5, 1.5),
+               complex(rng.uniform(0.1, 120), rng.choice([0.0, rng.uniform(0, 5)])),
+               float(rng.choice([1e-4, 1e-9, 1e-12]))) for _ in range(40)]
+    for bs, zk, eps in cases:
+        out = []
+        for lib in (eng, ref):
+            n, ier = C.c_int32(0), C.c_int32(0)
+            lib.h2dterms_(C.byref(C.c_double(bs)), (C.c_double * 2)(complex(zk).real, complex(zk).imag),
+                          C.byref(C.c_double(eps)), C.byref(n), C.byref(ier))
+            out.append((n.value, ier.value))
+        assert out[0] == out[1], (bs, zk, eps, out)
