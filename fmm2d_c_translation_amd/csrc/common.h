@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstddef>
 
@@ -55,6 +56,48 @@ FMM_HD cd inv(cd z) {
   return cd{r / d, -1.0 / d};
 }
 FMM_HD cd cdiv(cd a, cd b) { return a * inv(b); }
+
+// Natural log of a positive finite double.  x = 2^k m with m in
+// [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172,
+// from the series 2 (s + s^3/3 + ... + s^23/23) (the first omitted term is
+// below 1e-18 of the sum); k ln 2 in two parts (Cody and Waite), the high
+// part with 21 significant bits so that k ln2_hi is exact.  About thirty
+// instructions and a couple of ulp, without branches: zero gives -inf,
+// +inf gives +inf, negative and NaN arguments give NaN.
+FMM_HD double log_pos(double x) {
+  int k;
+  double m = frexp(x, &k);  // [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    --k;
+  }
+  const double a = m - 1.0, b = m + 1.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);  // refined by two Newton steps
+  r = fma(r, fma(-b, r, 1.0), r);
+  r = fma(r, fma(-b, r, 1.0), r);
+#else
+  const double r = 1.0 / b;
+#endif
+  const double s = a * r, s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = fma(p, s2, 1.0 / 21.0);
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double lm = fma(2.0 * s * s2, p, 2.0 * s);
+  const double ln2_hi = 0x1.62e42p-1, ln2_lo = 0x1.fdf473de6af28p-22;
+  const double kd = (double)k;
+  const double v = fma(kd, ln2_hi, fma(kd, ln2_lo, lm));
+  const bool normal = x > 0.0 && x <= 1.7976931348623157e308;
+  return normal ? v : x == 0.0 ? -HUGE_VAL : x > 0.0 ? x : NAN;
+}
 
 // Quadrant digit of a point relative to a box centre: bit 0 set for the
 // right half (x - cx >= 0), bit 1 for the upper half (y - cy >= 0).  Child

@@ -624,17 +624,20 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
             if (Q) {
               cd q = ld(&sq[m]);
               double v1 = sv[m].x, v2 = sv[m].y;
-              double ct = dx * ri, st = dy * ri;
-              A.Pd += h1 * ri * q * (dx * v1 + dy * v2);
+              const double dv = dx * v1 + dy * v2;
+              A.Pd += (h1 * (ri * dv)) * q;
               if (PG >= 2) {
-                cd zi = NR > 0 ? cd{azi, 0.0} : inv(z);
-                cd h2 = h1 * zi * 2.0 - h0;
-                cd f1 = h2 * (ct * ct - 0.5) - h0 * 0.5, f2 = h2 * (ct * st);
-                cd f3 = h2 * (st * st - 0.5) - h0 * 0.5;
-                cd qk = -(q * zk);
-                A.Gx += qk * (f1 * v1 + f2 * v2);
-                A.Gy += qk * (f2 * v1 + f3 * v2);
+                // grad of H1(kr) (x-y).v / r:  -k [H2 (dv / r^2) (dx, dy)
+                //   - (H1 / (k r)) v],  H2 = 2 H1 / (k r) - H0
+                const cd zi = NR > 0 ? cd{azi, 0.0} : inv(z);
+                const cd w = h1 * zi;
+                const cd h2 = w * 2.0 - h0;
+                const cd u = h2 * (dv * ri * ri);
+                const cd qk = NR > 0 ? cd{-q.x * zk.x, -q.y * zk.x} : -(q * zk);
+                A.Gx += qk * (u * dx - w * v1);
+                A.Gy += qk * (u * dy - w * v2);
                 if (PG >= 3) {
+                  double ct = dx * ri, st = dy * ri;
                   cd h3 = h2 * zi * 4.0 - h1;
                   cd a = (h1 * 0.75 - h3 * (0.5 * (ct * ct - 0.5 - st * st))) * ct;
                   cd bb = (h1 * 0.25 - h3 * (0.5 * (1.5 * ct * ct - 0.5 * st * st))) * st;

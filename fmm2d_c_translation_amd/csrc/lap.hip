@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree
           w0 = mk(1, 0);
           step = r * zi;
           qw = -(q * zi);
-          term0 = c * (0.5 * log(norm2(z))) + qw;
+          term0 = c * (0.5 * log_pos(norm2(z))) + qw;
         }
         cd pw = w0;
 #pragma unroll
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(WAVE) k_eval_mp(const int* work, int nwork, DT
           if (PG >= 2) s2 += v * (double)n;
           if (PG >= 3) s3 += v * (double)(n * (n + 1));
         }
-        sp += m0 * (0.5 * log(norm2(z))) + s1;
+        sp += m0 * (0.5 * log_pos(norm2(z))) + s1;
         if (PG >= 2) sg += (m0 - s2) * zi;
         if (PG >= 3) sh += (s3 - m0) * (zi * zi);
       }
@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
             const cd z2 = -(zi * zi);
             if (C) {
               const cd cc = mk(sc[m].x, sc[m].y);
-              ap += cc * (0.5 * log(rr));
+              ap += cc * (0.5 * log_pos(rr));
               if (PG >= 2) ag += cc * zi;
               if (PG >= 3) ah += cc * z2;
             }

@@ -84,7 +84,7 @@ FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
     dq = fma(dq, w, q);
     q = fma(q, w, JY_YC[k]);
   }
-  const double hx = 0.5 * x, lg = log(hx) + kEuler;
+  const double hx = 0.5 * x, lg = log_pos(hx) + kEuler;
   const double j1 = -dp * hx;
   const double y0 = (lg * p + q) * (2.0 / kPi);
   const double y1 = -(2.0 / kPi) * (p * xi - lg * j1 + dq * hx);

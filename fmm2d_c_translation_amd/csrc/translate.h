@@ -230,24 +230,38 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
       }
       // Branch-free body: indexes are clamped into range and out-of-range
       // operands zeroed by selects, so the accumulators stay in one register
-      // class across the loop.
+      // class across the loop.  The operands of step k0 + 4 are loaded
+      // before the matrix operations of step k0 (the last step reloads a
+      // valid entry).
+      int irow[MW];
+      bool rok[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) {
+        const int i = 16 * (rt0 + t) + n;
+        rok[t] = i < nout;
+        irow[t] = rok[t] ? i : nout - 1;
+      }
+      double2 xl = xs[kq < nin ? kq : nin - 1];
+      double2 al[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)(kq < nin ? kq : nin - 1) * nout + irow[t]];
       for (int k0 = 0; k0 < nin; k0 += 4) {
-        const int j = k0 + kq;
-        const bool jok = j < nin;
-        const int jc = jok ? j : nin - 1;
-        const double2 xl = xs[jc];
+        const bool jok = k0 + kq < nin;
         const bool xok = sok && jok;
         const double2 x = make_double2(xok ? xl.x : 0.0, xok ? xl.y : 0.0);
         const double xin = -x.y;
         double ar[MW], ai[MW];
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
-          const int i = 16 * (rt0 + t) + n;
-          const bool aok = jok && i < nout;
-          const double2 a = Ts[(size_t)jc * nout + (i < nout ? i : nout - 1)];
-          ar[t] = aok ? a.x : 0.0;
-          ai[t] = aok ? a.y : 0.0;
+          const bool aok = jok && rok[t];
+          ar[t] = aok ? al[t].x : 0.0;
+          ai[t] = aok ? al[t].y : 0.0;
         }
+        const int jn = k0 + 4 + kq < nin ? k0 + 4 + kq : nin - 1;
+        xl = xs[jn];
+#pragma unroll
+        for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)jn * nout + irow[t]];
+        __builtin_amdgcn_sched_barrier(0);  // keep the next step's loads ahead
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
           cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], x.x, cr[t], 0, 0, 0);
@@ -345,23 +359,35 @@ k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ 
         cr[t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
         ci[t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
       }
+      int irow[MW];
+      bool rok[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) {
+        const int i = 16 * (rt0 + t) + n;
+        rok[t] = i < nout;
+        irow[t] = rok[t] ? i : nout - 1;
+      }
+      double2 xl = xs[kq < nin ? kq : nin - 1];
+      float2 al[MW];
+#pragma unroll
+      for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)(kq < nin ? kq : nin - 1) * nout + irow[t]];
       for (int k0 = 0; k0 < nin; k0 += 4) {
-        const int j = k0 + kq;
-        const bool jok = j < nin;
-        const int jc = jok ? j : nin - 1;
-        const double2 xl = xs[jc];
+        const bool jok = k0 + kq < nin;
         const bool xok = sok && jok;
         const float xr = xok ? (float)xl.x : 0.0f, xi = xok ? (float)xl.y : 0.0f;
         const float xin = -xi;
         float ar[MW], ai[MW];
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
-          const int i = 16 * (rt0 + t) + n;
-          const bool aok = jok && i < nout;
-          const float2 a = Ts[(size_t)jc * nout + (i < nout ? i : nout - 1)];
-          ar[t] = aok ? a.x : 0.0f;
-          ai[t] = aok ? a.y : 0.0f;
+          const bool aok = jok && rok[t];
+          ar[t] = aok ? al[t].x : 0.0f;
+          ai[t] = aok ? al[t].y : 0.0f;
         }
+        const int jn = k0 + 4 + kq < nin ? k0 + 4 + kq : nin - 1;
+        xl = xs[jn];
+#pragma unroll
+        for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)jn * nout + irow[t]];
+        __builtin_amdgcn_sched_barrier(0);  // keep the next step's loads ahead
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
           cr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[t], xr, cr[t], 0, 0, 0);
