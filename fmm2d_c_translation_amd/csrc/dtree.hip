@@ -30,6 +30,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "translate.h"
 
 namespace fmm {
 namespace {
@@ -560,11 +561,21 @@ __global__ void k_wl_levels(Levels lv, const int* info, const int* out, int* lev
 
 // M2L list entries keyed by (level, child quadrant): a box of level l >= 1 is
 // child q = (id - loff[l]) & 3 of its parent
-__global__ void k_m2l_keys(Levels lv, const int* list, int n, unsigned char* key, int* val) {
+// M2L grouping key of a target: level above the 49-bit mask of the list-2
+// offsets (slots) whose source box holds sources
+__global__ void k_m2l_keys(Levels lv, const double2* ctr, const int* o2, const int* l2,
+                           const int2* srng, const int* list, int n, uint64_t* key, int* val) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  int b = list[e], l = level_of(lv, b);
-  key[e] = (unsigned char)(4 * l + ((b - lv.loff[l]) & 3));
+  const int b = list[e], l = level_of(lv, b);
+  uint64_t mask = 0;
+  for (int k = o2[b]; k < o2[b + 1]; ++k) {
+    const int j = l2[k];
+    if (srng[j].y <= srng[j].x) continue;
+    const int sl = m2l_slot(ctr[j], ctr[b], lv.bs[l]);
+    if (sl >= 0) mask |= 1ull << sl;
+  }
+  key[e] = ((uint64_t)l << 49) | mask;
   val[e] = b;
 }
 
@@ -804,23 +815,26 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   plan.l2l_kids.n = 4 * plan.l2l.n;
   plan.l2l_kids.lev.resize(W);
   for (int l = 0; l < W; ++l) plan.l2l_kids.lev[l] = 4 * plan.l2l.lev[l];
-  // M2L targets of a level grouped by child quadrant (stable, so each group
-  // stays in Morton order): the 16 targets of an M2L tile then share their
-  // 27 list-2 offsets
+  // M2L targets of a level grouped by the set of list-2 offsets that carry
+  // sources (stable, so equal sets stay in Morton order): the 16 targets of
+  // an M2L tile then mostly share the slots they use, and a slot is skipped
+  // only when no target of the tile has a source there
   if (plan.m2l.n > 0) {
     const int n = plan.m2l.n;
-    unsigned char* k0 = S.get<unsigned char>(n);
-    unsigned char* k1 = S.get<unsigned char>(n);
+    uint64_t* k0 = S.get<uint64_t>(n);
+    uint64_t* k1 = S.get<uint64_t>(n);
     int* v0 = S.get<int>(n);
     if (S.err) return S.err;
-    k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, work + plan.m2l.off, n, k0, v0);
+    k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, ctr, lo.o2, lo.l2, srng, work + plan.m2l.off,
+                                               n, k0, v0);
+    const int kb = 49 + 5;  // level (< 32) above the 49 slot bits
     size_t tbs = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tbs, k0, k1, v0, work + plan.m2l.off, n,
-                                               0, 8, c.st));
+                                               0, kb, c.st));
     void* tmps = S.get<char>(tbs);
     if (S.err) return S.err;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n, 0,
-                                               8, c.st));
+                                               kb, c.st));
   }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
   const unsigned long long* hm = reinterpret_cast<const unsigned long long*>(hs.data() + m2l_at);

@@ -79,7 +79,9 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
         double2 s = pts[c0 + j];
         double dx = s.x - ctr.x, dy = s.y - ctr.y;
         double r = sqrt(dx * dx + dy * dy);
-        double th = (dx == 0 && dy == 0) ? 0.0 : atan2(dy, dx);
+        // e^{i theta} = (dx, dy) / r, theta = 0 at the centre
+        const double ri = r > 0.0 ? 1.0 / r : 0.0;
+        const double ct = r > 0.0 ? dx * ri : 1.0, st = dy * ri;
         if constexpr (NR > 0) {
           const double x = zk.x * r;
           if constexpr (MP) jvals_real_point(p + 1, x, rs, reinterpret_cast<double*>(V) + j, CH);
@@ -89,7 +91,7 @@ k_hform(const int* work, DTree D, const HLev* lev, cd zk, const double2* pts, co
           if (MP) jvals(p + 1, z, rs, reinterpret_cast<cd*>(V) + j, CH);
           else hvals(p + 2, z, rs, reinterpret_cast<cd*>(V) + j, CH);
         }
-        cd zm = cd{cos(th), -sin(th)}, pw = mk(1, 0);   // e^{-i theta}
+        cd zm = cd{ct, -st}, pw = mk(1, 0);   // e^{-i theta}
         for (int n = 0; n <= Q; ++n) {
           Z[(size_t)n * CH + j] = pw;
           pw = pw * zm;
@@ -216,12 +218,12 @@ __device__ inline void eval_term(const EA& E, int n, V v, cd pos, cd neg, EvalAc
 
 // multipole at a point: v_n = H_n rs^n by upward recurrence, n = 0..p+2
 template <int PG, class EA>
-__device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+__device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, cd e1, EvalAcc& A) {
   cd z = zk * r;
   if (cabs_(z) < 1e-200) return;
   cd h0, h1;
   hank01(z, h0, h1);
-  cd e1 = mk(cos(th), sin(th)), pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  cd pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
   cd zi = inv(z) * rs;
   double s2 = rs * rs;
   cd vm = h0, v = h1 * rs;
@@ -240,7 +242,7 @@ __device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, double th, E
 // local at a point: v_n = J_n / rs^n by Miller's downward recurrence with
 // the generating-function normalisation applied at the end
 template <int PG, class EA>
-__device__ void eval_lo_at(const EA& E, cd zk, double rs, double r, double th, EvalAcc& A) {
+__device__ void eval_lo_at(const EA& E, cd zk, double rs, double r, cd e1, EvalAcc& A) {
   cd z = zk * r;
   const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
   if (cabs_(z) < 1e-200) {  // J_0 = 1, others 0
@@ -251,7 +253,6 @@ __device__ void eval_lo_at(const EA& E, cd zk, double rs, double r, double th, E
   cd zr = inv(z) * rs;
   double r2 = rs * rs;
   cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1), ur = u * rs;
-  cd e1 = mk(cos(th), sin(th));
   // phases at n = top_n downward
   cd pos = mk(1, 0);
   for (int k = 0; k < top_n; ++k) pos = pos * e1;
@@ -298,13 +299,13 @@ __device__ __noinline__ H01 hank_real_point(double x) {
 }
 
 template <int PG, int NR, class EA>
-__device__ void eval_mp_at_real(const EA& E, double k, double rs, double r, double th,
+__device__ void eval_mp_at_real(const EA& E, double k, double rs, double r, cd e1,
                                 EvalAcc& A) {
   const double x = k * r;
   if (x < 1e-200) return;
   const H01 hh = hank_real_point<NR>(x);
   const cd h0 = hh.h0, h1 = hh.h1;
-  cd e1 = mk(cos(th), sin(th)), pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  cd pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
   const double zi = rs / x, s2 = rs * rs;
   cd vm = h0, v = h1 * rs;
   eval_term<PG>(E, 0, h0, pos, neg, A);
@@ -332,7 +333,7 @@ __device__ inline int miller_top_real(int nterms, double x) {
 }
 
 template <int PG, class EA>
-__device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, double th,
+__device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, cd e1,
                                 EvalAcc& A) {
   const double x = k * r;
   const int top_n = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
@@ -342,7 +343,6 @@ __device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, doub
   }
   const int top = miller_top_real(top_n, x);
   const double zr = rs / x, r2 = rs * rs;
-  cd e1 = mk(cos(th), sin(th));
   cd pos = mk(1, 0);
   for (int q = 0; q < top_n; ++q) pos = pos * e1;
   const cd ie1 = conj(e1);
@@ -445,13 +445,14 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
           const double2 cc = D.ctr[s];
           const double ddx = x.x - cc.x, ddy = x.y - cc.y;
           const double r = sqrt(ddx * ddx + ddy * ddy);
-          const double th = (ddx == 0 && ddy == 0) ? 0.0 : atan2(ddy, ddx);
+          const double ri = r > 0.0 ? 1.0 / r : 0.0;
+          const cd e1 = r > 0.0 ? mk(ddx * ri, ddy * ri) : mk(1, 0);  // e^{i theta}
           if (NR > 0) {
-            if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, th, A);
-            else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, th, A);
+            if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, e1, A);
+            else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, e1, A);
           } else {
-            if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
-            else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+            if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, e1, A);
+            else eval_lo_at<PG>(E, zk, lv.rs, r, e1, A);
           }
         }
       }
@@ -492,15 +493,16 @@ k_heval(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int2*
         double2 c = D.ctr[s];
         double ddx = x.x - c.x, ddy = x.y - c.y;
         double r = sqrt(ddx * ddx + ddy * ddy);
-        double th = (ddx == 0 && ddy == 0) ? 0.0 : atan2(ddy, ddx);
+        const double ri = r > 0.0 ? 1.0 / r : 0.0;
+        const cd e1 = r > 0.0 ? mk(ddx * ri, ddy * ri) : mk(1, 0);  // e^{i theta}
         double rsd = MPK ? lv.rs : 1.0 / lv.rs;
         Ex E{ex + (size_t)s * per + (size_t)d * (2 * lv.p + 1), lv.p, zk * (0.5 / rsd), zk * (0.5 * rsd)};
         if (NR > 0) {
-          if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, th, A);
-          else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, th, A);
+          if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, e1, A);
+          else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, e1, A);
         } else {
-          if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, th, A);
-          else eval_lo_at<PG>(E, zk, lv.rs, r, th, A);
+          if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, e1, A);
+          else eval_lo_at<PG>(E, zk, lv.rs, r, e1, A);
         }
       }
       addc(pot + (size_t)d * N + i, A.P * i4);
@@ -1031,7 +1033,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, bs[l], map);
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, D.srng, bs[l], map);
     if (dT32)
       HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0],
                                dT32 + tab[l].m2l[0], 2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));

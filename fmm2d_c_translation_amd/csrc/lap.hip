@@ -668,7 +668,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, plan.bs[l], map);
+    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, D.srng, plan.bs[l], map);
     if (a.mixed)
       HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, TM, tab.m2l32, P, mp, per, lo, per,
                                nd, std::log(plan.bs[l])));

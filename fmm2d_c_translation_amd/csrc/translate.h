@@ -105,16 +105,19 @@ __device__ inline int m2l_slot(double2 cs, double2 ct, double bs) {
 
 // maps for the three translation kinds, one thread per work item
 // M2L: slot map from list 2
+// (list-2 boxes without sources have zero multipoles and are left out)
 __global__ void k_map_m2l(const int* work, int nwork, const double2* ctr, const int* o2,
-                          const int* l2, double bs, int* map) {
+                          const int* l2, const int2* srng, double bs, int* map) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork) return;
   int b = work[w];
   int* m = map + (size_t)w * 49;
   for (int s = 0; s < 49; ++s) m[s] = -1;
   for (int k = o2[b]; k < o2[b + 1]; ++k) {
-    int s = m2l_slot(ctr[l2[k]], ctr[b], bs);
-    if (s >= 0) m[s] = l2[k];
+    const int j = l2[k];
+    if (srng[j].y <= srng[j].x) continue;
+    int s = m2l_slot(ctr[j], ctr[b], bs);
+    if (s >= 0) m[s] = j;
   }
 }
 // M2M: slot q = child quadrant, children with sources only
