@@ -23,6 +23,7 @@
 // integer cell coordinates; the reference's 1.05-scaled centre-distance tests
 // separate exactly the same cell offsets.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -230,6 +231,11 @@ struct Levels {
   double bs[kMaxDepth + 1];
   double c0x, c0y;
 };
+
+// onesweep radix sort at every size (rocprim's default merges below 2^20 items)
+using OnesweepSort =
+    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                               rocprim::default_config, 0>;
 
 __device__ inline int level_of(const Levels& lv, int b) {
   int l = 0;
@@ -601,7 +607,7 @@ int sync_read(Ctx& c, const int* d, int n, int* h) {
 
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
                 double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
-                int ifpghtarg, Plan& plan, double* host_s) {
+                int ifpghtarg, int sorted_levels, Plan& plan, double* host_s) {
   double t0 = wall();
   // ---- 0. depth of the adaptive tree (one read-back)
   int* small0 = S.get<int>(2 * (kMaxDepth + 2) + 2);
@@ -621,6 +627,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   int e = sync_read(c, d_depth, 1, &Ls);
   if (e) return e;
   if (idivflag < 0 || idivflag > 2) Ls = -1;
+  // exact while every box that splits lies above the sorted levels: a box
+  // at level sorted_levels holding more than ndiv points shows up as a
+  // split at that level or below, and then the keys need a full sort
+  if (Ls >= sorted_levels && sorted_levels < kMaxDepth) return IER_RESORT;
   if (Ls >= kMaxDepth) return IER_DEPTH;
   const int L = Ls + 1;
 
@@ -697,12 +707,12 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
       k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(cand, d_roff, d_cbase, L, cand);
     size_t tb1 = 0, tb2 = 0;
     const int kbits = 2 * L + 5;  // level (< 32) above 2L key bits
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
+    HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(nullptr, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     void* tmp = S.get<char>(std::max(tb1, tb2));
     if (S.err) return S.err;
     if (nc > 0) {
-      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
+      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     } else {
       HIP_TRY(hipMemsetAsync(d_nu, 0, sizeof(int), c.st));
@@ -829,12 +839,12 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                                                n, k0, v0);
     const int kb = 49 + 5;  // level (< 32) above the 49 slot bits
     size_t tbs = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tbs, k0, k1, v0, work + plan.m2l.off, n,
-                                               0, kb, c.st));
+    HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tbs, k0, k1, v0,
+                                                    work + plan.m2l.off, n, 0, kb, c.st));
     void* tmps = S.get<char>(tbs);
     if (S.err) return S.err;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n, 0,
-                                               kb, c.st));
+    HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n,
+                                                    0, kb, c.st));
   }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
   const unsigned long long* hm = reinterpret_cast<const unsigned long long*>(hs.data() + m2l_at);

@@ -2,6 +2,7 @@
 // in-leaf ordering and gather/scatter of point data; the tree itself is
 // built by dtree.hip.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -275,9 +276,15 @@ __global__ void k_scatter_ch(const double2* in, double2* out, const int* perm, i
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
+// rocprim's default radix sort switches to a merge sort below 2^20 items,
+// about three times slower here than its onesweep radix sort
+using OnesweepSort =
+    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                               rocprim::default_config, 0>;
+
 // keys and the key-sorted permutation of one point set (keys stay on the device)
 int sort_set(Ctx& c, Scratch& S, const double2* pts, int n, double cx, double cy, double bs0,
-             int* perm, uint64_t** sorted) {
+             int begin_bit, int* perm, uint64_t** sorted) {
   uint64_t* k0 = S.get<uint64_t>(n);
   uint64_t* k1 = S.get<uint64_t>(n);
   int* i0 = S.get<int>(n);
@@ -285,13 +292,15 @@ int sort_set(Ctx& c, Scratch& S, const double2* pts, int n, double cx, double cy
   *sorted = k1;
   if (n == 0) return IER_OK;
   k_keys<<<nblk(n, 256), 256, 0, c.st>>>(pts, n, cx, cy, bs0, k0, i0);
+  // stable LSD radix sort (onesweep) of (key, original index) on key bits
+  // [begin_bit, 2 kMaxDepth)
   size_t tb = 0;
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, perm, n, 0, 2 * kMaxDepth,
-                                             c.st));
+  HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tb, k0, k1, i0, perm, n, begin_bit,
+                                                  2 * kMaxDepth, c.st));
   void* tmp = S.get<char>(tb);
   if (S.err) return S.err;
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, perm, n, 0, 2 * kMaxDepth,
-                                             c.st));
+  HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmp, tb, k0, k1, i0, perm, n, begin_bit,
+                                                  2 * kMaxDepth, c.st));
   return IER_OK;
 }
 
@@ -341,21 +350,29 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   P.c0y = (ymin + ymax) / 2;
   th[1] = wall();
 
-  // ---- keys + sort
+  // ---- keys + sort.  Boxes down to level D only need the keys ordered by
+  // their top 2D bits (box ranges and the depth search compare prefixes;
+  // order inside a leaf is restored below), so the sort first covers
+  // kSortLevels levels and is redone on every bit when the tree turns out
+  // deeper.
   P.perm_s = S.get<int>(ns);
   P.perm_t = S.get<int>(nt);
   if (S.err) return S.err;
-  uint64_t *ks = nullptr, *kt = nullptr;
-  int e = sort_set(c, S, src, ns, P.c0x, P.c0y, P.bs0, P.perm_s, &ks);
-  if (e) return e;
-  e = sort_set(c, S, trg, nt, P.c0x, P.c0y, P.bs0, P.perm_t, &kt);
-  if (e) return e;
-  th[2] = wall();
-
-  // ---- tree, lists and work lists on the device
   double tt[3] = {0, 0, 0};
-  e = build_dtree(c, S, ks, ns, kt, nt, P.c0x, P.c0y, P.bs0, idivflag, ndiv, ifpgh, ifpghtarg,
-                  plan, tt);
+  int e = IER_OK;
+  for (int levels : {kSortLevels, kMaxDepth}) {
+    const int begin_bit = 2 * (kMaxDepth - levels);
+    uint64_t *ks = nullptr, *kt = nullptr;
+    e = sort_set(c, S, src, ns, P.c0x, P.c0y, P.bs0, begin_bit, P.perm_s, &ks);
+    if (e) return e;
+    e = sort_set(c, S, trg, nt, P.c0x, P.c0y, P.bs0, begin_bit, P.perm_t, &kt);
+    if (e) return e;
+    th[2] = wall();
+    // ---- tree, lists and work lists on the device
+    e = build_dtree(c, S, ks, ns, kt, nt, P.c0x, P.c0y, P.bs0, idivflag, ndiv, ifpgh, ifpghtarg,
+                    levels, plan, tt);
+    if (e != IER_RESORT) break;
+  }
   if (e) return e;
   th[3] = th[2] + tt[0];
   th[4] = th[3] + tt[1];
