@@ -33,6 +33,14 @@ struct HLev {   // per-level parameters on the device
   double rs;
 };
 
+// orders LDS writes and reads among the lanes of one wavefront
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
 // ---------------------------------------------------------------------
 // formation: multipoles of leaves (scaled J) and locals from list 4
 // (scaled H); charge part helmrouts2d.c:43-64, dipole part :67-105
@@ -298,6 +306,198 @@ __device__ __noinline__ H01 hank_real_point(double x) {
   return h;
 }
 
+// ---------------------------------------------------------------------
+// Formation for a real positive wavenumber with every lane busy: four
+// boxes per wave, a 16-lane group per box, lane u on point u of each chunk
+// of 16 source points.  A lane streams its point's Bessel values in
+// registers -- H_n rs^n by the upward recurrence for locals, J_n / rs^n
+// for multipoles by Miller's downward recurrence run twice (once for the
+// normalisation, once to emit the values, which equal jvals_real's bit for
+// bit) -- and forms its terms of the coefficients in the order they become
+// available.  A 16 x 16 tile in LDS then hands the 16 terms of each
+// coefficient to one lane, which sums them in point order and adds the sum
+// to the expansion.  The per-point factors (charge; dipole terms of orders
+// m > 0, m < 0 and m = 0) and the term of coefficient m,
+//   t_m = (F0 V_n + FA V_{n-1} + FB V_{n+1}) e^{-i m theta},  n = |m|,
+// are those of k_hform (helmrouts2d.c:294-391, 513-612).  Needs p <= 63.
+// ---------------------------------------------------------------------
+constexpr int FG_PAD = 17;  // tile row length (16 terms + 1: no bank conflicts)
+
+template <bool MP, int NR>
+__global__ void __launch_bounds__(WAVE)
+k_hform_g(const int* work, int nwork, DTree D, const HLev* lev, double k, const double2* pts,
+          const double2* chg, const double2* dip, const double* dv, int N, int nd, size_t per,
+          double2* out) {
+  __shared__ cd tile[4 * 16 * FG_PAD];
+  const int grp = (int)threadIdx.x >> 4, u = (int)threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + grp;
+  if (w >= nwork) return;  // the whole group leaves; only wave-level syncs below
+  cd* T = tile + grp * 16 * FG_PAD;
+  const int b = work[w];
+  const int L = D.level[b];
+  const int p = lev[L].p;
+  const double rs = lev[L].rs;
+  const double rsd = MP ? rs : 1.0 / rs;
+  const double2 ctr = D.ctr[b];
+  const int nsb = MP ? 1 : D.o4[b + 1] - D.o4[b];
+  const int ncoef = 2 * p + 1;
+  // coefficient of emission e: locals ascending 0, +1, -1, +2, ...;
+  // multipoles descending +p, -p, +(p-1), ..., 0
+  auto m_of = [&](int e) {
+    if (MP) return e == 2 * p ? 0 : ((e & 1) ? -(p - e / 2) : p - e / 2);
+    return e == 0 ? 0 : ((e & 1) ? (e + 1) / 2 : -(e / 2));
+  };
+  for (int d = 0; d < nd; ++d) {
+    double2* o = out + (size_t)b * per + (size_t)d * ncoef + p;  // o[m], |m| <= p
+    for (int sb = 0; sb < nsb; ++sb) {
+      const int jb = MP ? b : D.l4[D.o4[b] + sb];
+      const int2 rg = D.srng[jb];
+      for (int c0 = rg.x; c0 < rg.y; c0 += 16) {
+        const int j = c0 + u;
+        const bool ok = j < rg.y;
+        const int jj = ok ? j : rg.x;
+        const double2 s = pts[jj];
+        const double dx = s.x - ctr.x, dy = s.y - ctr.y;
+        const double r = sqrt(dx * dx + dy * dy);
+        const double ri = r > 0.0 ? 1.0 / r : 0.0;
+        const double ct = r > 0.0 ? dx * ri : 1.0, st = dy * ri;
+        const cd zm = cd{ct, -st}, zi = cd{ct, st};  // e^{-i theta}, e^{i theta}
+        // per-point factors of this channel (zero for lanes past the chunk)
+        const size_t g = (size_t)d * N + jj;
+        cd F0 = mk(0, 0), FAp = mk(0, 0), FBp = mk(0, 0), FAm = mk(0, 0), FBm = mk(0, 0),
+           F0d = mk(0, 0);
+        if (ok && chg) F0 = ld(chg + g);
+        if (ok && dip) {
+          const cd qf = ld(dip + g) * (-0.5 * k);
+          const double v1 = dv[g], v2 = dv[(size_t)nd * N + g];
+          FAp = ((zi / rsd) * mk(-v1, v2)) * qf;
+          FBp = ((zm * rsd) * mk(v1, v2)) * qf;
+          FAm = ((zm / rsd) * mk(-v1, -v2)) * qf;
+          FBm = ((zi * rsd) * mk(v1, -v2)) * qf;
+          F0d = (qf * rsd) * ((zm + zi) * v1 + mul_i(zm - zi) * v2);
+        }
+        int e = 0;  // emissions so far (uniform over the group)
+        auto emit = [&](cd term) {
+          T[(e & 15) * FG_PAD + u] = term;
+          ++e;
+          if ((e & 15) == 0 || e == ncoef) {
+            const int base = (e - 1) & ~15, nfill = e - base;
+            wave_sync();
+            if (u < nfill) {
+              cd sum = mk(0, 0);
+              for (int q = 0; q < 16; ++q) sum += T[u * FG_PAD + q];
+              const int m = m_of(base + u);
+              o[m] = make_double2(o[m].x + sum.x, o[m].y + sum.y);
+            }
+            wave_sync();
+          }
+        };
+        // terms of orders +n and -n from V_{n-1}, V_n, V_{n+1} and Z_n = e^{-i n theta}
+        auto pair = [&](auto vm, auto v0, auto vp, cd zn, int n) {
+          const cd tp = F0 * v0 + FAp * vm + FBp * vp;
+          const cd tm = F0 * v0 + FAm * vm + FBm * vp;
+          const cd zc = (n & 1) ? -conj(zn) : conj(zn);  // (-e^{i theta})^n
+          emit(tp * zn);
+          emit(tm * zc);
+        };
+        const double x = k * r;
+        if (!MP) {
+          // H_n rs^n upward (hvals_real); every list-4 source is off centre
+          const H01 hh = hank_real_point<(NR > 0 ? NR : 1)>(x);
+          const double zr = rs / x, s2 = rs * rs;
+          cd vm = hh.h0, v0 = hh.h1 * rs;  // V_0, V_1
+          emit(F0 * vm + F0d * v0);        // m = 0: one dipole term
+          cd zn = mk(1, 0);
+          for (int n = 1; n <= p; ++n) {
+            const cd vp = v0 * (zr * (2.0 * n)) - vm * s2;  // V_{n+1}
+            zn = zn * zm;
+            pair(vm, v0, vp, zn, n);
+            vm = v0;
+            v0 = vp;
+          }
+        } else {
+          // J_n / rs^n, n = 0..p+1 (jvals_real), emitted downward.  Each
+          // lane's recurrence starts at its own top; the loop index is the
+          // group's, so every lane emits at the same iterations
+          const int nmax = p + 1;
+          cd zn = mk(1, 0);
+          for (int n = 1; n <= p; ++n) zn = zn * zm;  // Z_p
+          const bool centre = x < 1e-200;             // J_0 = 1, the rest 0
+          const double xi = centre ? 0.0 : 1.0 / x;
+          int top = nmax + 1;
+          if (!centre) {
+            top = nmax + 10000;
+            double fn = 1.0, fm = 0.0;
+            for (int i = nmax; i <= nmax + 10000; ++i) {
+              const double fp = xi * fn * (2.0 * i) - fm;
+              if (fp * fp > 1e40) {
+                top = i + 1;
+                break;
+              }
+              fm = fn;
+              fn = fp;
+            }
+          }
+          int topg = top;
+#pragma unroll
+          for (int q = 8; q >= 1; q >>= 1) topg = max(topg, __shfl_xor(topg, q, 16));
+          const double zr = xi * rs, r2 = rs * rs;
+          // pass 1: normalisation and the number of rescalings
+          double gp = 0.0, g1 = 1.0, H = 0.0;
+          int R = 0;
+          if (!centre) {
+            for (int n = top - 1; n >= 1; --n) {
+              H = rs * ((n & 1) ? H : g1 + H);
+              const double gm = zr * g1 * (2.0 * n) - gp * r2;
+              gp = g1;
+              g1 = gm;
+              if (g1 * g1 > 1e40) {
+                g1 *= 1e-40; gp *= 1e-40; H *= 1e-40;
+                ++R;
+              }
+            }
+          }
+          const double sc = centre ? 1.0 : 1.0 / (g1 + 2.0 * H);
+          // pass 2: the same recurrence; J_n = g_n (1e-40)^(rescalings at
+          // or below n) sc
+          double g = 1.0;
+          gp = 0.0;
+          int seen = 0;
+          double va = 0.0, vb = 0.0;  // J_{n+1}, J_{n+2} of the next emission
+          for (int n = topg - 1; n >= 1; --n) {
+            if (n <= nmax) {  // n < top here: the lane's recurrence has started
+              double v = 0.0;
+              if (!centre) {
+                v = g;
+                for (int q = seen; q < R; ++q) v *= 1e-40;
+                v *= sc;
+              }
+              if (n <= p - 1) {  // J_n completes order n + 1
+                pair(v, va, vb, zn, n + 1);
+                zn = zn * zi;
+              }
+              vb = va;
+              va = v;
+            }
+            if (!centre && n <= top - 1) {
+              const double gm = zr * g * (2.0 * n) - gp * r2;
+              gp = g;
+              g = gm;
+              if (g * g > 1e40) {
+                g *= 1e-40; gp *= 1e-40;
+                ++seen;
+              }
+            }
+          }
+          const double j0 = centre ? 1.0 : g * sc;
+          pair(j0, va, vb, zn, 1);    // order 1: J_0, J_1, J_2
+          emit(F0 * j0 + F0d * va);   // order 0: J_0 and J_1
+        }
+      }
+    }
+  }
+}
+
 template <int PG, int NR, class EA>
 __device__ void eval_mp_at_real(const EA& E, double k, double rs, double r, cd e1,
                                 EvalAcc& A) {
@@ -382,12 +582,6 @@ struct ExS {
   template <bool Y1, bool Y2>
   __device__ cd d2(int) const { return mk(0, 0); }  // not staged (PG <= 2 only)
 };
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Four boxes per wave, 16 lanes per box, lane per point; for each source
 // expansion the box's 16 lanes stage it in LDS with its derivative
@@ -1006,9 +1200,17 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   }
   tm->mark(c.st);
   // step 1: leaf multipoles
-  if (s_form.n)
-    (rz ? k_hform<true, 1> : k_hform<true, 0>)<<<s_form.n, WAVE, shm_mp, c.st>>>(wptr(s_form), D, dlev, zk, Pt.src, chg, dip, dv,
-                                                Pt.ns, nd, CH, per, mp);
+  // all-lanes formation for a real wavenumber up to order 63
+  const bool fg = rz && pmax <= 63;
+  if (s_form.n) {
+    if (fg)
+      k_hform_g<true, 1><<<nblk(s_form.n, 4), WAVE, 0, c.st>>>(wptr(s_form), s_form.n, D, dlev,
+                                                              zk.x, Pt.src, chg, dip, dv, Pt.ns,
+                                                              nd, per, mp);
+    else
+      (rz ? k_hform<true, 1> : k_hform<true, 0>)<<<s_form.n, WAVE, shm_mp, c.st>>>(
+          wptr(s_form), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, mp);
+  }
   tm->mark(c.st);
   // step 3: M2M
   int maxw = 1;
@@ -1025,9 +1227,17 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   }
   tm->mark(c.st);
   // step 2: list 4
-  if (s_l4.n)
-    (nr4 == JY_NTERMS0 ? k_hform<false, JY_NTERMS0> : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1> : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv,
-                                               Pt.ns, nd, CH, per, lo);
+  if (s_l4.n) {
+    if (fg && nr4 == JY_NTERMS0)
+      k_hform_g<false, JY_NTERMS0><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
+          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+    else if (fg && nr4 == JY_NTERMS1)
+      k_hform_g<false, JY_NTERMS1><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
+          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+    else
+      (nr4 == JY_NTERMS0 ? k_hform<false, JY_NTERMS0> : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1> : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(
+          wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
+  }
   tm->mark(c.st);
   // step 4: M2L (49 slot matrices per level, contiguous)
   for (int l = 2; l <= L; ++l) {
