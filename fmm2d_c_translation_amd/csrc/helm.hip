@@ -212,15 +212,17 @@ __device__ inline void eval_term(const EA& E, int n, V v, cd pos, cd neg, EvalAc
     return;
   }
   const int p = E.p;
-  if (n <= p) A.P += v * (E.c(n) * pos + E.c(-n) * neg);
+  // the order's value times its two phases, shared by every quantity
+  const cd wp = v * pos, wn = v * neg;
+  if (n <= p) A.P += E.c(n) * wp + E.c(-n) * wn;
   if (PG >= 2 && n <= p + 1) {
-    A.Gx += v * (E.dx(n) * pos + E.dx(-n) * neg);
-    A.Gy += v * (E.dy(n) * pos + E.dy(-n) * neg);
+    A.Gx += E.dx(n) * wp + E.dx(-n) * wn;
+    A.Gy += E.dy(n) * wp + E.dy(-n) * wn;
   }
   if (PG >= 3 && n <= p + 2) {
-    A.Hxx += v * (E.template d2<false, false>(n) * pos + E.template d2<false, false>(-n) * neg);
-    A.Hxy += v * (E.template d2<false, true>(n) * pos + E.template d2<false, true>(-n) * neg);
-    A.Hyy += v * (E.template d2<true, true>(n) * pos + E.template d2<true, true>(-n) * neg);
+    A.Hxx += E.template d2<false, false>(n) * wp + E.template d2<false, false>(-n) * wn;
+    A.Hxy += E.template d2<false, true>(n) * wp + E.template d2<false, true>(-n) * wn;
+    A.Hyy += E.template d2<true, true>(n) * wp + E.template d2<true, true>(-n) * wn;
   }
 }
 
@@ -231,7 +233,7 @@ __device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, cd e1, EvalA
   if (cabs_(z) < 1e-200) return;
   cd h0, h1;
   hank01(z, h0, h1);
-  cd pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  cd pos = mk(1, 0), neg = mk(1, 0);
   cd zi = inv(z) * rs;
   double s2 = rs * rs;
   cd vm = h0, v = h1 * rs;
@@ -239,7 +241,7 @@ __device__ void eval_mp_at(const EA& E, cd zk, double rs, double r, cd e1, EvalA
   const int top = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
   for (int n = 1; n <= top; ++n) {
     pos = pos * e1;
-    neg = neg * mi;
+    neg = (n & 1) ? -conj(pos) : conj(pos);  // (-e^{-i theta})^n
     eval_term<PG>(E, n, v, pos, neg, A);
     cd vn = zi * v * (2.0 * n) - vm * s2;
     vm = v;
@@ -505,14 +507,14 @@ __device__ void eval_mp_at_real(const EA& E, double k, double rs, double r, cd e
   if (x < 1e-200) return;
   const H01 hh = hank_real_point<NR>(x);
   const cd h0 = hh.h0, h1 = hh.h1;
-  cd pos = mk(1, 0), neg = mk(1, 0), mi = -conj(e1);
+  cd pos = mk(1, 0), neg = mk(1, 0);
   const double zi = rs / x, s2 = rs * rs;
   cd vm = h0, v = h1 * rs;
   eval_term<PG>(E, 0, h0, pos, neg, A);
   const int top = E.p + (PG >= 3 ? 2 : PG >= 2 ? 1 : 0);
   for (int n = 1; n <= top; ++n) {
     pos = pos * e1;
-    neg = neg * mi;
+    neg = (n & 1) ? -conj(pos) : conj(pos);  // (-e^{-i theta})^n
     eval_term<PG>(E, n, v, pos, neg, A);
     cd vn = v * (zi * (2.0 * n)) - vm * s2;
     vm = v;
@@ -571,14 +573,15 @@ __device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, cd e
 }
 
 // An expansion staged in LDS with its x and y derivative expansions
-// (value and first derivatives only): c(k) for |k| <= p+2 (zero beyond p),
-// dx(k), dy(k) for |k| <= p+1, each array addressed at its k = 0 entry.
+// (value and first derivatives only), interleaved as (c(k), dx(k), dy(k))
+// for |k| <= p+2 (c zero beyond p, the derivatives zero beyond p+1) and
+// addressed at the k = 0 triple.
 struct ExS {
-  const cd *e0, *ex, *ey;
+  const cd* e;
   int p;
-  __device__ cd c(int k) const { return e0[k]; }
-  __device__ cd dx(int k) const { return ex[k]; }
-  __device__ cd dy(int k) const { return ey[k]; }
+  __device__ cd c(int k) const { return e[3 * k]; }
+  __device__ cd dx(int k) const { return e[3 * k + 1]; }
+  __device__ cd dy(int k) const { return e[3 * k + 2]; }
   template <bool Y1, bool Y2>
   __device__ cd d2(int) const { return mk(0, 0); }  // not staged (PG <= 2 only)
 };
@@ -596,9 +599,7 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
   const int grp = (int)threadIdx.x >> 4, gl = (int)threadIdx.x & 15;
   const int w = blockIdx.x * 4 + grp;
   if (w >= nwork) return;  // the whole group leaves; no block barriers below
-  cd* E0 = reinterpret_cast<cd*>(shx) + (size_t)grp * 3 * W;
-  cd* EX = E0 + W;
-  cd* EY = EX + W;
+  cd* E3 = reinterpret_cast<cd*>(shx) + (size_t)grp * 3 * W;  // W triples
   const int b = work[w];
   const int2 rg = rng[b];
   const cd i4 = mk(0, 0.25);
@@ -617,25 +618,27 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
         const cd z1 = zk * (0.5 / rsd), z2 = zk * (0.5 * rsd);
         const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1);
         wave_sync();  // the group's previous readers are done
+        cd* c3 = E3 + 3 * (p + 2);  // the k = 0 triple
         for (int kk = gl; kk < 2 * p + 5; kk += 16) {
           const int q = kk - (p + 2);
-          E0[kk] = (q < -p || q > p) ? mk(0, 0) : ld(e + q + p);
+          c3[3 * q] = (q < -p || q > p) ? mk(0, 0) : ld(e + q + p);
+          c3[3 * q + 1] = mk(0, 0);
+          c3[3 * q + 2] = mk(0, 0);
         }
         wave_sync();
-        const cd* c0 = E0 + (p + 2);
         for (int kk = gl; kk < 2 * p + 3; kk += 16) {
           const int q = kk - (p + 1);
-          const cd a = c0[q + 1], bm = c0[q - 1];
+          const cd a = c3[3 * (q + 1)], bm = c3[3 * (q - 1)];
           cd gx, gy;
           if (q > 0) { gx = z2 * a - z1 * bm; gy = mul_i(z2 * a + z1 * bm); }
           else if (q < 0) { gx = z1 * a - z2 * bm; gy = mul_i(z1 * a + z2 * bm); }
           else { gx = z2 * (a - bm); gy = mul_i(z2 * (a + bm)); }
-          EX[kk] = gx;
-          EY[kk] = gy;
+          c3[3 * q + 1] = gx;
+          c3[3 * q + 2] = gy;
         }
         wave_sync();
         if (act) {
-          const ExS E{c0, EX + (p + 1), EY + (p + 1), p};
+          const ExS E{c3, p};
           const double2 cc = D.ctr[s];
           const double ddx = x.x - cc.x, ddy = x.y - cc.y;
           const double r = sqrt(ddx * ddx + ddy * ddy);
