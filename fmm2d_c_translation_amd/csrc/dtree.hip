@@ -132,8 +132,17 @@ __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
     const uint64_t x = k[i] ^ k[i + ndiv];
     c = x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
   }
-  for (int o = 32; o >= 1; o >>= 1) c = max(c, __shfl_xor(c, o));  // one atomic per wave
-  if ((threadIdx.x & 63) == 0 && c >= 0) atomicMax(depth, c);
+  // one atomic per block, and none when the block cannot raise the value
+  __shared__ int wmax[4];
+  for (int o = 32; o >= 1; o >>= 1) c = max(c, __shfl_xor(c, o));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = wmax[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = max(m, wmax[k]);
+    if (m >= 0 && m > __hip_atomic_load(depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(depth, m);
+  }
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }

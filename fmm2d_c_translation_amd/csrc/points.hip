@@ -225,10 +225,24 @@ __global__ void k_keys(const double2* p, int n, double cx, double cy, double bs0
 
 // Inside every leaf, restore original-index order (the reference's
 // partitions are stable).  One thread per leaf, insertion sort.
+// original indices of each leaf's points in increasing order: a wave per
+// leaf; a leaf of at most 64 points ranks its (distinct) indices through
+// shuffles, a larger one is sorted in place by one lane
 __global__ void k_leaf_sort(int* idx, const int2* rng, const int* leaves, int nleaves) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.x;
   if (w >= nleaves) return;
-  int2 r = rng[leaves[w]];
+  const int lane = threadIdx.x;
+  const int2 r = rng[leaves[w]];
+  const int n = r.y - r.x;
+  if (n <= 1) return;
+  if (n <= 64) {
+    const int v = lane < n ? idx[r.x + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += __shfl(v, j) < v ? 1 : 0;
+    if (lane < n) idx[r.x + rank] = v;
+    return;
+  }
+  if (lane != 0) return;
   for (int i = r.x + 1; i < r.y; ++i) {
     int v = idx[i], j = i - 1;
     while (j >= r.x && idx[j] > v) {
@@ -381,9 +395,9 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   const Span lss = plan.leaves_s.all(), lst = plan.leaves_t.all();
   // ---- original-index order inside leaves, then sorted coordinates
   if (lss.n)
-    k_leaf_sort<<<nblk(lss.n, 64), 64, 0, c.st>>>(P.perm_s, D.srng, plan.dwork + lss.off, lss.n);
+    k_leaf_sort<<<lss.n, 64, 0, c.st>>>(P.perm_s, D.srng, plan.dwork + lss.off, lss.n);
   if (lst.n)
-    k_leaf_sort<<<nblk(lst.n, 64), 64, 0, c.st>>>(P.perm_t, D.trng, plan.dwork + lst.off, lst.n);
+    k_leaf_sort<<<lst.n, 64, 0, c.st>>>(P.perm_t, D.trng, plan.dwork + lst.off, lst.n);
   P.src = S.get<double2>(ns);
   P.trg = S.get<double2>(nt);
   if (S.err) return S.err;
