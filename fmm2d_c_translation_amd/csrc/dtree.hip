@@ -7,7 +7,8 @@
 // by level.  The box set is the reference's:
 //   1. adaptive refinement (c_translation/src/pts_tree2d.c:530-610): a box
 //      is split while its point count (the length of its key range in the
-//      sorted key array) exceeds ndiv;
+//      sorted key array) exceeds ndiv; all split boxes of all levels are
+//      found in one pass over the sorted keys, each by its first point;
 //   2. level restriction (pts_tree2d.c:767-960 with tree_routs2d.c:197-318):
 //      the reference's flag passes split exactly the leaves that the 2:1
 //      rule forces ("every same-level neighbour of a split box exists"),
@@ -82,46 +83,6 @@ __device__ inline int2 key_range(const uint64_t* k, int n, uint64_t key, int lev
   return make_int2(a, b);
 }
 
-struct SplitPred {
-  const uint64_t *ks, *kt;
-  int ns, nt, level, idivflag, ndiv;
-  __device__ bool operator()(const uint64_t& key) const {
-    int cs = 0, ct = 0;
-    if (idivflag != 1) {
-      int2 r = key_range(ks, ns, key, level);
-      cs = r.y - r.x;
-    }
-    if (idivflag != 0) {
-      int2 r = key_range(kt, nt, key, level);
-      ct = r.y - r.x;
-    }
-    // idivflag outside 0..2 counts nothing, so nothing is split
-    int c = idivflag == 0 ? cs : idivflag == 1 ? ct : idivflag == 2 ? max(cs, ct) : 0;
-    return c > ndiv;
-  }
-};
-
-__global__ void k_children(const uint64_t* R, int n, uint64_t* K) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 4 * n) return;
-  K[i] = 4 * R[i >> 2] + (uint64_t)(i & 3);
-}
-
-// parents of the 8 in-domain neighbours of each split box of level k (an
-// out-of-domain neighbour repeats the box's own parent)
-__global__ void k_balance_cand(const uint64_t* R, int n, int k, uint64_t* out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int ix, iy;
-  key_cell(R[i], ix, iy);
-  const int side = 1 << k;
-  for (int s = 0; s < 9; ++s) {
-    int jx = ix + s % 3 - 1, jy = iy + s / 3 - 1;
-    bool in = jx >= 0 && jy >= 0 && jx < side && jy < side;
-    out[(size_t)9 * i + s] = (in ? cell_key(jx, jy) : R[i]) >> 2;
-  }
-}
-
 // Deepest split level: points i .. i+ndiv of the sorted keys share the
 // level-c box, c = the number of leading key digits the two ends agree on,
 // so that box holds more than ndiv points and is split; no deeper box is.
@@ -147,34 +108,56 @@ __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
-// split flags of the cnt (on the device) boxes of one level
-__global__ void k_split_flags(const uint64_t* K, const int* cnt, int cap, SplitPred pred,
-                              int* flag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cap) return;
-  flag[i] = (i < *cnt && pred(K[i])) ? 1 : 0;
+// leading base-4 digits (levels) two keys share, at most kMaxDepth
+__device__ inline int common_levels(uint64_t a, uint64_t b) {
+  const uint64_t x = a ^ b;
+  return x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
 }
 
-// split keys (sorted) and the next level's boxes; nR[l] and cnt[l+1] from
-// the last box; a split count above capR sets *overflow
-__global__ void k_split_scatter(const uint64_t* K, const int* cnt, const int* flag,
-                                const int* pos, int capR, uint64_t* R, uint64_t* Knext,
-                                int* nR, int* cnt_next, int* overflow) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = *cnt;
+// Adaptive refinement in one pass over the sorted keys.  Point i starts its
+// level-l box when it shares fewer than l digits with point i-1, and that
+// box holds more than ndiv points -- and is split -- when point i+ndiv
+// shares l digits with point i.  So the split boxes first met at point i
+// are those of levels a_i < l <= b_i: every split box of every level is
+// found once, by its first point, in Morton order within its level.  (A box
+// with more than ndiv points has ancestors with more, all split, so it is
+// a box of the tree.)  Levels are capped at the deepest split level Ls.
+__device__ inline int2 split_span(const uint64_t* k, int n, int ndiv, int Ls, int i) {
+  const int a = i == 0 ? -1 : common_levels(k[i - 1], k[i]);
+  const int b = i + ndiv < n ? min(common_levels(k[i], k[i + ndiv]), Ls) : -1;
+  return make_int2(a + 1, b);  // levels a+1 .. b
+}
+// cnt[0..n]: split boxes first met at each point (cnt[n] = 0, so an
+// exclusive scan over n+1 entries ends with the total)
+__global__ void k_split_count(const uint64_t* k, int n, int ndiv, int Ls, int* cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    cnt[n] = 0;
+    return;
+  }
+  const int2 r = split_span(k, n, ndiv, Ls, i);
+  cnt[i] = max(0, r.y - r.x + 1);
+}
+// the split boxes as (level << 2L) | Morton prefix, written from *base + off[i]
+// (cap bounds the output: at most n / (ndiv + 1) split boxes per level)
+__global__ void k_split_emit(const uint64_t* k, int n, int ndiv, int Ls, int L, const int* off,
+                             const int* base, long cap, uint64_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int p = pos[i];
-  if (flag[i] && p < capR) {
-    const uint64_t key = K[i];
-    R[p] = key;
-    for (int q = 0; q < 4; ++q) Knext[4 * p + q] = 4 * key + (uint64_t)q;
-  }
-  if (i == n - 1) {
-    const int nr = p + flag[i];
-    *nR = min(nr, capR);
-    *cnt_next = 4 * min(nr, capR);
-    if (nr > capR) *overflow = 1;
-  }
+  const int2 r = split_span(k, n, ndiv, Ls, i);
+  long o = off[i] + (base ? *base : 0);
+  for (int l = r.x; l <= r.y && o < cap; ++l)
+    out[o++] = ((uint64_t)l << (2 * L)) | (k[i] >> (2 * (kMaxDepth - l)));
+}
+__global__ void k_sum2(const int* a, const int* b, int* out) {
+  *out = (a ? *a : 0) + (b ? *b : 0);
+}
+__global__ void k_min_int(int* p, int v) { *p = min(*p, v); }
+// pad [*used, cap) with v (sorts after every split key)
+__global__ void k_pad_tail(uint64_t* a, const int* used, long cap, uint64_t v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap && i >= *used) a[i] = v;
 }
 
 // Level-restriction closure in one pass.  A split box r of level k forces
@@ -183,16 +166,15 @@ __global__ void k_split_scatter(const uint64_t* K, const int* cnt, const int* fl
 // level the forced boxes form a rectangle of at most 3 x 3 cells.  Written
 // as (level << 2L | key) for a tree of depth L, 9 slots per level below k
 // (short rectangles repeat their first cell); the adaptive split boxes
-// themselves come first.
+// themselves come first, already level-tagged.
 __global__ void k_closure_cand(const uint64_t* R, const int* roff, const long* cbase, int L,
                                uint64_t* out) {
   const int kLevShift = 2 * L;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= roff[L + 1]) return;
-  int k = 0;
-  while (k < L && r >= roff[k + 1]) ++k;
-  const uint64_t key = R[r];
-  out[r] = ((uint64_t)k << kLevShift) | key;
+  const int k = (int)(R[r] >> kLevShift);  // R: (level << 2L) | key, sorted
+  const uint64_t key = R[r] & ((uint64_t(1) << kLevShift) - 1);
+  out[r] = R[r];
   if (k < 2) return;
   uint64_t* o = out + cbase[k] + (long)(r - roff[k]) * 9 * (k - 1);
   int ix, iy;
@@ -619,15 +601,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                 int ifpghtarg, int sorted_levels, Plan& plan, double* host_s) {
   double t0 = wall();
   // ---- 0. depth of the adaptive tree (one read-back)
-  int* small0 = S.get<int>(2 * (kMaxDepth + 2) + 2);
+  int* d_depth = S.get<int>(1);  // deepest split level, -1 none
   if (S.err) return S.err;
-  int* d_depth = small0;                    // [0]: deepest split level, -1 none
-  int* d_over = small0 + 1;                 // [1]: overflow flag
-  int* d_cnt = small0 + 2;                  // [2 ..]: boxes per level
-  int* d_nR = small0 + 2 + (kMaxDepth + 2); // split boxes per level
-  HIP_TRY(hipMemsetAsync(small0, 0, sizeof(int) * (2 * (kMaxDepth + 2) + 2), c.st));
   HIP_TRY(hipMemsetAsync(d_depth, 0xff, sizeof(int), c.st));
-  k_set_int<<<1, 1, 0, c.st>>>(d_cnt, 1);
   if (idivflag != 1 && ns > ndiv)
     k_depth<<<nblk(ns - ndiv, 256), 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
   if ((idivflag == 1 || idivflag == 2) && nt > ndiv)
@@ -643,41 +619,71 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   if (Ls >= kMaxDepth) return IER_DEPTH;
   const int L = Ls + 1;
 
-  // ---- 1. adaptive refinement, level by level, counts kept on the device
-  const long npe = idivflag == 0 ? ns / (ndiv + 1)
-                 : idivflag == 1 ? nt / (ndiv + 1)
-                                 : ns / (ndiv + 1) + nt / (ndiv + 1);
-  std::vector<uint64_t*> R(L + 1, nullptr);
-  std::vector<int> capR(L + 1, 0);
-  uint64_t* K = S.get<uint64_t>(1);
+  // ---- 1. adaptive refinement in one pass over the points (split_span):
+  // per-point counts, a scan, the level-tagged split keys, one sort by
+  // (level, key) -- padded to a capacity, so no read-back -- and the
+  // per-level offsets.  With idivflag 2 both point sets contribute and
+  // duplicates are removed.
+  const bool use_s = idivflag == 0 || idivflag == 2, use_t = idivflag == 1 || idivflag == 2;
+  const long cap = std::max<long>(
+      1, (long)(Ls + 1) * ((use_s ? ns / (ndiv + 1) : 0) + (use_t ? nt / (ndiv + 1) : 0)));
+  const int kbits = 2 * L + 5;  // level (< 32) above 2L key bits
+  const uint64_t kPad = (uint64_t(1) << kbits) - 1;
+  uint64_t* Rs = S.get<uint64_t>(cap);   // split keys, sorted by (level, key)
+  int* d_roff = S.get<int>(L + 2);
+  int* d_nsplit = S.get<int>(2);
   if (S.err) return S.err;
-  HIP_TRY(hipMemsetAsync(K, 0, sizeof(uint64_t), c.st));
-  long capK = 1;
-  for (int l = 0; l <= Ls; ++l) {
-    capR[l] = (int)std::max<long>(1, std::min<long>(capK, npe));
-    int* flag = S.get<int>(capK);
-    int* pos = S.get<int>(capK);
-    R[l] = S.get<uint64_t>(capR[l]);
-    uint64_t* Kn = S.get<uint64_t>(4L * capR[l]);
+  {
+    uint64_t* Re = S.get<uint64_t>(cap);  // emitted, unsorted
+    int* cs = use_s ? S.get<int>((size_t)ns + 1) : nullptr;
+    int* os = use_s ? S.get<int>((size_t)ns + 1) : nullptr;
+    int* ct = use_t ? S.get<int>((size_t)nt + 1) : nullptr;
+    int* ot = use_t ? S.get<int>((size_t)nt + 1) : nullptr;
     if (S.err) return S.err;
-    SplitPred pred{ks, kt, ns, nt, l, idivflag, ndiv};
-    k_split_flags<<<nblk(capK, 256), 256, 0, c.st>>>(K, d_cnt + l, (int)capK, pred, flag);
-    size_t tb = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, pos, (int)capK, c.st));
-    void* tmp = S.get<char>(tb);
+    size_t tbs = 0, tbt = 0, tbr = 0, tbu = 0;
+    if (use_s) HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbs, cs, os, ns + 1, c.st));
+    if (use_t) HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbt, ct, ot, nt + 1, c.st));
+    HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(nullptr, tbr, Re, Rs, (int)cap, 0, kbits, c.st));
+    if (idivflag == 2)
+      HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tbu, Re, Rs, d_nsplit, (int)cap, c.st));
+    void* tmp = S.get<char>(std::max({tbs, tbt, tbr, tbu, size_t(1)}));
     if (S.err) return S.err;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)capK, c.st));
-    k_split_scatter<<<nblk(capK, 256), 256, 0, c.st>>>(K, d_cnt + l, flag, pos, capR[l], R[l], Kn,
-                                                      d_nR + l, d_cnt + l + 1, d_over);
-    K = Kn;
-    capK = 4L * capR[l];
+    int* d_used = d_nsplit + 1;  // emitted keys
+    HIP_TRY(hipMemsetAsync(d_used, 0, sizeof(int), c.st));
+    if (Ls >= 0) {
+      if (use_s) {
+        k_split_count<<<nblk(ns + 1, 256), 256, 0, c.st>>>(ks, ns, ndiv, Ls, cs);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tbs, cs, os, ns + 1, c.st));
+        k_split_emit<<<nblk(ns, 256), 256, 0, c.st>>>(ks, ns, ndiv, Ls, L, os, nullptr, cap, Re);
+      }
+      if (use_t) {
+        k_split_count<<<nblk(nt + 1, 256), 256, 0, c.st>>>(kt, nt, ndiv, Ls, ct);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tbt, ct, ot, nt + 1, c.st));
+        k_split_emit<<<nblk(nt, 256), 256, 0, c.st>>>(kt, nt, ndiv, Ls, L, ot,
+                                                      use_s ? os + ns : nullptr, cap, Re);
+      }
+      k_sum2<<<1, 1, 0, c.st>>>(use_s ? os + ns : nullptr, use_t ? ot + nt : nullptr, d_used);
+      k_min_int<<<1, 1, 0, c.st>>>(d_used, (int)std::min<long>(cap, 0x7fffffff));
+    }
+    k_pad_tail<<<(unsigned)((cap + 255) / 256), 256, 0, c.st>>>(Re, d_used, cap, kPad);
+    if (idivflag == 2) {
+      uint64_t* Rt = S.get<uint64_t>(cap);
+      if (S.err) return S.err;
+      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tbr, Re, Rt, (int)cap, 0, kbits, c.st));
+      HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tbu, Rt, Rs, d_nsplit, (int)cap, c.st));
+    } else {
+      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tbr, Re, Rs, (int)cap, 0, kbits, c.st));
+      k_set_int<<<1, 1, 0, c.st>>>(d_nsplit, (int)cap);
+    }
+    k_split_levels<<<1, 64, 0, c.st>>>(Rs, d_nsplit, L, d_roff);
   }
-  std::vector<int> hsm(2 * (kMaxDepth + 2) + 2);
-  e = sync_read(c, small0, (int)hsm.size(), hsm.data());
-  if (e) return e;
-  if (hsm[1]) return IER_ARGS;  // more split boxes than points allow: cannot happen
   std::vector<int> nR(L + 1, 0);
-  for (int l = 0; l <= Ls; ++l) nR[l] = hsm[2 + (kMaxDepth + 2) + l];
+  {
+    std::vector<int> ro(L + 2);
+    e = sync_read(c, d_roff, L + 2, ro.data());
+    if (e) return e;
+    for (int l = 0; l <= L; ++l) nR[l] = ro[l + 1] - ro[l];
+  }
   if (Ls >= 0 && nR[Ls] == 0) return IER_ARGS;  // the depth and the refinement disagree
   double t1 = wall();
 
@@ -693,7 +699,6 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
       cbase[l] = nc;
       if (l >= 2) nc += 9L * (l - 1) * nR[l];
     }
-    int* d_roff = S.get<int>(L + 2);
     long* d_cbase = S.get<long>(L + 1);
     uint64_t* cand = S.get<uint64_t>(std::max<long>(nc, 1));
     uint64_t* sorted = S.get<uint64_t>(std::max<long>(nc, 1));
@@ -701,21 +706,14 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     int* d_nu = S.get<int>(1);
     int* d_loff = S.get<int>(L + 2);
     if (S.err) return S.err;
-    int* hro = static_cast<int*>(pinned(sizeof(int) * (L + 2) + sizeof(long) * (L + 1) + 16, 5));
-    if (!hro) return IER_ALLOC;
-    std::memcpy(hro, roff.data(), sizeof(int) * (L + 2));
-    long* hcb = reinterpret_cast<long*>(reinterpret_cast<char*>(hro) + ((sizeof(int) * (L + 2) + 7) & ~size_t(7)));
+    long* hcb = static_cast<long*>(pinned(sizeof(long) * (L + 1), 5));
+    if (!hcb) return IER_ALLOC;
     std::memcpy(hcb, cbase.data(), sizeof(long) * (L + 1));
-    HIP_TRY(hipMemcpyAsync(d_roff, hro, sizeof(int) * (L + 2), hipMemcpyHostToDevice, c.st));
     HIP_TRY(hipMemcpyAsync(d_cbase, hcb, sizeof(long) * (L + 1), hipMemcpyHostToDevice, c.st));
-    for (int l = 0; l < L; ++l)
-      HIP_TRY(hipMemcpyAsync(cand + roff[l], R[l], sizeof(uint64_t) * nR[l], hipMemcpyDeviceToDevice, c.st));
-    // cand[0 .. roff[L]) holds the split keys without level bits; the kernel
-    // rewrites them with their level and appends the forced boxes
+    // cand[0 .. roff[L]): the split keys (level-tagged), then the forced boxes
     if (roff[L] > 0)
-      k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(cand, d_roff, d_cbase, L, cand);
+      k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(Rs, d_roff, d_cbase, L, cand);
     size_t tb1 = 0, tb2 = 0;
-    const int kbits = 2 * L + 5;  // level (< 32) above 2L key bits
     HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(nullptr, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     void* tmp = S.get<char>(std::max(tb1, tb2));
