@@ -63,8 +63,9 @@ FMM_HD cd cdiv(cd a, cd b) { return a * inv(b); }
 // below 1e-18 of the sum); k ln 2 in two parts (Cody and Waite), the high
 // part with 21 significant bits so that k ln2_hi is exact.  About thirty
 // instructions and a couple of ulp, without branches: zero gives -inf,
-// +inf gives +inf, negative and NaN arguments give NaN.
-FMM_HD double log_pos(double x) {
+// +inf gives +inf, negative and NaN arguments give NaN.  log_pos_core is
+// the same for a positive normal finite x, without the special cases.
+FMM_HD double log_pos_core(double x) {
   int k;
   double m = frexp(x, &k);  // [0.5, 1)
   if (m < 0.70710678118654752440) {
@@ -94,7 +95,10 @@ FMM_HD double log_pos(double x) {
   const double lm = fma(2.0 * s * s2, p, 2.0 * s);
   const double ln2_hi = 0x1.62e42p-1, ln2_lo = 0x1.fdf473de6af28p-22;
   const double kd = (double)k;
-  const double v = fma(kd, ln2_hi, fma(kd, ln2_lo, lm));
+  return fma(kd, ln2_hi, fma(kd, ln2_lo, lm));
+}
+FMM_HD double log_pos(double x) {
+  const double v = log_pos_core(x);
   const bool normal = x > 0.0 && x <= 1.7976931348623157e308;
   return normal ? v : x == 0.0 ? -HUGE_VAL : x > 0.0 ? x : NAN;
 }

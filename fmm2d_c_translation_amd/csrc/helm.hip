@@ -782,7 +782,50 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
           }
         }
         __syncthreads();
-        if (act) {
+        if (act && NR > 0 && PG <= 2) {
+          // real k, branch-free: a pair under the threshold (a coincident one
+          // included) runs at r = 1 with zero strengths, so adds exactly 0.
+          // With g = H1 / r and k H2 = 2 g - k H0, the charge and dipole
+          // gradients are -a (dx, dy) + b v, a = g c + q k H2 (d.v) / r^2,
+          // b = q g, and the dipole potential is b (d.v).
+          for (int m = k; m < cnt; m += K) {
+            const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
+            const double rr = dx * dx + dy * dy;
+            const double ri0 = rsqrt(fmax(rr, 0x1p-1022));
+            const double az0 = zk.x * (rr * ri0);
+            const bool skip = cp_mode ? az0 <= thresh : az0 < thresh;
+            const double az = skip ? 1.0 : az0, ri = skip ? 1.0 : ri0;
+            const double wt = skip ? 0.0 : 1.0;  // strengths are finite
+            cd h0, h1;
+            hank_series_real<(NR > 0 ? NR : 1), true>(az, ri * izk, h0, h1);
+            const cd g = h1 * ri;
+            cd a = mk(0, 0);
+            if (C) {
+              const double2 cv = sc[m];
+              const cd c = cd{cv.x * wt, cv.y * wt};
+              A.Pc += h0 * c;
+              if (PG >= 2) a = g * c;
+            }
+            if (Q) {
+              const double2 qv = sq[m];
+              const cd q = cd{qv.x * wt, qv.y * wt};
+              const double v1 = sv[m].x, v2 = sv[m].y;
+              const double dv = dx * v1 + dy * v2;
+              const cd bq = q * g;
+              A.Pd += bq * dv;
+              if (PG >= 2) {
+                const cd kh2 = g * 2.0 - h0 * zk.x;
+                a += (q * kh2) * (dv * (ri * ri));
+                A.Gx += bq * v1;
+                A.Gy += bq * v2;
+              }
+            }
+            if (PG >= 2) {
+              A.Gx -= a * dx;
+              A.Gy -= a * dy;
+            }
+          }
+        } else if (act) {
           for (int m = k; m < cnt; m += K) {
             double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
             double rr = dx * dx + dy * dy, r, ri;

@@ -73,7 +73,9 @@ FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
 // Horner with their derivatives, J1 = -P'(w) x/2 and
 // Y1 = -(2/pi) [J0/x - (ln(x/2) + gamma) J1 + Q'(w) x/2]
 // (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
-template <int N>
+// Core = true: x/2 must be a positive normal double (log without the
+// special cases).
+template <int N, bool Core = false>
 FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
   const double w = x * x * 0.25;
   double p = JY_JA[N - 1], dp = 0.0, q = JY_YC[N - 1], dq = 0.0;
@@ -84,7 +86,7 @@ FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
     dq = fma(dq, w, q);
     q = fma(q, w, JY_YC[k]);
   }
-  const double hx = 0.5 * x, lg = log_pos(hx) + kEuler;
+  const double hx = 0.5 * x, lg = (Core ? log_pos_core(hx) : log_pos(hx)) + kEuler;
   const double j1 = -dp * hx;
   const double y0 = (lg * p + q) * (2.0 / kPi);
   const double y1 = -(2.0 / kPi) * (p * xi - lg * j1 + dq * hx);
