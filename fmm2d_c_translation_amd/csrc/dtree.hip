@@ -576,6 +576,28 @@ __global__ void k_m2l_keys(Levels lv, const double2* ctr, const int* o2, const i
   val[e] = b;
 }
 
+// Keys that order the adjacent work lists list3_s, list3_t, leaves_s and
+// leaves_t (one sort over the four) by list, level, then decreasing cost: the
+// 16-point chunks of the box's points times, for list 3, its list-3 count
+// (capped at 255).  The evaluation kernels run four boxes per wave, so boxes
+// of equal cost side by side keep the wave's groups equally busy, and the
+// heaviest waves of a level start first.
+struct CostLists {
+  int end[4];  // end of list j, relative to the first
+};
+__global__ void k_cost_keys(const int* work, int n, CostLists cl, const int* level,
+                            const int2* srng, const int2* trng, const int* o3, uint32_t* key,
+                            int* val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = (i >= cl.end[0]) + (i >= cl.end[1]) + (i >= cl.end[2]);
+  const int b = work[i];
+  const int2 r = (j & 1) ? trng[b] : srng[b];
+  const int cost = ((r.y - r.x + 15) >> 4) * (j < 2 ? o3[b + 1] - o3[b] : 1);
+  key[i] = ((uint32_t)j << 13) | ((uint32_t)level[b] << 8) | (uint32_t)(255 - min(cost, 255));
+  val[i] = b;
+}
+
 // FMM2DHIP_DEBUG=1 in the environment prints the tree's shape per solve
 bool debug_enabled() {
   static const bool on = [] {
@@ -852,6 +874,31 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     if (S.err) return S.err;
     HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n,
                                                     0, kb, c.st));
+  }
+  // list-3 and leaf work in decreasing cost within each level (the four
+  // lists are adjacent in the work buffer)
+  static_assert(WL_LIST3_T == WL_LIST3_S + 1 && WL_LEAVES_S == WL_LIST3_T + 1 &&
+                    WL_LEAVES_T == WL_LEAVES_S + 1, "adjacent lists");
+  {
+    const int first = plan.list3_s.off;
+    CostLists cl;
+    const WorkList* q4[4] = {&plan.list3_s, &plan.list3_t, &plan.leaves_s, &plan.leaves_t};
+    for (int j = 0; j < 4; ++j) cl.end[j] = q4[j]->off + q4[j]->n - first;
+    const int n = cl.end[3];
+    if (n > 1) {
+      uint32_t* k0 = S.get<uint32_t>(n);
+      uint32_t* k1 = S.get<uint32_t>(n);
+      int* v0 = S.get<int>(n);
+      size_t tbs = 0;
+      HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tbs, k0, k1, v0, v0, n, 0, 15,
+                                                      c.st));
+      void* tmps = S.get<char>(tbs);
+      if (S.err) return S.err;
+      k_cost_keys<<<nblk(n, 256), 256, 0, c.st>>>(work + first, n, cl, level, srng, trng, lo.o3,
+                                                  k0, v0);
+      HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmps, tbs, k0, k1, v0, work + first, n, 0,
+                                                      15, c.st));
+    }
   }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
   const unsigned long long* hm = reinterpret_cast<const unsigned long long*>(hs.data() + m2l_at);

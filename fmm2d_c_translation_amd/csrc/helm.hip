@@ -572,34 +572,163 @@ __device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, cd e
   A.Hxx += B.Hxx * sc; A.Hxy += B.Hxy * sc; A.Hyy += B.Hyy * sc;
 }
 
-// An expansion staged in LDS with its x and y derivative expansions
-// (value and first derivatives only), interleaved as (c(k), dx(k), dy(k))
-// for |k| <= p+2 (c zero beyond p, the derivatives zero beyond p+1) and
-// addressed at the k = 0 triple.
-struct ExS {
-  const cd* e;
-  int p;
-  __device__ cd c(int k) const { return e[3 * k]; }
-  __device__ cd dx(int k) const { return e[3 * k + 1]; }
-  __device__ cd dy(int k) const { return e[3 * k + 2]; }
-  template <bool Y1, bool Y2>
-  __device__ cd d2(int) const { return mk(0, 0); }  // not staged (PG <= 2 only)
+// An expansion staged in LDS per order n = 0 .. top for the staged
+// evaluation: with pos = e^{i n theta} = C + i S and the negative order's
+// phase (-e^{-i theta})^n = (-1)^n conj(pos), a quantity Q (the expansion,
+// or its x or y derivative expansion) contributes v (Q(n) pos + Q(-n)
+// (-1)^n conj(pos)) = v (s C + d S) at order n, with
+//   s = Q(n) + (-1)^n Q(-n),  d = i (Q(n) - (-1)^n Q(-n))   (s = Q(0), d = 0
+// at n = 0).  Entries (s, d) of the expansion, then of the x and y
+// derivatives (PG >= 2), NQ = 2 or 6 per order.
+template <int PG>
+struct SdTerm {
+  static constexpr int NQ = PG >= 2 ? 6 : 2;
+  // v (s C + d S) for each staged quantity of order n
+  template <class V>
+  __device__ static void add(const cd* e, V v, double C, double S, EvalAcc& A) {
+    A.P += v * (e[0] * C + e[1] * S);
+    if (PG >= 2) {
+      A.Gx += v * (e[2] * C + e[3] * S);
+      A.Gy += v * (e[4] * C + e[5] * S);
+    }
+  }
 };
 
+// multipole at a point from the staged form, real positive k: H_n rs^n by
+// the upward recurrence from hank_series_real<NR> values
+template <int PG, int NR>
+__device__ void eval_mp_sd_real(const cd* E, int top, double k, double rs, double r, cd e1,
+                                EvalAcc& A) {
+  using T = SdTerm<PG>;
+  const double x = k * r;
+  if (x < 1e-200) return;
+  const H01 hh = hank_real_point<NR>(x);
+  const double zi = rs / x, s2 = rs * rs;
+  cd vm = hh.h0, v = hh.h1 * rs, pos = mk(1, 0);
+  T::add(E, hh.h0, 1.0, 0.0, A);
+  for (int n = 1; n <= top; ++n) {
+    pos = pos * e1;
+    T::add(E + T::NQ * n, v, pos.x, pos.y, A);
+    const cd vn = v * (zi * (2.0 * n)) - vm * s2;
+    vm = v;
+    v = vn;
+  }
+}
+
+// multipole at a point from the staged form, any k
+template <int PG>
+__device__ void eval_mp_sd(const cd* E, int top, cd zk, double rs, double r, cd e1, EvalAcc& A) {
+  using T = SdTerm<PG>;
+  const cd z = zk * r;
+  if (cabs_(z) < 1e-200) return;
+  cd h0, h1;
+  hank01(z, h0, h1);
+  const cd zi = inv(z) * rs;
+  const double s2 = rs * rs;
+  cd vm = h0, v = h1 * rs, pos = mk(1, 0);
+  T::add(E, h0, 1.0, 0.0, A);
+  for (int n = 1; n <= top; ++n) {
+    pos = pos * e1;
+    T::add(E + T::NQ * n, v, pos.x, pos.y, A);
+    const cd vn = zi * v * (2.0 * n) - vm * s2;
+    vm = v;
+    v = vn;
+  }
+}
+
+// local at a point from the staged form, real positive k: J_n / rs^n by
+// Miller's downward recurrence, normalised with J_0 + 2 sum_k J_2k = 1
+template <int PG>
+__device__ void eval_lo_sd_real(const cd* E, int top_n, double k, double rs, double r, cd e1,
+                                EvalAcc& A) {
+  using T = SdTerm<PG>;
+  const double x = k * r;
+  if (x < 1e-200) {  // J_0 = 1, others 0
+    T::add(E, 1.0, 1.0, 0.0, A);
+    return;
+  }
+  const int top = miller_top_real(top_n, x);
+  const double zr = rs / x, r2 = rs * rs;
+  cd pos = mk(1, 0);
+  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  const cd ie1 = conj(e1);
+  EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double gp = 0.0, g = 1.0, H = 0.0;  // H = sum over even n >= 2 of rs^n g_n (Horner)
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= top_n) {
+      T::add(E + T::NQ * n, g, pos.x, pos.y, B);
+      pos = pos * ie1;
+    }
+    H = rs * ((n & 1) ? H : g + H);
+    const double gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (g * g > 1e40) {
+      g *= 1e-40; gp *= 1e-40; H *= 1e-40;
+      B.P = B.P * 1e-40; B.Gx = B.Gx * 1e-40; B.Gy = B.Gy * 1e-40;
+    }
+  }
+  T::add(E, g, 1.0, 0.0, B);
+  const double sc = 1.0 / (g + 2.0 * H);
+  A.P += B.P * sc; A.Gx += B.Gx * sc; A.Gy += B.Gy * sc;
+}
+
+// local at a point from the staged form, any k (generating-function
+// normalisation, as eval_lo_at)
+template <int PG>
+__device__ void eval_lo_sd(const cd* E, int top_n, cd zk, double rs, double r, cd e1,
+                           EvalAcc& A) {
+  using T = SdTerm<PG>;
+  const cd z = zk * r;
+  if (cabs_(z) < 1e-200) {  // J_0 = 1, others 0
+    T::add(E, 1.0, 1.0, 0.0, A);
+    return;
+  }
+  const int top = miller_top(top_n, z);
+  const cd zr = inv(z) * rs;
+  const double r2 = rs * rs;
+  const cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1), ur = u * rs;
+  cd pos = mk(1, 0);
+  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  const cd ie1 = conj(e1);
+  EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  cd gp = mk(0, 0), g = mk(1, 0), H = mk(0, 0);
+  for (int n = top - 1; n >= 1; --n) {
+    if (n <= top_n) {
+      T::add(E + T::NQ * n, g, pos.x, pos.y, B);
+      pos = pos * ie1;
+    }
+    H = ur * (g + H);
+    const cd gm = zr * g * (2.0 * n) - gp * r2;
+    gp = g;
+    g = gm;
+    if (norm2(g) > 1e40) {
+      g = g * 1e-40; gp = gp * 1e-40; H = H * 1e-40;
+      B.P = B.P * 1e-40; B.Gx = B.Gx * 1e-40; B.Gy = B.Gy * 1e-40;
+    }
+  }
+  T::add(E, g, 1.0, 0.0, B);
+  const cd sc = inv((g + H * 2.0) * cexp_(-(u * z)));
+  A.P += B.P * sc; A.Gx += B.Gx * sc; A.Gy += B.Gy * sc;
+}
+
 // Four boxes per wave, 16 lanes per box, lane per point; for each source
-// expansion the box's 16 lanes stage it in LDS with its derivative
-// expansions (computed once per box instead of once per point and order),
-// then evaluate.  PG <= 2.  W >= 2 pmax + 5 entries per staged array.
+// expansion the box's 16 lanes stage it in LDS in the (s, d) form of
+// SdTerm (the derivative expansions computed once per box instead of once
+// per point and order), then evaluate.  The work list is ordered by
+// decreasing cost within each level, so the four boxes of a wave cost about
+// the same.  PG <= 2.  W >= NQ (pmax + 2) entries per group.
 template <bool MPK, int PG, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int2* rng,
           const double2* pts, int N, int nd, size_t per, const double2* ex, int W,
           double2* pot, double2* grad) {
+  using T = SdTerm<PG>;
   extern __shared__ double2 shx[];
   const int grp = (int)threadIdx.x >> 4, gl = (int)threadIdx.x & 15;
   const int w = blockIdx.x * 4 + grp;
   if (w >= nwork) return;  // the whole group leaves; no block barriers below
-  cd* E3 = reinterpret_cast<cd*>(shx) + (size_t)grp * 3 * W;  // W triples
+  cd* E = reinterpret_cast<cd*>(shx) + (size_t)grp * W;
   const int b = work[w];
   const int2 rg = rng[b];
   const cd i4 = mk(0, 0.25);
@@ -613,43 +742,59 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
       for (int k = 0; k < nsrc; ++k) {
         const int s = MPK ? D.l3[D.o3[b] + k] : b;
         const HLev lv = lev[D.level[s]];
-        const int p = lv.p;
+        const int p = lv.p, top = p + (PG >= 2 ? 1 : 0);
         const double rsd = MPK ? lv.rs : 1.0 / lv.rs;
         const cd z1 = zk * (0.5 / rsd), z2 = zk * (0.5 * rsd);
-        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1);
-        wave_sync();  // the group's previous readers are done
-        cd* c3 = E3 + 3 * (p + 2);  // the k = 0 triple
-        for (int kk = gl; kk < 2 * p + 5; kk += 16) {
-          const int q = kk - (p + 2);
-          c3[3 * q] = (q < -p || q > p) ? mk(0, 0) : ld(e + q + p);
-          c3[3 * q + 1] = mk(0, 0);
-          c3[3 * q + 2] = mk(0, 0);
-        }
-        wave_sync();
-        for (int kk = gl; kk < 2 * p + 3; kk += 16) {
-          const int q = kk - (p + 1);
-          const cd a = c3[3 * (q + 1)], bm = c3[3 * (q - 1)];
-          cd gx, gy;
+        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1) + p;  // order 0
+        auto co = [&](int q) { return (q < -p || q > p) ? mk(0, 0) : ld(e + q); };
+        // the derivative expansions at order q (from orders q - 1 and q + 1)
+        auto grads = [&](int q, cd& gx, cd& gy) {
+          const cd a = co(q + 1), bm = co(q - 1);
           if (q > 0) { gx = z2 * a - z1 * bm; gy = mul_i(z2 * a + z1 * bm); }
           else if (q < 0) { gx = z1 * a - z2 * bm; gy = mul_i(z1 * a + z2 * bm); }
           else { gx = z2 * (a - bm); gy = mul_i(z2 * (a + bm)); }
-          c3[3 * q + 1] = gx;
-          c3[3 * q + 2] = gy;
+        };
+        wave_sync();  // the group's previous readers are done
+        for (int n = gl; n <= top; n += 16) {
+          cd* o = E + T::NQ * n;
+          if (n == 0) {
+            o[0] = co(0);
+            o[1] = mk(0, 0);
+            if (PG >= 2) {
+              cd gx, gy;
+              grads(0, gx, gy);
+              o[2] = gx; o[3] = mk(0, 0);
+              o[4] = gy; o[5] = mk(0, 0);
+            }
+          } else {
+            const double sg = (n & 1) ? -1.0 : 1.0;
+            const cd cp = co(n), cm = co(-n) * sg;
+            o[0] = cp + cm;
+            o[1] = mul_i(cp - cm);
+            if (PG >= 2) {
+              cd gxp, gyp, gxm, gym;
+              grads(n, gxp, gyp);
+              grads(-n, gxm, gym);
+              gxm = gxm * sg;
+              gym = gym * sg;
+              o[2] = gxp + gxm; o[3] = mul_i(gxp - gxm);
+              o[4] = gyp + gym; o[5] = mul_i(gyp - gym);
+            }
+          }
         }
         wave_sync();
         if (act) {
-          const ExS E{c3, p};
           const double2 cc = D.ctr[s];
           const double ddx = x.x - cc.x, ddy = x.y - cc.y;
           const double r = sqrt(ddx * ddx + ddy * ddy);
           const double ri = r > 0.0 ? 1.0 / r : 0.0;
           const cd e1 = r > 0.0 ? mk(ddx * ri, ddy * ri) : mk(1, 0);  // e^{i theta}
           if (NR > 0) {
-            if (MPK) eval_mp_at_real<PG, (NR > 0 ? NR : 1)>(E, zk.x, lv.rs, r, e1, A);
-            else eval_lo_at_real<PG>(E, zk.x, lv.rs, r, e1, A);
+            if (MPK) eval_mp_sd_real<PG, (NR > 0 ? NR : 1)>(E, top, zk.x, lv.rs, r, e1, A);
+            else eval_lo_sd_real<PG>(E, top, zk.x, lv.rs, r, e1, A);
           } else {
-            if (MPK) eval_mp_at<PG>(E, zk, lv.rs, r, e1, A);
-            else eval_lo_at<PG>(E, zk, lv.rs, r, e1, A);
+            if (MPK) eval_mp_sd<PG>(E, top, zk, lv.rs, r, e1, A);
+            else eval_lo_sd<PG>(E, top, zk, lv.rs, r, e1, A);
           }
         }
       }
@@ -1013,11 +1158,11 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
     else nr = xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
   }
   const int nbx = (nw + 3) / 4;
-  // staged expansions (value and gradient) while three arrays of 2 pmax + 5
-  // entries per group fit comfortably in LDS
-  const int W = 2 * pmax + 5;
+  // staged expansions (value and gradient, NQ entries per order) while
+  // they fit comfortably in LDS
+  const int W = (pg >= 2 ? 6 : 2) * (pmax + 2);
   const bool staged = pg <= 2 && pmax <= 77;
-  const size_t shs = (size_t)4 * 3 * W * sizeof(double2);
+  const size_t shs = (size_t)4 * W * sizeof(double2);
 #define FMM_HEVAL(PGV, NRV)                                                                      \
   do {                                                                                           \
     if (staged && PGV <= 2)                                                                      \
