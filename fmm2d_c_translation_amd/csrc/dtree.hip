@@ -556,12 +556,13 @@ __global__ void k_wl_levels(Levels lv, const int* info, const int* out, int* lev
   lev[t] = lo;
 }
 
-// M2L list entries keyed by (level, child quadrant): a box of level l >= 1 is
-// child q = (id - loff[l]) & 3 of its parent
-// M2L grouping key of a target: level above the 49-bit mask of the list-2
-// offsets (slots) whose source box holds sources
+// M2L grouping key of a target: its level (5 bits) above a 19-bit hash
+// (Fibonacci hashing) of the 49-bit mask of the list-2 offsets (slots) whose
+// source box holds sources.  Equal masks get equal keys; a rare collision
+// only puts two masks in one group.
+constexpr int kM2lKeyBits = 5 + 19;
 __global__ void k_m2l_keys(Levels lv, const double2* ctr, const int* o2, const int* l2,
-                           const int2* srng, const int* list, int n, uint64_t* key, int* val) {
+                           const int2* srng, const int* list, int n, uint32_t* key, int* val) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int b = list[e], l = level_of(lv, b);
@@ -572,7 +573,7 @@ __global__ void k_m2l_keys(Levels lv, const double2* ctr, const int* o2, const i
     const int sl = m2l_slot(ctr[j], ctr[b], lv.bs[l]);
     if (sl >= 0) mask |= 1ull << sl;
   }
-  key[e] = ((uint64_t)l << 49) | mask;
+  key[e] = ((uint32_t)l << 19) | (uint32_t)((mask * 0x9e3779b97f4a7c15ull) >> (64 - 19));
   val[e] = b;
 }
 
@@ -860,13 +861,13 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   // only when no target of the tile has a source there
   if (plan.m2l.n > 0) {
     const int n = plan.m2l.n;
-    uint64_t* k0 = S.get<uint64_t>(n);
-    uint64_t* k1 = S.get<uint64_t>(n);
+    uint32_t* k0 = S.get<uint32_t>(n);
+    uint32_t* k1 = S.get<uint32_t>(n);
     int* v0 = S.get<int>(n);
     if (S.err) return S.err;
     k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, ctr, lo.o2, lo.l2, srng, work + plan.m2l.off,
                                                n, k0, v0);
-    const int kb = 49 + 5;  // level (< 32) above the 49 slot bits
+    const int kb = kM2lKeyBits;
     size_t tbs = 0;
     HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tbs, k0, k1, v0,
                                                     work + plan.m2l.off, n, 0, kb, c.st));
