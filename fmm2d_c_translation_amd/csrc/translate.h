@@ -187,7 +187,11 @@ k_translate_direct(const int* __restrict__ work, const int* __restrict__ map, in
 // B[l/16][l%16] = X[k0 + l/16] of column l%16, and accumulates
 // D[4v + l/16][l%16] (v = 0..3) -- the v_mfma_f64_16x16x4f64 operand layout.
 // A slot is skipped when none of the wave's 16 columns has a source there,
-// so tiles of work items that share a child quadrant waste no work.  Grid:
+// so tiles of work items that share a child quadrant waste no work: the
+// wave stages its 16 map rows in LDS and takes the set of used slots from
+// one ballot.  The operands of step k0 + 4 are loaded before the matrix
+// operations of step k0, and the last step of a slot loads the next used
+// slot's first operands.  nslot <= 64.  Grid:
 // x = column tiles (4 per block), y = row-tile groups, z = slot groups
 // (slots s = z, z + G, ...).  With G > 1 each slot group writes its partial
 // sums to part[(g nd + d) nwork + w][i] and k_xlate_reduce adds them up in
@@ -207,75 +211,101 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
              const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
              size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
              int G, double2* __restrict__ part) {
-  const int lane = threadIdx.x & 63;
+  __shared__ int smap[4][16 * 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
-  const int w = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + n;
+  const int tile = blockIdx.x * 4 + wv;
+  const int w = tile * 16 + n;
   const bool wok = w < nwork;
   const int rt0 = blockIdx.y * MW, g = blockIdx.z;
+  int* sm = smap[wv];
+  {
+    const size_t base = (size_t)tile * 16 * nslot, end = (size_t)nwork * nslot;
+    for (int e = lane; e < 16 * nslot; e += 64) sm[e] = base + e < end ? map[base + e] : -1;
+  }
+  __syncthreads();
+  bool used = false;
+  if (lane < nslot && lane % G == g)
+    for (int c = 0; c < 16; ++c) used |= sm[c * nslot + lane] >= 0;
+  const uint64_t act = __ballot(used);
   for (int d = 0; d < nd; ++d) {
-    mfma_d4 cr[MW], ci[MW];
+    mfma_d4 k1[MW], k2[MW], k3[MW];
 #pragma unroll
     for (int t = 0; t < MW; ++t) {
-      cr[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-      ci[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      k1[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      k2[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      k3[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
     }
     double s0r = 0.0, s0i = 0.0;  // sum of x_0 over the sources (lanes kq == 0)
-    for (int s = g; s < nslot; s += G) {
-      const int src = wok ? map[(size_t)w * nslot + s] : -1;
-      if (!__any(src >= 0)) continue;
+    int irow[MW];
+    bool rok[MW];
+#pragma unroll
+    for (int t = 0; t < MW; ++t) {
+      const int i = 16 * (rt0 + t) + n;
+      rok[t] = i < nout;
+      irow[t] = rok[t] ? i : nout - 1;
+    }
+    const int j0 = kq < nin ? kq : nin - 1;
+    uint64_t rem = act;
+    int s = rem ? __builtin_ctzll(rem) : 0;
+    int src = sm[n * nslot + s];
+    const double2* xs = X + (size_t)(src >= 0 ? src : 0) * xstride + (size_t)d * nin;
+    const double2* Ts = T + (size_t)s * nin * nout;
+    double2 xl = xs[j0];
+    double2 al[MW];
+#pragma unroll
+    for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)j0 * nout + irow[t]];
+    while (rem) {
+      rem &= rem - 1;
       const bool sok = src >= 0;
-      const double2* xs = X + (size_t)(sok ? src : 0) * xstride + (size_t)d * nin;
-      const double2* Ts = T + (size_t)s * nin * nout;
       if (sok && kq == 0) {
         const double2 x0 = xs[0];
         s0r += x0.x;
         s0i += x0.y;
       }
+      // the next used slot (its first operands load during this slot's last step)
+      const int sn = rem ? __builtin_ctzll(rem) : s;
+      const int srcn = sm[n * nslot + sn];
+      const double2* xsn = X + (size_t)(srcn >= 0 ? srcn : 0) * xstride + (size_t)d * nin;
+      const double2* Tsn = T + (size_t)sn * nin * nout;
       // Branch-free body: indexes are clamped into range and out-of-range
       // operands zeroed by selects, so the accumulators stay in one register
-      // class across the loop.  The operands of step k0 + 4 are loaded
-      // before the matrix operations of step k0 (the last step reloads a
-      // valid entry).
-      int irow[MW];
-      bool rok[MW];
-#pragma unroll
-      for (int t = 0; t < MW; ++t) {
-        const int i = 16 * (rt0 + t) + n;
-        rok[t] = i < nout;
-        irow[t] = rok[t] ? i : nout - 1;
-      }
-      double2 xl = xs[kq < nin ? kq : nin - 1];
-      double2 al[MW];
-#pragma unroll
-      for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)(kq < nin ? kq : nin - 1) * nout + irow[t]];
+      // class across the loop.
       for (int k0 = 0; k0 < nin; k0 += 4) {
         const bool jok = k0 + kq < nin;
         const bool xok = sok && jok;
         const double2 x = make_double2(xok ? xl.x : 0.0, xok ? xl.y : 0.0);
-        const double xin = -x.y;
-        double ar[MW], ai[MW];
+        const double b1 = x.x, b2 = x.y - x.x, b3 = x.x + x.y;
+        double ar[MW], ai[MW], as[MW];
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
           const bool aok = jok && rok[t];
           ar[t] = aok ? al[t].x : 0.0;
           ai[t] = aok ? al[t].y : 0.0;
+          as[t] = ar[t] + ai[t];
         }
-        const int jn = k0 + 4 + kq < nin ? k0 + 4 + kq : nin - 1;
-        xl = xs[jn];
+        if (k0 + 4 < nin) {
+          const int jn = k0 + 4 + kq < nin ? k0 + 4 + kq : nin - 1;
+          xl = xs[jn];
 #pragma unroll
-        for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)jn * nout + irow[t]];
+          for (int t = 0; t < MW; ++t) al[t] = Ts[(size_t)jn * nout + irow[t]];
+        } else {
+          xl = xsn[j0];
+#pragma unroll
+          for (int t = 0; t < MW; ++t) al[t] = Tsn[(size_t)j0 * nout + irow[t]];
+        }
         __builtin_amdgcn_sched_barrier(0);  // keep the next step's loads ahead
 #pragma unroll
         for (int t = 0; t < MW; ++t) {
-          cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], x.x, cr[t], 0, 0, 0);
-          ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], x.y, ci[t], 0, 0, 0);
-        }
-#pragma unroll
-        for (int t = 0; t < MW; ++t) {
-          cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], xin, cr[t], 0, 0, 0);
-          ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], x.x, ci[t], 0, 0, 0);
+          k1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[t], b1, k1[t], 0, 0, 0);
+          k2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], b2, k2[t], 0, 0, 0);
+          k3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], b3, k3[t], 0, 0, 0);
         }
       }
+      s = sn;
+      src = srcn;
+      xs = xsn;
+      Ts = Tsn;
     }
     if (!wok) continue;
     double2* y = G == 1 ? Y + (size_t)work[w] * ystride + (size_t)d * nout
@@ -286,7 +316,7 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
       for (int v = 0; v < 4; ++v) {
         const int i = 16 * (rt0 + t) + 4 * v + kq;
         if (i >= nout) continue;
-        double re = cr[t][v], im = ci[t][v];
+        double re = k1[t][v] - k3[t][v], im = k1[t][v] + k2[t][v];
         if (i == 0 && lap_log != 0.0) {
           re += s0r * lap_log;
           im += s0i * lap_log;
@@ -442,7 +472,7 @@ inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int 
                                int nd, double lap_log) {
   if (nwork <= 0) return hipSuccess;
   const int MT = (nout + 15) / 16;
-  if (MT > 4)
+  if (MT > 4 || nslot > 64)
     return launch_translate(st, work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, nd, lap_log);
   const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
   constexpr int kWaves = 4096;
@@ -452,7 +482,7 @@ inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int 
     const int per_g = ntile * MT;
     G = std::min(std::min(maxg, nslot), std::max(1, (kWaves + per_g - 1) / per_g));
   } else if (ntile < kWaves) {
-    mw = MT >= 2 ? 2 : 1;
+    mw = MT % 2 == 0 ? 2 : 1;  // no wave on padding rows only
   }
   double2* part = nullptr;
   if (G > 1) {
