@@ -1406,8 +1406,8 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // step 3: M2M
   int maxw = 1;
   for (int l = 0; l <= L; ++l)
-    maxw = std::max({maxw, plan.m2m.at(l).n, plan.m2l.at(l).n, plan.l2l_kids.at(l).n});
-  int* map = S.get<int>((size_t)maxw * 49);
+    maxw = std::max({maxw, plan.m2m.at(l).n, plan.l2l_kids.at(l).n});
+  int* map = S.get<int>((size_t)maxw * 4);  // M2M and L2L slot maps
   if (S.err) return S.err;
   for (int l = L - 1; l >= 2; --l) {
     const Span sp = plan.m2m.at(l);
@@ -1430,16 +1430,24 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
           wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
   }
   tm->mark(c.st);
-  // step 4: M2L (49 slot matrices per level, contiguous)
+  // step 4: M2L (49 slot matrices per level, contiguous); the slot maps of
+  // every level in one launch
+  int* map2 = S.get<int>((size_t)std::max(plan.m2l.n, 1) * 49);
+  if (S.err) return S.err;
+  if (plan.m2l.n) {
+    HIP_TRY(hipMemsetAsync(map2, 0xff, sizeof(int) * 49 * (size_t)plan.m2l.n, c.st));
+    k_map_m2l_all<<<nblk(32L * plan.m2l.n, 256), 256, 0, c.st>>>(W + plan.m2l.off, plan.m2l.n, D,
+                                                                map2);
+  }
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, D.srng, bs[l], map);
+    const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
     if (dT32)
-      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0],
+      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, dT + tab[l].m2l[0],
                                dT32 + tab[l].m2l[0], 2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
     else
-      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
+      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
                          per, lo, per, nd, 0.0));
   }
   tm->mark(c.st);

@@ -649,8 +649,8 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   // step 3: upward pass
   int maxw = 1;
   for (int l = 0; l <= L; ++l)
-    maxw = std::max({maxw, plan.m2m.at(l).n, plan.m2l.at(l).n, plan.l2l_kids.at(l).n});
-  int* map = S.get<int>((size_t)maxw * 49);
+    maxw = std::max({maxw, plan.m2m.at(l).n, plan.l2l_kids.at(l).n});
+  int* map = S.get<int>((size_t)maxw * 4);  // M2M and L2L slot maps
   if (S.err) return S.err;
   for (int l = L - 1; l >= 1; --l) {
     const Span sp = plan.m2m.at(l);
@@ -664,16 +664,24 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     k_form<false><<<nblk(plan.list4.n, 4), WAVE, 0, c.st>>>(W + plan.list4.off, plan.list4.n, D,
                                                            Pt.src, chg, dip, Pt.ns, nd, p, lo);
   tm->mark(c.st);
-  // step 4: M2L level by level (log of the box size enters entry (0,0) only)
+  // step 4: M2L level by level (log of the box size enters entry (0,0) only);
+  // the slot maps of every level in one launch
+  int* map2 = S.get<int>((size_t)std::max(plan.m2l.n, 1) * 49);
+  if (S.err) return S.err;
+  if (plan.m2l.n) {
+    HIP_TRY(hipMemsetAsync(map2, 0xff, sizeof(int) * 49 * (size_t)plan.m2l.n, c.st));
+    k_map_m2l_all<<<nblk(32L * plan.m2l.n, 256), 256, 0, c.st>>>(W + plan.m2l.off, plan.m2l.n, D,
+                                                                map2);
+  }
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
-    k_map_m2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.ctr, D.o2, D.l2, D.srng, plan.bs[l], map);
+    const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
     if (a.mixed)
-      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, map, TM, tab.m2l32, P, mp, per, lo, per,
+      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, TM, tab.m2l32, P, mp, per, lo, per,
                                nd, std::log(plan.bs[l])));
     else
-      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, map, TM, P, mp, per, lo, per, nd,
+      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, ml, TM, P, mp, per, lo, per, nd,
                          std::log(plan.bs[l])));
   }
   tm->mark(c.st);

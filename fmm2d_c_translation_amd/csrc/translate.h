@@ -103,23 +103,24 @@ __device__ inline int m2l_slot(double2 cs, double2 ct, double bs) {
   return (dy + 3) * 7 + (dx + 3);
 }
 
-// maps for the three translation kinds, one thread per work item
-// M2L: slot map from list 2
-// (list-2 boxes without sources have zero multipoles and are left out)
-__global__ void k_map_m2l(const int* work, int nwork, const double2* ctr, const int* o2,
-                          const int* l2, const int2* srng, double bs, int* map) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwork) return;
-  int b = work[w];
-  int* m = map + (size_t)w * 49;
-  for (int s = 0; s < 49; ++s) m[s] = -1;
-  for (int k = o2[b]; k < o2[b + 1]; ++k) {
-    const int j = l2[k];
-    if (srng[j].y <= srng[j].x) continue;
-    int s = m2l_slot(ctr[j], ctr[b], bs);
-    if (s >= 0) m[s] = j;
-  }
+// maps for the translation kinds
+// M2L slot maps of a whole M2L work list (every level) in one launch: row e
+// of map (49 entries, filled with -1 beforehand) gets, at each list-2
+// offset of work item e whose box holds sources, that box.  One thread per
+// (work item, list-2 entry), so the entry, source-range and centre loads of
+// one item run in parallel; list 2 holds at most 27 boxes.
+__global__ void k_map_m2l_all(const int* work, int nwork, DTree D, int* map) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = (int)(t >> 5), k = (int)(t & 31);
+  if (e >= nwork) return;
+  const int b = work[e], k0 = D.o2[b];
+  if (k >= D.o2[b + 1] - k0) return;
+  const int j = D.l2[k0 + k];
+  if (D.srng[j].y <= D.srng[j].x) return;
+  const int s = m2l_slot(D.ctr[j], D.ctr[b], D.bs[D.level[b]]);
+  if (s >= 0) map[(size_t)e * 49 + s] = j;
 }
+
 // M2M: slot q = child quadrant, children with sources only
 __global__ void k_map_m2m(const int* work, int nwork, const int4* child, const int2* srng,
                           int* map) {
