@@ -1439,6 +1439,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     k_map_m2l_all<<<nblk(32L * plan.m2l.n, 256), 256, 0, c.st>>>(W + plan.m2l.off, plan.m2l.n, D,
                                                                 map2);
   }
+  std::vector<M2LLevel> m2l_levels;
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
@@ -1447,9 +1448,9 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, dT + tab[l].m2l[0],
                                dT32 + tab[l].m2l[0], 2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
     else
-      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, mp,
-                         per, lo, per, nd, 0.0));
+      m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0});
   }
+  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);
   // step 5: L2L
   for (int l = 2; l <= L - 1; ++l) {

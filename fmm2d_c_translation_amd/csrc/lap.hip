@@ -673,6 +673,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     k_map_m2l_all<<<nblk(32L * plan.m2l.n, 256), 256, 0, c.st>>>(W + plan.m2l.off, plan.m2l.n, D,
                                                                 map2);
   }
+  std::vector<M2LLevel> m2l_levels;
   for (int l = 2; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
@@ -681,9 +682,9 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
       HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, TM, tab.m2l32, P, mp, per, lo, per,
                                nd, std::log(plan.bs[l])));
     else
-      HIP_TRY(launch_m2l(c.st, S, W + sp.off, sp.n, ml, TM, P, mp, per, lo, per, nd,
-                         std::log(plan.bs[l])));
+      m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, std::log(plan.bs[l])});
   }
+  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);
   // step 5: L2L into the children of boxes that hold targets
   for (int l = 1; l <= L - 1; ++l) {

@@ -14,6 +14,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace fmm {
 // every translation unit gets its own copy of these kernels and launcher
@@ -206,19 +207,21 @@ inline hipError_t launch_translate(hipStream_t st, const int* work, int nwork, c
                                    const double2* X, size_t xs, double2* Y, size_t ys, int nd,
                                    double lap_log);
 
+// One block of k_xlate_mfma: column-tile block bx, row-tile group by, slot
+// group g.
 template <int MW>
-__global__ void __launch_bounds__(256)
-k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
-             const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
-             size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
-             int G, double2* __restrict__ part) {
+__device__ __forceinline__ void xlate_block(
+    int bx, int by, int g, const int* __restrict__ work, int nwork, const int* __restrict__ map,
+    int nslot, const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+    size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log, int G,
+    double2* __restrict__ part) {
   __shared__ int smap[4][16 * 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
-  const int tile = blockIdx.x * 4 + wv;
+  const int tile = bx * 4 + wv;
   const int w = tile * 16 + n;
   const bool wok = w < nwork;
-  const int rt0 = blockIdx.y * MW, g = blockIdx.z;
+  const int rt0 = by * MW;
   int* sm = smap[wv];
   {
     const size_t base = (size_t)tile * 16 * nslot, end = (size_t)nwork * nslot;
@@ -326,6 +329,45 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
         else y[i] = make_double2(re, im);
       }
   }
+}
+
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
+             const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+             size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
+             int G, double2* __restrict__ part) {
+  xlate_block<MW>(blockIdx.x, blockIdx.y, blockIdx.z, work, nwork, map, nslot, T, nin, nout, X,
+                  xstride, Y, ystride, nd, lap_log, G, part);
+}
+
+// Several M2L levels in one launch (they read and write disjoint expansions):
+// level j owns the blocks [block0, next level's block0), a wave per 16
+// targets with every row tile (MW = the levels' common row-tile count) and
+// every slot.
+struct XLevel {
+  const int* work;
+  int nwork;
+  const int* map;
+  const double2* T;
+  int nin, nout;
+  double lap_log;
+  int block0;
+};
+constexpr int kMaxXLevels = 30;
+struct XLevels {
+  XLevel lv[kMaxXLevels];
+  int n;
+};
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_xlate_mfma_levels(XLevels L, int nslot, const double2* __restrict__ X, size_t xstride,
+                    double2* __restrict__ Y, size_t ystride, int nd) {
+  int j = 0;
+  while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
+  const XLevel& v = L.lv[j];
+  xlate_block<MW>(blockIdx.x - v.block0, 0, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
+                  v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr);
 }
 
 // Y[work[w]] += sum over slot groups of the partials, group 0 first
@@ -513,6 +555,48 @@ inline hipError_t launch_m2l(hipStream_t st, Scratch& S, const int* work, int nw
                              const int* map, const double2* T, int P, const double2* X, size_t xs,
                              double2* Y, size_t ys, int nd, double lap_log) {
   return launch_xlate(st, S, work, nwork, map, 49, 27, T, P, P, X, xs, Y, ys, nd, lap_log);
+}
+
+// FP64 M2L of several levels.  Levels whose order fits the MFMA tiles
+// (P <= 64) are grouped by row-tile count, and each group runs as one
+// k_xlate_mfma_levels launch, so the small levels share the GPU with the
+// large ones instead of running alone; larger orders go through
+// launch_translate level by level.
+struct M2LLevel {
+  const int* work;
+  int nwork;
+  const int* map;     // 49 slots per work item
+  const double2* T;   // the level's 49 operators, P x P each
+  int P;
+  double lap_log;
+};
+inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
+                                    const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
+  for (int MT = 1; MT <= 4; ++MT) {
+    XLevels L;
+    L.n = 0;
+    int nb = 0;
+    for (const M2LLevel& v : lv) {
+      if (v.nwork <= 0 || (v.P + 15) / 16 != MT) continue;
+      if (L.n == kMaxXLevels) return hipErrorInvalidValue;
+      L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, nb};
+      nb += ((v.nwork + 15) / 16 + 3) / 4;
+    }
+    if (!L.n) continue;
+    switch (MT) {
+      case 1: k_xlate_mfma_levels<1><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+      case 2: k_xlate_mfma_levels<2><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+      case 3: k_xlate_mfma_levels<3><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+      default: k_xlate_mfma_levels<4><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+    }
+  }
+  for (const M2LLevel& v : lv) {
+    if (v.nwork <= 0 || (v.P + 15) / 16 <= 4) continue;
+    const hipError_t e = launch_translate(st, v.work, v.nwork, v.map, 49, v.T, v.P, v.P, X, xs, Y,
+                                          ys, nd, v.lap_log);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
 }
 
 // Mixed-precision M2L: k_xlate_mfma32 with the FP32 operators T32 when the
