@@ -620,8 +620,8 @@ int sync_read(Ctx& c, const int* d, int n, int* h) {
 }  // namespace
 
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
-                double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
-                int ifpghtarg, int sorted_levels, Plan& plan, double* host_s) {
+                const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
+                int sorted_levels, Plan& plan, double* host_s) {
   double t0 = wall();
   // ---- 0. depth of the adaptive tree (one read-back)
   int* d_depth = S.get<int>(1);  // deepest split level, -1 none
@@ -758,9 +758,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   // ---- 3. boxes
   Levels lv{};
   lv.L = L;
-  lv.c0x = c0x;
-  lv.c0y = c0y;
-  lv.bs[0] = bs0;
+  lv.c0x = geo[0];  // valid: the read-backs above synchronised the stream
+  lv.c0y = geo[1];
+  lv.bs[0] = geo[2];
   for (int l = 1; l <= L; ++l) lv.bs[l] = lv.bs[l - 1] / 2;
   lv.loff[0] = 0;
   lv.roff[0] = 0;

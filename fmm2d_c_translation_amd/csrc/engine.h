@@ -115,13 +115,15 @@ struct Plan {
 
 // Device tree and lists from keys sorted on their top 2 sorted_levels bits
 // (dtree.hip).  Returns IER_RESORT, before building anything, when the tree
-// needs more levels than the keys are sorted on.  host_s receives the wall
+// needs more levels than the keys are sorted on.  geo: pinned host copy of
+// the root box {centre x, centre y, size}, written by a copy already queued
+// on the stream (read after the first read-back).  host_s receives the wall
 // time of refinement, level restriction and boxes+lists.
 constexpr int kSortLevels = 16;
 constexpr int IER_RESORT = -1;  // internal: sort on more key bits and retry
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
-                double c0x, double c0y, double bs0, int idivflag, int ndiv, int ifpgh,
-                int ifpghtarg, int sorted_levels, Plan& plan, double* host_s);
+                const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
+                int sorted_levels, Plan& plan, double* host_s);
 
 // Pinned host staging reused across the calls of one thread (slot < 8).
 void* pinned(size_t bytes, int slot);

@@ -214,12 +214,42 @@ __global__ void k_bbox_partial(const double2* a, int na, const double2* b, int n
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_keys(const double2* p, int n, double cx, double cy, double bs0,
-                       uint64_t* key, int* idx) {
+// the partial boxes reduced (min and max are exact in any order) to the
+// root box: geo = {centre x, centre y, size}, as pts_tree2d.c:493-516
+__global__ void __launch_bounds__(1024) k_bbox_final(const double4* part, int nbk, double* geo) {
+  __shared__ double4 sh[1024];
+  double4 m = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+  for (int i = threadIdx.x; i < nbk; i += blockDim.x) {
+    const double4 o = part[i];
+    m.x = fmin(m.x, o.x); m.y = fmax(m.y, o.y);
+    m.z = fmin(m.z, o.z); m.w = fmax(m.w, o.w);
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double4 o = sh[threadIdx.x + s];
+      m.x = fmin(m.x, o.x); m.y = fmax(m.y, o.y);
+      m.z = fmin(m.z, o.z); m.w = fmax(m.w, o.w);
+      sh[threadIdx.x] = m;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double bs0 = m.y - m.x;
+    if (m.w - m.z > bs0) bs0 = m.w - m.z;
+    geo[0] = (m.x + m.y) / 2;
+    geo[1] = (m.z + m.w) / 2;
+    geo[2] = bs0;
+  }
+}
+
+// geo = {centre x, centre y, size} of the root box, on the device
+__global__ void k_keys(const double2* p, int n, const double* geo, uint64_t* key, int* idx) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double2 q = p[i];
-  key[i] = point_key(q.x, q.y, cx, cy, bs0);
+  key[i] = point_key(q.x, q.y, geo[0], geo[1], geo[2]);
   idx[i] = i;
 }
 
@@ -297,15 +327,15 @@ using OnesweepSort =
                                rocprim::default_config, 0>;
 
 // keys and the key-sorted permutation of one point set (keys stay on the device)
-int sort_set(Ctx& c, Scratch& S, const double2* pts, int n, double cx, double cy, double bs0,
-             int begin_bit, int* perm, uint64_t** sorted) {
+int sort_set(Ctx& c, Scratch& S, const double2* pts, int n, const double* geo, int begin_bit,
+             int* perm, uint64_t** sorted) {
   uint64_t* k0 = S.get<uint64_t>(n);
   uint64_t* k1 = S.get<uint64_t>(n);
   int* i0 = S.get<int>(n);
   if (S.err) return S.err;
   *sorted = k1;
   if (n == 0) return IER_OK;
-  k_keys<<<nblk(n, 256), 256, 0, c.st>>>(pts, n, cx, cy, bs0, k0, i0);
+  k_keys<<<nblk(n, 256), 256, 0, c.st>>>(pts, n, geo, k0, i0);
   // stable LSD radix sort (onesweep) of (key, original index) on key bits
   // [begin_bit, 2 kMaxDepth)
   size_t tb = 0;
@@ -348,20 +378,15 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   int nbk = std::min(1024, std::max(1, nblk(ns + nt, kBB)));
   double4* part = S.get<double4>(nbk);
   if (S.err) return S.err;
+  // The root box stays on the device for the keys; its host copy is valid
+  // after the tree's first read-back synchronises the stream.
+  double* d_geo = S.get<double>(3);
+  if (S.err) return S.err;
   k_bbox_partial<<<nbk, kBB, 0, c.st>>>(src, ns, trg, nt, part);
-  double4* hp = static_cast<double4*>(pinned(sizeof(double4) * nbk, 2));
-  if (!hp) return IER_ALLOC;
-  HIP_TRY(hipMemcpyAsync(hp, part, sizeof(double4) * nbk, hipMemcpyDeviceToHost, c.st));
-  HIP_TRY(hipStreamSynchronize(c.st));
-  double xmin = hp[0].x, xmax = hp[0].y, ymin = hp[0].z, ymax = hp[0].w;
-  for (int i = 1; i < nbk; ++i) {
-    xmin = std::min(xmin, hp[i].x); xmax = std::max(xmax, hp[i].y);
-    ymin = std::min(ymin, hp[i].z); ymax = std::max(ymax, hp[i].w);
-  }
-  P.bs0 = xmax - xmin;
-  if (ymax - ymin > P.bs0) P.bs0 = ymax - ymin;
-  P.c0x = (xmin + xmax) / 2;
-  P.c0y = (ymin + ymax) / 2;
+  k_bbox_final<<<1, 1024, 0, c.st>>>(part, nbk, d_geo);
+  double* h_geo = static_cast<double*>(pinned(sizeof(double) * 3, 2));
+  if (!h_geo) return IER_ALLOC;
+  HIP_TRY(hipMemcpyAsync(h_geo, d_geo, sizeof(double) * 3, hipMemcpyDeviceToHost, c.st));
   th[1] = wall();
 
   // ---- keys + sort.  Boxes down to level D only need the keys ordered by
@@ -377,17 +402,20 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   for (int levels : {kSortLevels, kMaxDepth}) {
     const int begin_bit = 2 * (kMaxDepth - levels);
     uint64_t *ks = nullptr, *kt = nullptr;
-    e = sort_set(c, S, src, ns, P.c0x, P.c0y, P.bs0, begin_bit, P.perm_s, &ks);
+    e = sort_set(c, S, src, ns, d_geo, begin_bit, P.perm_s, &ks);
     if (e) return e;
-    e = sort_set(c, S, trg, nt, P.c0x, P.c0y, P.bs0, begin_bit, P.perm_t, &kt);
+    e = sort_set(c, S, trg, nt, d_geo, begin_bit, P.perm_t, &kt);
     if (e) return e;
     th[2] = wall();
     // ---- tree, lists and work lists on the device
-    e = build_dtree(c, S, ks, ns, kt, nt, P.c0x, P.c0y, P.bs0, idivflag, ndiv, ifpgh, ifpghtarg,
+    e = build_dtree(c, S, ks, ns, kt, nt, h_geo, idivflag, ndiv, ifpgh, ifpghtarg,
                     levels, plan, tt);
     if (e != IER_RESORT) break;
   }
   if (e) return e;
+  P.c0x = h_geo[0];  // the tree's read-backs have synchronised the stream
+  P.c0y = h_geo[1];
+  P.bs0 = h_geo[2];
   th[3] = th[2] + tt[0];
   th[4] = th[3] + tt[1];
   const DTree& D = plan.D;
