@@ -106,8 +106,6 @@ __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
   }
 }
 
-__global__ void k_set_int(int* p, int v) { *p = v; }
-
 // leading base-4 digits (levels) two keys share, at most kMaxDepth
 __device__ inline int common_levels(uint64_t a, uint64_t b) {
   const uint64_t x = a ^ b;
@@ -150,14 +148,18 @@ __global__ void k_split_emit(const uint64_t* k, int n, int ndiv, int Ls, int L, 
   for (int l = r.x; l <= r.y && o < cap; ++l)
     out[o++] = ((uint64_t)l << (2 * L)) | (k[i] >> (2 * (kMaxDepth - l)));
 }
-__global__ void k_sum2(const int* a, const int* b, int* out) {
-  *out = (a ? *a : 0) + (b ? *b : 0);
-}
-__global__ void k_min_int(int* p, int v) { *p = min(*p, v); }
-// pad [*used, cap) with v (sorts after every split key)
-__global__ void k_pad_tail(uint64_t* a, const int* used, long cap, uint64_t v) {
+// used = the split keys emitted (the totals of the two scans, capped at
+// cap); pads [used, cap) with v, which sorts after every split key.  Thread
+// 0 records used, and cap in *nsplit when no unique pass follows.
+__global__ void k_pad_tail(uint64_t* a, const int* tot_s, const int* tot_t, long cap, uint64_t v,
+                           int* used, int* nsplit) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cap && i >= *used) a[i] = v;
+  const long u = min(cap, (long)(tot_s ? *tot_s : 0) + (tot_t ? *tot_t : 0));
+  if (i == 0) {
+    *used = (int)u;
+    if (nsplit) *nsplit = (int)cap;
+  }
+  if (i < cap && i >= u) a[i] = v;
 }
 
 // Level-restriction closure in one pass.  A split box r of level k forces
@@ -650,7 +652,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   const bool use_s = idivflag == 0 || idivflag == 2, use_t = idivflag == 1 || idivflag == 2;
   const long cap = std::max<long>(
       1, (long)(Ls + 1) * ((use_s ? ns / (ndiv + 1) : 0) + (use_t ? nt / (ndiv + 1) : 0)));
-  const int kbits = 2 * L + 5;  // level (< 32) above 2L key bits
+  // level above 2L key bits, in as many bits as L + 1 needs: the pad key's
+  // level field (all ones) then exceeds every level the splits can have
+  const int kbits = 2 * L + (32 - __builtin_clz((unsigned)(L + 1)));
   const uint64_t kPad = (uint64_t(1) << kbits) - 1;
   uint64_t* Rs = S.get<uint64_t>(cap);   // split keys, sorted by (level, key)
   int* d_roff = S.get<int>(L + 2);
@@ -672,7 +676,6 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     void* tmp = S.get<char>(std::max({tbs, tbt, tbr, tbu, size_t(1)}));
     if (S.err) return S.err;
     int* d_used = d_nsplit + 1;  // emitted keys
-    HIP_TRY(hipMemsetAsync(d_used, 0, sizeof(int), c.st));
     if (Ls >= 0) {
       if (use_s) {
         k_split_count<<<nblk(ns + 1, 256), 256, 0, c.st>>>(ks, ns, ndiv, Ls, cs);
@@ -685,10 +688,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
         k_split_emit<<<nblk(nt, 256), 256, 0, c.st>>>(kt, nt, ndiv, Ls, L, ot,
                                                       use_s ? os + ns : nullptr, cap, Re);
       }
-      k_sum2<<<1, 1, 0, c.st>>>(use_s ? os + ns : nullptr, use_t ? ot + nt : nullptr, d_used);
-      k_min_int<<<1, 1, 0, c.st>>>(d_used, (int)std::min<long>(cap, 0x7fffffff));
     }
-    k_pad_tail<<<(unsigned)((cap + 255) / 256), 256, 0, c.st>>>(Re, d_used, cap, kPad);
+    k_pad_tail<<<(unsigned)((cap + 255) / 256), 256, 0, c.st>>>(
+        Re, Ls >= 0 && use_s ? os + ns : nullptr, Ls >= 0 && use_t ? ot + nt : nullptr, cap, kPad,
+        d_used, idivflag == 2 ? nullptr : d_nsplit);
     if (idivflag == 2) {
       uint64_t* Rt = S.get<uint64_t>(cap);
       if (S.err) return S.err;
@@ -696,7 +699,6 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tbu, Rt, Rs, d_nsplit, (int)cap, c.st));
     } else {
       HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tbr, Re, Rs, (int)cap, 0, kbits, c.st));
-      k_set_int<<<1, 1, 0, c.st>>>(d_nsplit, (int)cap);
     }
     k_split_levels<<<1, 64, 0, c.st>>>(Rs, d_nsplit, L, d_roff);
   }
