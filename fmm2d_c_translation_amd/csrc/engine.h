@@ -119,7 +119,7 @@ struct Plan {
 // the root box {centre x, centre y, size}, written by a copy already queued
 // on the stream (read after the first read-back).  host_s receives the wall
 // time of refinement, level restriction and boxes+lists.
-constexpr int kSortLevels = 16;
+constexpr int kSortLevels = 19;  // with the target flag, 39 bits: five 8-bit passes
 constexpr int IER_RESORT = -1;  // internal: sort on more key bits and retry
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
                 const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,

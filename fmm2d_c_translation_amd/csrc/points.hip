@@ -244,13 +244,23 @@ __global__ void __launch_bounds__(1024) k_bbox_final(const double4* part, int nb
   }
 }
 
-// geo = {centre x, centre y, size} of the root box, on the device
-__global__ void k_keys(const double2* p, int n, const double* geo, uint64_t* key, int* idx) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double2 q = p[i];
-  key[i] = point_key(q.x, q.y, geo[0], geo[1], geo[2]);
-  idx[i] = i;
+// Keys of both point sets in one array, targets flagged just above the
+// 2 kMaxDepth key bits, so that one sort leaves the sources and then the
+// targets, each in key order; idx is the index within the point's own set.
+// geo = {centre x, centre y, size} of the root box, on the device.
+constexpr uint64_t kTargetFlag = uint64_t(1) << (2 * kMaxDepth);
+__global__ void k_keys(const double2* s, int ns, const double2* t, int nt, const double* geo,
+                       uint64_t* key, int* idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns + nt) return;
+  const bool tg = i >= ns;
+  const double2 q = tg ? t[i - ns] : s[i];
+  key[i] = point_key(q.x, q.y, geo[0], geo[1], geo[2]) | (tg ? kTargetFlag : 0);
+  idx[i] = tg ? i - ns : i;
+}
+__global__ void k_unflag(uint64_t* key, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] &= kTargetFlag - 1;
 }
 
 // Inside every leaf, restore original-index order (the reference's
@@ -326,25 +336,29 @@ using OnesweepSort =
     rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                rocprim::default_config, 0>;
 
-// keys and the key-sorted permutation of one point set (keys stay on the device)
-int sort_set(Ctx& c, Scratch& S, const double2* pts, int n, const double* geo, int begin_bit,
-             int* perm, uint64_t** sorted) {
+// Keys and key-sorted permutations of both point sets (one stable LSD radix
+// sort, onesweep, on key bits [begin_bit, 2 kMaxDepth) and the target
+// flag): perm[0 .. ns) for the sources, perm[ns .. ns + nt) for the
+// targets; *ks and *kt are the sorted keys of each set (on the device).
+int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* trg, int nt,
+                const double* geo, int begin_bit, int* perm, uint64_t** ks, uint64_t** kt) {
+  const int n = ns + nt;
   uint64_t* k0 = S.get<uint64_t>(n);
   uint64_t* k1 = S.get<uint64_t>(n);
   int* i0 = S.get<int>(n);
   if (S.err) return S.err;
-  *sorted = k1;
+  *ks = k1;
+  *kt = k1 + ns;
   if (n == 0) return IER_OK;
-  k_keys<<<nblk(n, 256), 256, 0, c.st>>>(pts, n, geo, k0, i0);
-  // stable LSD radix sort (onesweep) of (key, original index) on key bits
-  // [begin_bit, 2 kMaxDepth)
+  k_keys<<<nblk(n, 256), 256, 0, c.st>>>(src, ns, trg, nt, geo, k0, i0);
   size_t tb = 0;
   HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tb, k0, k1, i0, perm, n, begin_bit,
-                                                  2 * kMaxDepth, c.st));
+                                                  2 * kMaxDepth + 1, c.st));
   void* tmp = S.get<char>(tb);
   if (S.err) return S.err;
   HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmp, tb, k0, k1, i0, perm, n, begin_bit,
-                                                  2 * kMaxDepth, c.st));
+                                                  2 * kMaxDepth + 1, c.st));
+  if (nt) k_unflag<<<nblk(nt, 256), 256, 0, c.st>>>(k1 + ns, nt);
   return IER_OK;
 }
 
@@ -394,17 +408,15 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   // order inside a leaf is restored below), so the sort first covers
   // kSortLevels levels and is redone on every bit when the tree turns out
   // deeper.
-  P.perm_s = S.get<int>(ns);
-  P.perm_t = S.get<int>(nt);
+  P.perm_s = S.get<int>((size_t)ns + nt);
   if (S.err) return S.err;
+  P.perm_t = P.perm_s + ns;
   double tt[3] = {0, 0, 0};
   int e = IER_OK;
   for (int levels : {kSortLevels, kMaxDepth}) {
     const int begin_bit = 2 * (kMaxDepth - levels);
     uint64_t *ks = nullptr, *kt = nullptr;
-    e = sort_set(c, S, src, ns, d_geo, begin_bit, P.perm_s, &ks);
-    if (e) return e;
-    e = sort_set(c, S, trg, nt, d_geo, begin_bit, P.perm_t, &kt);
+    e = sort_points(c, S, src, ns, trg, nt, d_geo, begin_bit, P.perm_s, &ks, &kt);
     if (e) return e;
     th[2] = wall();
     // ---- tree, lists and work lists on the device
