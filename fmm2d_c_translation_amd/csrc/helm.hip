@@ -1404,15 +1404,21 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   }
   tm->mark(c.st);
   // step 3: M2M
-  int maxw = 1;
-  for (int l = 0; l <= L; ++l)
-    maxw = std::max({maxw, plan.m2m.at(l).n, plan.l2l_kids.at(l).n});
-  int* map = S.get<int>((size_t)maxw * 4);  // M2M and L2L slot maps
+  // the M2M and L2L slot maps of every level, one launch each
+  int* map_m2m = S.get<int>((size_t)std::max(plan.m2m.n, 1) * 4);
+  int* map_l2l = S.get<int>((size_t)std::max(plan.l2l_kids.n, 1) * 4);
   if (S.err) return S.err;
+  if (plan.m2m.n)
+    k_map_m2m<<<nblk(plan.m2m.n, 128), 128, 0, c.st>>>(W + plan.m2m.off, plan.m2m.n, D.child,
+                                                        D.srng, map_m2m);
+  if (plan.l2l_kids.n)
+    k_map_l2l<<<nblk(plan.l2l_kids.n, 128), 128, 0, c.st>>>(W + plan.l2l_kids.off,
+                                                             plan.l2l_kids.n, D.parent, D.child,
+                                                             map_l2l);
   for (int l = L - 1; l >= 2; --l) {
     const Span sp = plan.m2m.at(l);
     if (!sp.n || !gate[l]) continue;
-    k_map_m2m<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.child, D.srng, map);
+    const int* map = map_m2m + (size_t)(sp.off - plan.m2m.off) * 4;
     HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
                              2 * lev[l].p + 1, mp, per, mp, per, nd, 0.0));
   }
@@ -1456,7 +1462,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   for (int l = 2; l <= L - 1; ++l) {
     const Span sp = plan.l2l_kids.at(l);
     if (!sp.n || !gate[l]) continue;
-    k_map_l2l<<<nblk(sp.n, 128), 128, 0, c.st>>>(W + sp.off, sp.n, D.parent, D.child, map);
+    const int* map = map_l2l + (size_t)(sp.off - plan.l2l_kids.off) * 4;
     HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 1, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
                              2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
   }
