@@ -63,6 +63,26 @@ def cpu_baseline(n_sample, zk, eps, threads):
     return 2 * n_sample / dt / 1e6, dt
 
 
+def eps12_vs_cpu(prob, zk, threads, cpu_sample, steps=3):
+    """hfmm2d on the same points at eps = 1e-12 (ndiv 25, the reference's
+    leaf-size rule for that eps): GPU throughput against the oracle's OpenMP
+    path on the host cores, on the same sample as cpu_baseline."""
+    import torch
+    ti = np.zeros(8)
+    eps, ndiv = 1e-12, 25
+    prob.solve("h", eps, zk, ndiv, ti)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prob.solve("h", eps, zk, ndiv, ti)
+    torch.cuda.synchronize()
+    gpu = (prob.n + prob.nt) * steps / (time.perf_counter() - t0) / 1e6
+    cpu, dt = cpu_baseline(cpu_sample, zk, eps, threads)
+    return {"eps": eps, "gpu_mpts_s": round(gpu, 3), "cpu_mpts_s": round(cpu, 4),
+            "cores": threads, "ratio": round(gpu / cpu, 1),
+            "cpu_sample": f"oracle/ OpenMP restatement at ns=nt={cpu_sample} ({dt:.1f} s)"}
+
+
 class DeviceProblem:
     """Inputs resident in HBM in the reference layouts (points (2, n) as an
     (n, 2) row-major array, strengths (nd, n), dipole vectors (nd, 2, n)) and
@@ -411,6 +431,7 @@ def main():
                                "kind": "port",
                                "sample": f"oracle/ OpenMP restatement, same workload at "
                                          f"ns=nt={args.cpu_sample} ({dt:.1f} s)"}
+        out["eps_1e-12_vs_cpu"] = eps12_vs_cpu(prob, zk, threads, args.cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -56,6 +56,15 @@ FMM_HD cd inv(cd z) {
   return cd{r / d, -1.0 / d};
 }
 FMM_HD cd cdiv(cd a, cd b) { return a * inv(b); }
+// z^n, n >= 0, by repeated squaring
+FMM_HD cd cpow_n(cd z, int n) {
+  cd r = cd{1.0, 0.0};
+  for (; n > 0; n >>= 1) {
+    if (n & 1) r = r * z;
+    z = z * z;
+  }
+  return r;
+}
 
 // Natural log of a positive finite double.  x = 2^k m with m in
 // [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172,

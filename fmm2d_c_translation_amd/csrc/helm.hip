@@ -264,8 +264,7 @@ __device__ void eval_lo_at(const EA& E, cd zk, double rs, double r, cd e1, EvalA
   double r2 = rs * rs;
   cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1), ur = u * rs;
   // phases at n = top_n downward
-  cd pos = mk(1, 0);
-  for (int k = 0; k < top_n; ++k) pos = pos * e1;
+  cd pos = cpow_n(e1, top_n);
   cd ie1 = conj(e1);
   EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   cd gp = mk(0, 0), g = mk(1, 0), H = mk(0, 0);
@@ -545,8 +544,7 @@ __device__ void eval_lo_at_real(const EA& E, double k, double rs, double r, cd e
   }
   const int top = miller_top_real(top_n, x);
   const double zr = rs / x, r2 = rs * rs;
-  cd pos = mk(1, 0);
-  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  cd pos = cpow_n(e1, top_n);
   const cd ie1 = conj(e1);
   EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double gp = 0.0, g = 1.0, H = 0.0;  // H = sum over even n >= 2 of rs^n g_n (Horner)
@@ -649,8 +647,7 @@ __device__ void eval_lo_sd_real(const cd* E, int top_n, double k, double rs, dou
   }
   const int top = miller_top_real(top_n, x);
   const double zr = rs / x, r2 = rs * rs;
-  cd pos = mk(1, 0);
-  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  cd pos = cpow_n(e1, top_n);
   const cd ie1 = conj(e1);
   EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double gp = 0.0, g = 1.0, H = 0.0;  // H = sum over even n >= 2 of rs^n g_n (Horner)
@@ -688,8 +685,7 @@ __device__ void eval_lo_sd(const cd* E, int top_n, cd zk, double rs, double r, c
   const cd zr = inv(z) * rs;
   const double r2 = rs * rs;
   const cd u = z.y >= 0 ? mk(0, -1) : mk(0, 1), ur = u * rs;
-  cd pos = mk(1, 0);
-  for (int q = 0; q < top_n; ++q) pos = pos * e1;
+  cd pos = cpow_n(e1, top_n);
   const cd ie1 = conj(e1);
   EvalAcc B = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   cd gp = mk(0, 0), g = mk(1, 0), H = mk(0, 0);
@@ -1450,11 +1446,8 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
-    if (dT32)
-      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, dT + tab[l].m2l[0],
-                               dT32 + tab[l].m2l[0], 2 * lev[l].p + 1, mp, per, lo, per, nd, 0.0));
-    else
-      m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0});
+    m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0,
+                          dT32 ? dT32 + tab[l].m2l[0] : nullptr});
   }
   if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);

@@ -684,11 +684,8 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
     const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
-    if (a.mixed)
-      HIP_TRY(launch_m2l_mixed(c.st, S, W + sp.off, sp.n, ml, TM, tab.m2l32, P, mp, per, lo, per,
-                               nd, std::log(plan.bs[l])));
-    else
-      m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, std::log(plan.bs[l])});
+    m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, std::log(plan.bs[l]),
+                          a.mixed ? tab.m2l32 : nullptr});
   }
   if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);

@@ -342,9 +342,9 @@ k_xlate_mfma(const int* __restrict__ work, int nwork, const int* __restrict__ ma
 }
 
 // Several M2L levels in one launch (they read and write disjoint expansions):
-// level j owns the blocks [block0, next level's block0), a wave per 16
-// targets with every row tile (MW = the levels' common row-tile count) and
-// every slot.
+// entry j -- a level, or one of its row-tile groups -- owns the blocks
+// [block0, next entry's block0), a wave per 16 targets with MW row tiles
+// from row tile by * MW, and every slot.
 struct XLevel {
   const int* work;
   int nwork;
@@ -353,6 +353,7 @@ struct XLevel {
   int nin, nout;
   double lap_log;
   int block0;
+  int by;
 };
 constexpr int kMaxXLevels = 30;
 struct XLevels {
@@ -366,7 +367,7 @@ k_xlate_mfma_levels(XLevels L, int nslot, const double2* __restrict__ X, size_t 
   int j = 0;
   while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
   const XLevel& v = L.lv[j];
-  xlate_block<MW>(blockIdx.x - v.block0, 0, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
+  xlate_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
                   v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr);
 }
 
@@ -400,17 +401,32 @@ __global__ void k_xlate_reduce(const int* __restrict__ work, int nwork, int nout
 // ---------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
+// One block of k_xlate_mfma32 (column-tile block bx, row-tile group by,
+// slot group g); the wave's map rows are staged in LDS and the used slots
+// taken from one ballot, as in xlate_block.
 template <int MW>
-__global__ void __launch_bounds__(256)
-k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
-               const float2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
-               size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
-               int G, double2* __restrict__ part) {
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void xlate32_block(
+    int bx, int by, int g, const int* __restrict__ work, int nwork, const int* __restrict__ map,
+    int nslot, const float2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+    size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log, int G,
+    double2* __restrict__ part) {
+  __shared__ int smap[4][16 * 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
-  const int w = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + n;
+  const int tile = bx * 4 + wv;
+  const int w = tile * 16 + n;
   const bool wok = w < nwork;
-  const int rt0 = blockIdx.y * MW, g = blockIdx.z;
+  const int rt0 = by * MW;
+  int* sm = smap[wv];
+  {
+    const size_t base = (size_t)tile * 16 * nslot, end = (size_t)nwork * nslot;
+    for (int e = lane; e < 16 * nslot; e += 64) sm[e] = base + e < end ? map[base + e] : -1;
+  }
+  __syncthreads();
+  bool used = false;
+  if (lane < nslot && lane % G == g)
+    for (int c = 0; c < 16; ++c) used |= sm[c * nslot + lane] >= 0;
+  const uint64_t act = __ballot(used);
   for (int d = 0; d < nd; ++d) {
     double yr[MW][4], yi[MW][4];
 #pragma unroll
@@ -418,9 +434,9 @@ k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ 
 #pragma unroll
       for (int v = 0; v < 4; ++v) yr[t][v] = yi[t][v] = 0.0;
     double s0r = 0.0, s0i = 0.0;  // sum of x_0 over the sources (lanes kq == 0)
-    for (int s = g; s < nslot; s += G) {
-      const int src = wok ? map[(size_t)w * nslot + s] : -1;
-      if (!__any(src >= 0)) continue;
+    for (uint64_t rem = act; rem; rem &= rem - 1) {
+      const int s = __builtin_ctzll(rem);
+      const int src = sm[n * nslot + s];
       const bool sok = src >= 0;
       const double2* xs = X + (size_t)(sok ? src : 0) * xstride + (size_t)d * nin;
       const float2* Ts = T + (size_t)s * nin * nout;
@@ -504,8 +520,44 @@ k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ 
   }
 }
 
-// Translation through k_xlate_mfma when the output order fits (nout <= 64),
-// else k_translate.  Rows and slots are split over waves to keep about 4096
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_xlate_mfma32(const int* __restrict__ work, int nwork, const int* __restrict__ map, int nslot,
+               const float2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
+               size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log,
+               int G, double2* __restrict__ part) {
+  xlate32_block<MW>(blockIdx.x, blockIdx.y, blockIdx.z, work, nwork, map, nslot, T, nin, nout, X,
+                    xstride, Y, ystride, nd, lap_log, G, part);
+}
+
+// Several M2L levels in one FP32 launch (as k_xlate_mfma_levels).
+struct XLevel32 {
+  const int* work;
+  int nwork;
+  const int* map;
+  const float2* T;
+  int nin, nout;
+  double lap_log;
+  int block0;
+  int by;
+};
+struct XLevels32 {
+  XLevel32 lv[kMaxXLevels];
+  int n;
+};
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_xlate_mfma32_levels(XLevels32 L, int nslot, const double2* __restrict__ X, size_t xstride,
+                      double2* __restrict__ Y, size_t ystride, int nd) {
+  int j = 0;
+  while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
+  const XLevel32& v = L.lv[j];
+  xlate32_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
+                    v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr);
+}
+
+// Translation through k_xlate_mfma when the output order fits eight row
+// tiles (nout <= 128), else k_translate.  Rows and slots are split over waves to keep about 4096
 // waves in flight whatever the level's size; maxg bounds the slot groups
 // (the number of slots a work item can use: 27 for M2L, 4 for M2M, 1 for
 // L2L).  S provides the partial-sum buffer.
@@ -515,11 +567,11 @@ inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int 
                                int nd, double lap_log) {
   if (nwork <= 0) return hipSuccess;
   const int MT = (nout + 15) / 16;
-  if (MT > 4 || nslot > 64)
+  if (MT > 8 || nslot > 64)
     return launch_translate(st, work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, nd, lap_log);
   const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
   constexpr int kWaves = 4096;
-  int mw = MT, G = 1;
+  int mw = MT <= 4 ? MT : (MT + 1) / 2, G = 1;
   if (ntile * MT < kWaves) {  // small level: one row tile per wave and split the slots
     mw = 1;
     const int per_g = ntile * MT;
@@ -551,95 +603,69 @@ inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int 
   return hipGetLastError();
 }
 
-inline hipError_t launch_m2l(hipStream_t st, Scratch& S, const int* work, int nwork,
-                             const int* map, const double2* T, int P, const double2* X, size_t xs,
-                             double2* Y, size_t ys, int nd, double lap_log) {
-  return launch_xlate(st, S, work, nwork, map, 49, 27, T, P, P, X, xs, Y, ys, nd, lap_log);
-}
 
-// FP64 M2L of several levels.  Levels whose order fits the MFMA tiles
-// (P <= 64) are grouped by row-tile count, and each group runs as one
+// M2L of several levels.  Levels whose order fits eight MFMA row tiles
+// (P <= 128; beyond four, two row groups) are grouped by row tiles per wave,
+// FP64 and FP32 (mixed precision) apart, and each group runs as one
 // k_xlate_mfma_levels launch, so the small levels share the GPU with the
 // large ones instead of running alone; larger orders go through
 // launch_translate level by level.
 struct M2LLevel {
   const int* work;
   int nwork;
-  const int* map;     // 49 slots per work item
-  const double2* T;   // the level's 49 operators, P x P each
+  const int* map;       // 49 slots per work item
+  const double2* T;     // the level's 49 operators, P x P each
   int P;
   double lap_log;
+  const float2* T32;    // the same in FP32 (mixed precision), or null
 };
 inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
                                     const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
-  for (int MT = 1; MT <= 4; ++MT) {
+  // row tiles per wave: all of them up to 4, else two groups of ceil(MT/2)
+  auto tiles = [](int MT) { return MT <= 4 ? MT : (MT + 1) / 2; };
+  for (int mw = 1; mw <= 4; ++mw) {
     XLevels L;
-    L.n = 0;
-    int nb = 0;
+    XLevels32 L32;
+    L.n = L32.n = 0;
+    int nb = 0, nb32 = 0;
+    auto flush = [&]() {
+      if (L.n) switch (mw) {
+        case 1: k_xlate_mfma_levels<1><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+        case 2: k_xlate_mfma_levels<2><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+        case 3: k_xlate_mfma_levels<3><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+        default: k_xlate_mfma_levels<4><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
+      }
+      if (L32.n) switch (mw) {
+        case 1: k_xlate_mfma32_levels<1><<<nb32, 256, 0, st>>>(L32, 49, X, xs, Y, ys, nd); break;
+        case 2: k_xlate_mfma32_levels<2><<<nb32, 256, 0, st>>>(L32, 49, X, xs, Y, ys, nd); break;
+        case 3: k_xlate_mfma32_levels<3><<<nb32, 256, 0, st>>>(L32, 49, X, xs, Y, ys, nd); break;
+        default: k_xlate_mfma32_levels<4><<<nb32, 256, 0, st>>>(L32, 49, X, xs, Y, ys, nd); break;
+      }
+      L.n = L32.n = 0;
+      nb = nb32 = 0;
+    };
     for (const M2LLevel& v : lv) {
-      if (v.nwork <= 0 || (v.P + 15) / 16 != MT) continue;
-      if (L.n == kMaxXLevels) return hipErrorInvalidValue;
-      L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, nb};
-      nb += ((v.nwork + 15) / 16 + 3) / 4;
+      const int MT = (v.P + 15) / 16;
+      if (v.nwork <= 0 || MT > 8 || tiles(MT) != mw) continue;
+      const int nbl = ((v.nwork + 15) / 16 + 3) / 4;
+      for (int by = 0; by * mw < MT; ++by) {
+        if (L.n == kMaxXLevels || L32.n == kMaxXLevels) flush();
+        if (v.T32) {
+          L32.lv[L32.n++] = XLevel32{v.work, v.nwork, v.map, v.T32, v.P, v.P, v.lap_log, nb32, by};
+          nb32 += nbl;
+        } else {
+          L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, nb, by};
+          nb += nbl;
+        }
+      }
     }
-    if (!L.n) continue;
-    switch (MT) {
-      case 1: k_xlate_mfma_levels<1><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
-      case 2: k_xlate_mfma_levels<2><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
-      case 3: k_xlate_mfma_levels<3><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
-      default: k_xlate_mfma_levels<4><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
-    }
+    flush();
   }
   for (const M2LLevel& v : lv) {
-    if (v.nwork <= 0 || (v.P + 15) / 16 <= 4) continue;
+    if (v.nwork <= 0 || (v.P + 15) / 16 <= 8) continue;
     const hipError_t e = launch_translate(st, v.work, v.nwork, v.map, 49, v.T, v.P, v.P, X, xs, Y,
                                           ys, nd, v.lap_log);
     if (e != hipSuccess) return e;
-  }
-  return hipGetLastError();
-}
-
-// Mixed-precision M2L: k_xlate_mfma32 with the FP32 operators T32 when the
-// order fits its tiles (P <= 64), else the FP64 path with T.  Work is split
-// over waves as in launch_xlate.
-inline hipError_t launch_m2l_mixed(hipStream_t st, Scratch& S, const int* work, int nwork,
-                                   const int* map, const double2* T, const float2* T32, int P,
-                                   const double2* X, size_t xs, double2* Y, size_t ys, int nd,
-                                   double lap_log) {
-  if (nwork <= 0) return hipSuccess;
-  const int MT = (P + 15) / 16;
-  if (MT > 4 || !T32) return launch_m2l(st, S, work, nwork, map, T, P, X, xs, Y, ys, nd, lap_log);
-  const int nslot = 49, maxg = 27;
-  const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
-  constexpr int kWaves = 4096;
-  int mw = MT, G = 1;
-  if (ntile * MT < kWaves) {
-    mw = 1;
-    const int per_g = ntile * MT;
-    G = std::min(std::min(maxg, nslot), std::max(1, (kWaves + per_g - 1) / per_g));
-  } else if (ntile < kWaves) {
-    mw = MT >= 2 ? 2 : 1;
-  }
-  double2* part = nullptr;
-  if (G > 1) {
-    part = S.get<double2>((size_t)G * nd * nwork * P);
-    if (!part) return hipErrorOutOfMemory;
-  }
-  const dim3 grid(nb, (MT + mw - 1) / mw, G);
-#define FMM_XL32(MWV)                                                                           \
-  k_xlate_mfma32<MWV><<<grid, 256, 0, st>>>(work, nwork, map, nslot, T32, P, P, X, xs, Y, ys, nd, \
-                                            lap_log, G, part)
-  switch (mw) {
-    case 1: FMM_XL32(1); break;
-    case 2: FMM_XL32(2); break;
-    case 3: FMM_XL32(3); break;
-    default: FMM_XL32(4); break;
-  }
-#undef FMM_XL32
-  if (G > 1) {
-    const size_t ne = (size_t)nd * nwork * P;
-    k_xlate_reduce<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(work, nwork, P, nd, G, part, Y,
-                                                                ys);
   }
   return hipGetLastError();
 }

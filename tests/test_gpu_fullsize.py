@@ -12,7 +12,7 @@ itself where it travelled with the tree.
   C1: cfmm2d_ndiv, 1e5 uniform points, complex charges, pot + grad at the
       sources, ndiv 40, eps 1e-6 (FP64).
   C2: hfmm2d, 1e6 sources + 1e6 targets, zk = 40, charges + dipoles,
-      eps 1e-9 (FP64) and 1e-6 (mixed).
+      eps 1e-9 and 1e-12 (FP64) and 1e-6 (mixed).
   C3: cfmm2d, 4e6 points (90% on a thin circle), real charges, eps ladder
       1e-3 .. 1e-12, FP64 and (eps >= 1e-6) mixed; Re(pot) is gauge-free for
       real charges."""
@@ -69,7 +69,7 @@ def test_c1_matches_reference(ref):
         assert rel_l2(a["grad"], b["grad"]) < 1e-11
 
 
-@pytest.mark.parametrize("eps,mode", [(1e-9, F.FP64), (1e-6, F.MIXED)])
+@pytest.mark.parametrize("eps,mode", [(1e-9, F.FP64), (1e-6, F.MIXED), (1e-12, F.FP64)])
 def test_c2_within_eps_of_direct_sum(eps, mode):
     print(f"C2 hfmm2d 1e6+1e6 eps={eps} mode={mode}", flush=True)
     rng = np.random.default_rng(2)
