@@ -9,6 +9,7 @@
 
 #include "../../include/fmm2d_hip.h"
 #include "engine.h"
+#include "rsort.h"
 
 namespace fmm {
 namespace {
@@ -175,7 +176,7 @@ int run_lap(const LapReq& r) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.st));
   finish_timer(r.timeinfo);
-  return IER_OK;
+  return sort_status(plan);
 }
 
 int guarded(const LapReq& r) {
@@ -281,7 +282,7 @@ int run_helm(const HelmReq& r) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.st));
   finish_timer(r.timeinfo);
-  return IER_OK;
+  return sort_status(plan);
 }
 
 int guarded_h(const HelmReq& r) {
@@ -634,6 +635,7 @@ int fmm2dhip_test_devtree(const double* src, int ns, const double* targ, int nt,
         HIP_TRY(hipMemcpyAsync(o.data() + (size_t)k * (nb + 1), od[k], sizeof(int) * (nb + 1),
                                hipMemcpyDeviceToHost, c.st));
       HIP_TRY(hipStreamSynchronize(c.st));
+      if ((e = fmm::sort_status(plan))) return e;
       int tot = 0;
       for (int k = 0; k < 4; ++k) tot += o[(size_t)k * (nb + 1) + nb];
       *nlevels = plan.L;
@@ -684,6 +686,52 @@ int fmm2dhip_test_devtree(const double* src, int ns, const double* targ, int nt,
     }
     HIP_TRY(hipStreamSynchronize(c.st));
     return out;
+  } catch (...) {
+    return fmm::IER_ALLOC;
+  }
+}
+
+int fmm2dhip_test_sort(const void* keys, int n, int key_bytes, int begin, int end,
+                       void* keys_out, int* vals_out) {
+  try {
+    if (n < 0 || (key_bytes != 4 && key_bytes != 8)) return fmm::IER_ARGS;
+    if (n == 0) return fmm::IER_OK;
+    fmm::Ctx c;
+    int e = fmm::get_ctx(c);
+    if (e) return e;
+    uint32_t* flag = static_cast<uint32_t*>(fmm::pinned(sizeof(uint32_t), 6));
+    if (!flag) return fmm::IER_ALLOC;
+    *flag = 0;
+    {
+      fmm::Scratch S(c.st);
+      int* vi = vals_out ? S.get<int>(n) : nullptr;
+      int* vo = vals_out ? S.get<int>(n) : nullptr;
+      if (S.err) return S.err;
+      if (vi) {
+        std::vector<int> iota(n);
+        std::iota(iota.begin(), iota.end(), 0);
+        HIP_TRY(hipMemcpyAsync(vi, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, c.st));
+        HIP_TRY(hipStreamSynchronize(c.st));
+      }
+      if (key_bytes == 8) {
+        const uint64_t* kin = nullptr;
+        if ((e = fmm::upload(S, c.st, static_cast<const uint64_t*>(keys), n, &kin))) return e;
+        uint64_t* kout = S.get<uint64_t>(n);
+        if (S.err) return S.err;
+        if ((e = fmm::rsort::radix_sort(c, S, kin, kout, vi, vo, n, begin, end, flag))) return e;
+        HIP_TRY(hipMemcpyAsync(keys_out, kout, 8 * (size_t)n, hipMemcpyDeviceToHost, c.st));
+      } else {
+        const uint32_t* kin = nullptr;
+        if ((e = fmm::upload(S, c.st, static_cast<const uint32_t*>(keys), n, &kin))) return e;
+        uint32_t* kout = S.get<uint32_t>(n);
+        if (S.err) return S.err;
+        if ((e = fmm::rsort::radix_sort(c, S, kin, kout, vi, vo, n, begin, end, flag))) return e;
+        HIP_TRY(hipMemcpyAsync(keys_out, kout, 4 * (size_t)n, hipMemcpyDeviceToHost, c.st));
+      }
+      if (vo) HIP_TRY(hipMemcpyAsync(vals_out, vo, sizeof(int) * n, hipMemcpyDeviceToHost, c.st));
+      HIP_TRY(hipStreamSynchronize(c.st));
+    }
+    return *flag ? fmm::IER_HIP : fmm::IER_OK;
   } catch (...) {
     return fmm::IER_ALLOC;
   }

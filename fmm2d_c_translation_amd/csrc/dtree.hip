@@ -24,7 +24,6 @@
 // integer cell coordinates; the reference's 1.05-scaled centre-distance tests
 // separate exactly the same cell offsets.
 #include <hipcub/hipcub.hpp>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -32,6 +31,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "rsort.h"
 #include "translate.h"
 
 namespace fmm {
@@ -225,10 +225,6 @@ struct Levels {
   double c0x, c0y;
 };
 
-// onesweep radix sort at every size (rocprim's default merges below 2^20 items)
-using OnesweepSort =
-    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                               rocprim::default_config, 0>;
 
 __device__ inline int level_of(const Levels& lv, int b) {
   int l = 0;
@@ -667,13 +663,12 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     int* ct = use_t ? S.get<int>((size_t)nt + 1) : nullptr;
     int* ot = use_t ? S.get<int>((size_t)nt + 1) : nullptr;
     if (S.err) return S.err;
-    size_t tbs = 0, tbt = 0, tbr = 0, tbu = 0;
+    size_t tbs = 0, tbt = 0, tbu = 0;
     if (use_s) HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbs, cs, os, ns + 1, c.st));
     if (use_t) HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbt, ct, ot, nt + 1, c.st));
-    HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(nullptr, tbr, Re, Rs, (int)cap, 0, kbits, c.st));
     if (idivflag == 2)
       HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tbu, Re, Rs, d_nsplit, (int)cap, c.st));
-    void* tmp = S.get<char>(std::max({tbs, tbt, tbr, tbu, size_t(1)}));
+    void* tmp = S.get<char>(std::max({tbs, tbt, tbu, size_t(1)}));
     if (S.err) return S.err;
     int* d_used = d_nsplit + 1;  // emitted keys
     if (Ls >= 0) {
@@ -695,10 +690,12 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     if (idivflag == 2) {
       uint64_t* Rt = S.get<uint64_t>(cap);
       if (S.err) return S.err;
-      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tbr, Re, Rt, (int)cap, 0, kbits, c.st));
+      e = rsort::radix_sort(c, S, Re, Rt, nullptr, nullptr, (int)cap, 0, kbits, plan.sort_err);
+      if (e) return e;
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tbu, Rt, Rs, d_nsplit, (int)cap, c.st));
     } else {
-      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tbr, Re, Rs, (int)cap, 0, kbits, c.st));
+      e = rsort::radix_sort(c, S, Re, Rs, nullptr, nullptr, (int)cap, 0, kbits, plan.sort_err);
+      if (e) return e;
     }
     k_split_levels<<<1, 64, 0, c.st>>>(Rs, d_nsplit, L, d_roff);
   }
@@ -738,13 +735,13 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     // cand[0 .. roff[L]): the split keys (level-tagged), then the forced boxes
     if (roff[L] > 0)
       k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(Rs, d_roff, d_cbase, L, cand);
-    size_t tb1 = 0, tb2 = 0;
-    HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(nullptr, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
+    size_t tb2 = 0;
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
-    void* tmp = S.get<char>(std::max(tb1, tb2));
+    void* tmp = S.get<char>(std::max(tb2, size_t(1)));
     if (S.err) return S.err;
     if (nc > 0) {
-      HIP_TRY(rocprim::radix_sort_keys<OnesweepSort>(tmp, tb1, cand, sorted, (int)nc, 0, kbits, c.st));
+      e = rsort::radix_sort(c, S, cand, sorted, nullptr, nullptr, (int)nc, 0, kbits, plan.sort_err);
+      if (e) return e;
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     } else {
       HIP_TRY(hipMemsetAsync(d_nu, 0, sizeof(int), c.st));
@@ -870,13 +867,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, ctr, lo.o2, lo.l2, srng, work + plan.m2l.off,
                                                n, k0, v0);
     const int kb = kM2lKeyBits;
-    size_t tbs = 0;
-    HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tbs, k0, k1, v0,
-                                                    work + plan.m2l.off, n, 0, kb, c.st));
-    void* tmps = S.get<char>(tbs);
-    if (S.err) return S.err;
-    HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmps, tbs, k0, k1, v0, work + plan.m2l.off, n,
-                                                    0, kb, c.st));
+    e = rsort::radix_sort(c, S, k0, k1, v0, work + plan.m2l.off, n, 0, kb, plan.sort_err);
+    if (e) return e;
   }
   // list-3 and leaf work in decreasing cost within each level (the four
   // lists are adjacent in the work buffer)
@@ -892,15 +884,11 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
       uint32_t* k0 = S.get<uint32_t>(n);
       uint32_t* k1 = S.get<uint32_t>(n);
       int* v0 = S.get<int>(n);
-      size_t tbs = 0;
-      HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tbs, k0, k1, v0, v0, n, 0, 15,
-                                                      c.st));
-      void* tmps = S.get<char>(tbs);
       if (S.err) return S.err;
       k_cost_keys<<<nblk(n, 256), 256, 0, c.st>>>(work + first, n, cl, level, srng, trng, lo.o3,
                                                   k0, v0);
-      HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmps, tbs, k0, k1, v0, work + first, n, 0,
-                                                      15, c.st));
+      e = rsort::radix_sort(c, S, k0, k1, v0, work + first, n, 0, 15, plan.sort_err);
+      if (e) return e;
     }
   }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;

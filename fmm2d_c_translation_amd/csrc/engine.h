@@ -111,7 +111,13 @@ struct Plan {
   WorkList l2l_kids;            // the children of those (lev[] by parent level)
   WorkList list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
   std::vector<double> m2l_pairs;  // list-2 pairs of the m2l boxes, per level
+  // pinned host word, cleared before the front end's sorts and set by a sort
+  // whose look-back wait gave up; read after the stream synchronises
+  uint32_t* sort_err = nullptr;
 };
+// IER_HIP when a front-end sort of this plan reported a failed wait (call
+// after the stream has synchronised).
+inline int sort_status(const Plan& p) { return p.sort_err && *p.sort_err ? IER_HIP : IER_OK; }
 
 // Device tree and lists from keys sorted on their top 2 sorted_levels bits
 // (dtree.hip).  Returns IER_RESORT, before building anything, when the tree

@@ -2,7 +2,6 @@
 // in-leaf ordering and gather/scatter of point data; the tree itself is
 // built by dtree.hip.
 #include <hipcub/hipcub.hpp>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -10,6 +9,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "rsort.h"
 
 namespace fmm {
 
@@ -330,18 +330,13 @@ __global__ void k_scatter_ch(const double2* in, double2* out, const int* perm, i
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-// rocprim's default radix sort switches to a merge sort below 2^20 items,
-// about three times slower here than its onesweep radix sort
-using OnesweepSort =
-    rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                               rocprim::default_config, 0>;
-
 // Keys and key-sorted permutations of both point sets (one stable LSD radix
-// sort, onesweep, on key bits [begin_bit, 2 kMaxDepth) and the target
-// flag): perm[0 .. ns) for the sources, perm[ns .. ns + nt) for the
-// targets; *ks and *kt are the sorted keys of each set (on the device).
+// sort on key bits [begin_bit, 2 kMaxDepth) and the target flag):
+// perm[0 .. ns) for the sources, perm[ns .. ns + nt) for the targets; *ks and
+// *kt are the sorted keys of each set (on the device).
 int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* trg, int nt,
-                const double* geo, int begin_bit, int* perm, uint64_t** ks, uint64_t** kt) {
+                const double* geo, int begin_bit, uint32_t* err, int* perm, uint64_t** ks,
+                uint64_t** kt) {
   const int n = ns + nt;
   uint64_t* k0 = S.get<uint64_t>(n);
   uint64_t* k1 = S.get<uint64_t>(n);
@@ -351,13 +346,8 @@ int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* t
   *kt = k1 + ns;
   if (n == 0) return IER_OK;
   k_keys<<<nblk(n, 256), 256, 0, c.st>>>(src, ns, trg, nt, geo, k0, i0);
-  size_t tb = 0;
-  HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, tb, k0, k1, i0, perm, n, begin_bit,
-                                                  2 * kMaxDepth + 1, c.st));
-  void* tmp = S.get<char>(tb);
-  if (S.err) return S.err;
-  HIP_TRY(rocprim::radix_sort_pairs<OnesweepSort>(tmp, tb, k0, k1, i0, perm, n, begin_bit,
-                                                  2 * kMaxDepth + 1, c.st));
+  int e = rsort::radix_sort(c, S, k0, k1, i0, perm, n, begin_bit, 2 * kMaxDepth + 1, err);
+  if (e) return e;
   if (nt) k_unflag<<<nblk(nt, 256), 256, 0, c.st>>>(k1 + ns, nt);
   return IER_OK;
 }
@@ -408,6 +398,9 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   // order inside a leaf is restored below), so the sort first covers
   // kSortLevels levels and is redone on every bit when the tree turns out
   // deeper.
+  plan.sort_err = static_cast<uint32_t*>(pinned(sizeof(uint32_t), 6));
+  if (!plan.sort_err) return IER_ALLOC;
+  *plan.sort_err = 0;  // the previous solve on this thread has synchronised
   P.perm_s = S.get<int>((size_t)ns + nt);
   if (S.err) return S.err;
   P.perm_t = P.perm_s + ns;
@@ -416,7 +409,8 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   for (int levels : {kSortLevels, kMaxDepth}) {
     const int begin_bit = 2 * (kMaxDepth - levels);
     uint64_t *ks = nullptr, *kt = nullptr;
-    e = sort_points(c, S, src, ns, trg, nt, d_geo, begin_bit, P.perm_s, &ks, &kt);
+    e = sort_points(c, S, src, ns, trg, nt, d_geo, begin_bit, plan.sort_err, P.perm_s, &ks,
+                    &kt);
     if (e) return e;
     th[2] = wall();
     // ---- tree, lists and work lists on the device

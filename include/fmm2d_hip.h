@@ -214,6 +214,14 @@ int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, in
                        double *boxsize, int *isrc, int *isrcse, int *itarg, int *itargse,
                        int *nlist, int *lists);
 
+/* ---- test hook (no counterpart in the reference): the front end's device
+ * radix sort.  Sorts n keys of key_bytes (4 or 8) bytes stably by key bits
+ * [begin, end): keys_out receives the sorted keys and, when vals_out is not
+ * NULL, vals_out[i] the input position of the i-th sorted key.  Returns 0,
+ * 102 for bad arguments, or 101 when the sort reports a failed wait. */
+int fmm2dhip_test_sort(const void *keys, int n, int key_bytes, int begin, int end,
+                       void *keys_out, int *vals_out);
+
 /* ---- test hook (no counterpart in the reference): the solvers' device
  * tree for the given points (copied to the device first; pg = pgt = 1 work
  * lists).  Boxes are numbered level-major, Morton order inside a level
