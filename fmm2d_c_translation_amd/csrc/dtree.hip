@@ -87,11 +87,11 @@ __device__ inline int2 key_range(const uint64_t* k, int n, uint64_t key, int lev
 // level-c box, c = the number of leading key digits the two ends agree on,
 // so that box holds more than ndiv points and is split; no deeper box is.
 __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
   int c = -1;
-  if (i + ndiv < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i + ndiv < n; i += gridDim.x * blockDim.x) {
     const uint64_t x = k[i] ^ k[i + ndiv];
-    c = x == 0 ? kMaxDepth : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1);
+    c = max(c, x == 0 ? kMaxDepth
+                      : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1));
   }
   // one atomic per block, and none when the block cannot raise the value
   __shared__ int wmax[4];
@@ -239,12 +239,9 @@ __global__ void k_box_keys(Levels lv, const uint64_t* R, int nb, uint64_t* key) 
   key[b] = l == 0 ? 0 : 4 * R[lv.roff[l - 1] + (i >> 2)] + (uint64_t)(i & 3);
 }
 
-// per box: level, centre, point ranges, children (and their parent link),
-// colleagues in fixed slots s = (dy+1)*3 + (dx+1)
-__global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int nb,
-                        const uint64_t* ks, int ns, const uint64_t* kt, int nt, int* level,
-                        double2* ctr, int2* srng, int2* trng, int* nchild, int4* child,
-                        int* parent, int* nbr, double* bs_out) {
+// per box: level, centre, children (and their parent link)
+__global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int nb, int* level,
+                        double2* ctr, int* nchild, int4* child, int* parent, double* bs_out) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0)
     for (int l = 0; l <= lv.L; ++l) bs_out[l] = lv.bs[l];
@@ -259,8 +256,6 @@ __global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int n
     cy = child_coord(cy, q & 2, lv.bs[j]);
   }
   ctr[b] = make_double2(cx, cy);
-  srng[b] = key_range(ks, ns, k, l);
-  trng[b] = key_range(kt, nt, k, l);
   const int nr = lv.roff[l + 1] - lv.roff[l];
   const uint64_t* Rl = R + lv.roff[l];
   int rank = lbound(Rl, nr, k);
@@ -274,18 +269,39 @@ __global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int n
     child[b] = make_int4(-1, -1, -1, -1);
   }
   if (b == 0) parent[0] = -1;
+}
+
+// The binary searches of every box, one per thread (blockIdx.y = search):
+// 0-3 the ends of its source and target ranges, 4-12 the colleague in slot
+// s = (dy+1)*3 + (dx+1) (-1 none).  Independent searches in separate
+// threads overlap their load latencies.
+constexpr int kBoxSearches = 13;
+__global__ void k_box_search(Levels lv, const uint64_t* key, int nb, const uint64_t* ks, int ns,
+                             const uint64_t* kt, int nt, int* srng, int* trng, int* nbr) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int job = blockIdx.y;
+  const int l = level_of(lv, b);
+  const uint64_t k = key[b];
+  if (job < 4) {
+    const uint64_t* a = job < 2 ? ks : kt;
+    const int n = job < 2 ? ns : nt;
+    const int r = lbound(a, n, box_key_lo(k + (job & 1), l));
+    (job < 2 ? srng : trng)[2 * b + (job & 1)] = r;
+    return;
+  }
+  const int s = job - 4;
   int ix, iy;
   key_cell(k, ix, iy);
   const int side = 1 << l, first = lv.loff[l], n = lv.loff[l + 1] - first;
-  for (int s = 0; s < 9; ++s) {
-    int jx = ix + s % 3 - 1, jy = iy + s / 3 - 1, id = -1;
-    if (jx >= 0 && jy >= 0 && jx < side && jy < side) {
-      uint64_t kk = cell_key(jx, jy);
-      int p = lbound(key + first, n, kk);
-      if (p < n && key[first + p] == kk) id = first + p;
-    }
-    nbr[9 * b + s] = id;
+  const int jx = ix + s % 3 - 1, jy = iy + s / 3 - 1;
+  int id = -1;
+  if (jx >= 0 && jy >= 0 && jx < side && jy < side) {
+    const uint64_t kk = cell_key(jx, jy);
+    const int p = lbound(key + first, n, kk);
+    if (p < n && key[first + p] == kk) id = first + p;
   }
+  nbr[9 * b + s] = id;
 }
 
 struct Int4Sum {
@@ -452,49 +468,50 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
     ps += __shfl_xor(ps, w);
     pt += __shfl_xor(pt, w);
   }
-  if (lane == 0 && (ps != 0 || pt != 0)) {
-    atomicAdd(&pairs[0], ps);
-    atomicAdd(&pairs[1], pt);
+  __shared__ double sp[2][WLB / 64];
+  if (lane == 0) {
+    sp[0][threadIdx.x >> 6] = ps;
+    sp[1][threadIdx.x >> 6] = pt;
   }
   __syncthreads();
+  if (threadIdx.x < 2) {  // one atomic per block and sum
+    double t = 0;
+    for (int w = 0; w < WLB / 64; ++w) t += sp[threadIdx.x][w];
+    if (t != 0) atomicAdd(&pairs[threadIdx.x], t);
+  }
   if (threadIdx.x < WL_N) bcnt[threadIdx.x * nblocks + blockIdx.x] = sc[threadIdx.x];
   if (threadIdx.x <= kMaxDepth && sm2l[threadIdx.x])
     atomicAdd(&m2l_lev[threadIdx.x], sm2l[threadIdx.x]);
 }
 
-// exclusive scan of the per-block counts of every list (one block);
-// info[j] = list base in the output, info[WL_N + 1 + j] = list length
-__global__ void __launch_bounds__(1024) k_wl_scan(int* bcnt, int nblocks, int* info) {
-  __shared__ int part[1024];
-  __shared__ int base;
-  if (threadIdx.x == 0) base = 0;
-  const int per = (nblocks + 1023) / 1024;
-  for (int j = 0; j < WL_N; ++j) {
-    int* c = bcnt + (size_t)j * nblocks;
-    int lo = threadIdx.x * per, hi = min(nblocks, lo + per), s = 0;
-    for (int i = lo; i < hi; ++i) s += c[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      int v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
+// exclusive scan of the per-block counts of every list (one block, wave j
+// scans list j); info[j] = list base in the output, info[WL_N + 1 + j] =
+// list length
+__global__ void __launch_bounds__(64 * WL_N) k_wl_scan(int* bcnt, int nblocks, int* info) {
+  __shared__ int tot[WL_N];
+  const int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int* c = bcnt + (size_t)j * nblocks;
+  int run = 0;
+  for (int i0 = 0; i0 < nblocks; i0 += 64) {
+    const int i = i0 + lane;
+    const int v = i < nblocks ? c[i] : 0;
+    int inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
     }
-    int run = part[threadIdx.x] - s;
-    for (int i = lo; i < hi; ++i) {
-      int v = c[i];
-      c[i] = run;
-      run += v;
-    }
-    if (threadIdx.x == 1023) {
-      info[j] = base;
-      info[WL_N + 1 + j] = part[1023];
-      base += part[1023];
-    }
-    __syncthreads();
+    if (i < nblocks) c[i] = run + inc - v;
+    run += __shfl(inc, 63);
   }
-  if (threadIdx.x == 0) info[WL_N] = base;  // start of the list-2 L2L children
+  if (lane == 0) tot[j] = run;
+  __syncthreads();
+  if (lane == 0) {
+    int base = 0;
+    for (int q = 0; q < j; ++q) base += tot[q];
+    info[j] = base;
+    info[WL_N + 1 + j] = run;
+    if (j == WL_N - 1) info[WL_N] = base + run;  // start of the list-2 L2L children
+  }
 }
 
 __global__ void __launch_bounds__(WLB)
@@ -626,9 +643,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   if (S.err) return S.err;
   HIP_TRY(hipMemsetAsync(d_depth, 0xff, sizeof(int), c.st));
   if (idivflag != 1 && ns > ndiv)
-    k_depth<<<nblk(ns - ndiv, 256), 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
+    k_depth<<<std::min(1024, nblk(ns - ndiv, 256)), 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
   if ((idivflag == 1 || idivflag == 2) && nt > ndiv)
-    k_depth<<<nblk(nt - ndiv, 256), 256, 0, c.st>>>(kt, nt, ndiv, d_depth);
+    k_depth<<<std::min(1024, nblk(nt - ndiv, 256)), 256, 0, c.st>>>(kt, nt, ndiv, d_depth);
   int Ls = -1;
   int e = sync_read(c, d_depth, 1, &Ls);
   if (e) return e;
@@ -784,8 +801,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   int* o = S.get<int>(4 * ((size_t)nb + 1));
   if (S.err) return S.err;
   k_box_keys<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, nb, key);
-  k_boxes<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, key, nb, ks, ns, kt, nt, level, ctr, srng,
-                                          trng, nchild, child, parent, nbr, dbs);
+  k_boxes<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, key, nb, level, ctr, nchild, child, parent,
+                                          dbs);
+  k_box_search<<<dim3(nblk(nb, 256), kBoxSearches), 256, 0, c.st>>>(
+      lv, key, nb, ks, ns, kt, nt, reinterpret_cast<int*>(srng), reinterpret_cast<int*>(trng), nbr);
   // ---- lists 1-4 (count, scan, fill)
   ListOut lo{o, o + nb + 1, o + 2 * (nb + 1), o + 3 * (nb + 1), nullptr, nullptr, nullptr, nullptr};
   k_lists<false><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
@@ -824,7 +843,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   HIP_TRY(hipMemsetAsync(small, 0, sizeof(int) * (nsmall + 2), c.st));
   WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg};
   k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, bcnt, nbk, pairs, m2l_lev);
-  k_wl_scan<<<1, 1024, 0, c.st>>>(bcnt, nbk, info);
+  k_wl_scan<<<1, 64 * WL_N, 0, c.st>>>(bcnt, nbk, info);
   // every list holds at most nb entries; the L2L children at most 4 nb
   int* work = S.get<int>((size_t)WL_N * nb + 4 * (size_t)nb + 1);
   if (S.err) return S.err;
