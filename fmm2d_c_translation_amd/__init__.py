@@ -89,6 +89,14 @@ def lfmm2d(eps, sources, charges=None, dipstr=None, dipvec=None, pg=1, targets=N
                            ndiv, idivflag, ifnear)
 
 
+def rfmm2d(eps, sources, charges=None, dipstr=None, dipvec=None, pg=1, targets=None, pgt=0,
+           nd=1, ndiv=None, idivflag=0, ifnear=1):
+    """Log-kernel Laplace FMM, real densities (``rfmm2d_`` / ``rfmm2d_ndiv_``)."""
+    sym = "rfmm2d_ndiv_" if ndiv is not None else "rfmm2d_"
+    return abi.rfmm2d_call(lib(), sym, eps, sources, charges, dipstr, dipvec, pg, targets, pgt, nd,
+                           ndiv, idivflag, ifnear)
+
+
 def hfmm2d(eps, zk, sources, charges=None, dipstr=None, dipvec=None, pg=1, targets=None, pgt=0,
            nd=1, ndiv=None, idivflag=0, ifnear=1):
     """Helmholtz FMM (``hfmm2d_`` / ``hfmm2d_ndiv_``)."""

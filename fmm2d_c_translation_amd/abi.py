@@ -144,6 +144,38 @@ def lfmm2d_call(lib, symbol, eps, sources, charges=None, dipstr=None, dipvec=Non
     return res
 
 
+def rfmm2d_call(lib, symbol, eps, sources, charges=None, dipstr=None, dipvec=None, pg=1,
+                targets=None, pgt=0, nd=1, ndiv=None, idivflag=0, ifnear=1, timeinfo=None):
+    """Call an ``rfmm2d_``-family symbol (log kernel, real densities and
+    outputs; grad (nd,2,n), hess (nd,3,n), all float64); signature of
+    rfmm2d_ndiv.c:51 with ``ndiv``, of src/laplace/rfmm2d.f:27 without."""
+    src = _farr(sources, np.float64, (2, -1))
+    ns = src.shape[1]
+    tg = _farr(targets, np.float64, (2, -1)) if targets is not None else np.zeros((2, 1), order="F")
+    nt = tg.shape[1] if targets is not None else 0
+    ifc = int(charges is not None)
+    ifd = int(dipstr is not None)
+    ch = _farr(charges, np.float64, (nd, ns)) if ifc else np.zeros((nd, 1), order="F")
+    dp = _farr(dipstr, np.float64, (nd, ns)) if ifd else np.zeros((nd, 1), order="F")
+    dv = _farr(dipvec, np.float64, (nd, 2, ns)) if ifd else np.zeros((nd, 2, 1), order="F")
+    pot, grad, hess = (a.real.copy(order="F") for a in _outs(nd, ns, pg, 2, 3))
+    pt, gt, ht = (a.real.copy(order="F") for a in _outs(nd, nt, pgt, 2, 3))
+    iper = C.c_int32(7)
+    ier = C.c_int32(-1)
+    args = [_i(nd), _d(eps), _i(ns), _p(src), _i(ifc), _p(ch), _i(ifd), _p(dp), _p(dv),
+            C.byref(iper), _i(pg), _p(pot), _p(grad), _p(hess), _i(nt), _p(tg), _i(pgt),
+            _p(pt), _p(gt), _p(ht)]
+    ti = np.zeros(8) if timeinfo is None else timeinfo
+    if ndiv is not None:
+        args += [_i(ndiv), _i(idivflag), _i(ifnear), _p(ti)]
+    args += [C.byref(ier)]
+    getattr(lib, symbol)(*args)
+    res = FmmResult(ier=ier.value, iper=iper.value, timeinfo=ti)
+    _collect(res, "", nd, ns, pg, pot, grad, hess)
+    _collect(res, "targ_", nd, nt, pgt, pt, gt, ht)
+    return res
+
+
 def hfmm2d_call(lib, symbol, eps, zk, sources, charges=None, dipstr=None, dipvec=None, pg=1,
                 targets=None, pgt=0, nd=1, ndiv=None, idivflag=0, ifnear=1, timeinfo=None):
     """Call an ``hfmm2d_``-family symbol (Helmholtz, H0(k r) i/4 kernel;

@@ -61,8 +61,49 @@ __global__ void k_lfmm_unpack(const double2* P, const double2* G, const double2*
   }
 }
 
+// rfmm inputs (rfmm2d_ndiv.c:130-150): charge1 = charge, dipstr1 =
+// dipstr * (-(v1 + i v2)), gathered channel-major into sorted order
+__global__ void k_rfmm_prep(const double* chg, const double* dip, const double* dipvec,
+                            const int* perm, int n, int nd, double2* chg1, double2* dip1) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int s = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    if (chg1) chg1[(size_t)d * n + i] = make_double2(chg[(size_t)s * nd + d], 0.0);
+    if (dip1) {
+      double q = dip[(size_t)s * nd + d];
+      double v1 = dipvec[((size_t)s * 2 + 0) * nd + d], v2 = dipvec[((size_t)s * 2 + 1) * nd + d];
+      dip1[(size_t)d * n + i] = make_double2(-(q * v1), -(q * v2));
+    }
+  }
+}
+
+// rfmm unpack (rfmm2d_ndiv.c:156-214): pot = Re; grad = (Re, -Im) of d/dz;
+// hess = (Re, -Im, -Re) of d2/dz2; real outputs scattered to original order
+__global__ void k_rfmm_unpack(const double2* P, const double2* G, const double2* H,
+                              const int* perm, int n, int nd, int pg, double* pot, double* grad,
+                              double* hess) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int t = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    size_t a = (size_t)d * n + i;
+    pot[(size_t)t * nd + d] = P[a].x;
+    if (pg >= 2) {
+      grad[((size_t)t * 2 + 0) * nd + d] = G[a].x;
+      grad[((size_t)t * 2 + 1) * nd + d] = -G[a].y;
+    }
+    if (pg >= 3) {
+      hess[((size_t)t * 3 + 0) * nd + d] = H[a].x;
+      hess[((size_t)t * 3 + 1) * nd + d] = -H[a].y;
+      hess[((size_t)t * 3 + 2) * nd + d] = -H[a].x;
+    }
+  }
+}
+
 struct LapReq {
   bool logk = false;  // lfmm (log kernel, complex densities) vs cfmm (Cauchy)
+  bool realk = false;  // rfmm (log kernel, real densities and outputs)
   int nd = 1;
   double eps = 1e-6;
   int ns = 0, nt = 0;
@@ -96,15 +137,23 @@ int run_lap(const LapReq& r) {
     return IER_ARGS;
   Scratch S(c.st);
   const int nd = r.nd, ns = r.ns, nt = r.nt;
-  const int ncomp_g = r.logk ? 2 : 1, ncomp_h = r.logk ? 3 : 1;
+  const bool split = r.logk || r.realk;  // gradient (nd,2,n), Hessian (nd,3,n)
+  const int ncomp_g = split ? 2 : 1, ncomp_h = split ? 3 : 1;
+  const size_t esz = r.realk ? sizeof(double) : sizeof(double2);  // user element
   const double *src = r.src, *trg = r.trg, *dv = r.dipvec;
   const double2 *chg = static_cast<const double2*>(r.chg), *dip = static_cast<const double2*>(r.dip);
+  const double *rchg = static_cast<const double*>(r.chg), *rdip = static_cast<const double*>(r.dip);
   if (!r.device) {
     if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &src))) return e;
     if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &trg))) return e;
-    if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
-    if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
-    if (r.logk && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
+    if (r.realk) {
+      if (r.ifc && (e = upload(S, c.st, rchg, (size_t)nd * ns, &rchg))) return e;
+      if (r.ifd && (e = upload(S, c.st, rdip, (size_t)nd * ns, &rdip))) return e;
+    } else {
+      if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
+      if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
+    }
+    if (split && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
   }
   SortedPoints P;
   Plan plan;
@@ -123,7 +172,9 @@ int run_lap(const LapReq& r) {
   double2* cs = r.ifc ? S.get<double2>((size_t)ndc * ns) : nullptr;
   double2* qs = r.ifd ? S.get<double2>((size_t)ndc * ns) : nullptr;
   if (S.err) return S.err;
-  if (r.logk) {
+  if (r.realk) {
+    k_rfmm_prep<<<nblk(ns, 256), 256, 0, c.st>>>(rchg, rdip, dv, P.perm_s, ns, nd, cs, qs);
+  } else if (r.logk) {
     k_lfmm_prep<<<nblk(ns, 256), 256, 0, c.st>>>(chg, dip, dv, P.perm_s, ns, nd, cs, qs);
   } else {
     if (cs) gather_channels(c.st, chg, cs, P.perm_s, ns, nd, 1);
@@ -151,26 +202,30 @@ int run_lap(const LapReq& r) {
   for (auto& o : outs) {
     if (o.flag <= 0 || o.n == 0) continue;
     size_t np = (size_t)nd * o.n, ng = np * ncomp_g, nh = np * ncomp_h;
-    double2 *dp = static_cast<double2*>(o.up), *dg = static_cast<double2*>(o.ug),
-            *dh = static_cast<double2*>(o.uh);
+    void *dp = o.up, *dg = o.ug, *dh = o.uh;
     if (!r.device) {
-      dp = S.get<double2>(np);
-      dg = o.flag >= 2 ? S.get<double2>(ng) : nullptr;
-      dh = o.flag >= 3 ? S.get<double2>(nh) : nullptr;
+      dp = S.get<char>(esz * np);
+      dg = o.flag >= 2 ? S.get<char>(esz * ng) : nullptr;
+      dh = o.flag >= 3 ? S.get<char>(esz * nh) : nullptr;
       if (S.err) return S.err;
     }
-    if (r.logk) {
-      k_lfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag,
-                                                      dp, dg, dh);
+    if (r.realk) {
+      k_rfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(
+          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double*>(dp),
+          static_cast<double*>(dg), static_cast<double*>(dh));
+    } else if (r.logk) {
+      k_lfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(
+          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double2*>(dp),
+          static_cast<double2*>(dg), static_cast<double2*>(dh));
     } else {
-      scatter_channels(c.st, o.sp, dp, o.perm, o.n, nd, 1);
-      if (o.flag >= 2) scatter_channels(c.st, o.sg, dg, o.perm, o.n, nd, 1);
-      if (o.flag >= 3) scatter_channels(c.st, o.sh, dh, o.perm, o.n, nd, 1);
+      scatter_channels(c.st, o.sp, static_cast<double2*>(dp), o.perm, o.n, nd, 1);
+      if (o.flag >= 2) scatter_channels(c.st, o.sg, static_cast<double2*>(dg), o.perm, o.n, nd, 1);
+      if (o.flag >= 3) scatter_channels(c.st, o.sh, static_cast<double2*>(dh), o.perm, o.n, nd, 1);
     }
     if (!r.device) {
-      HIP_TRY(hipMemcpyAsync(o.up, dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, sizeof(double2) * ng, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, sizeof(double2) * nh, hipMemcpyDeviceToHost, c.st));
+      HIP_TRY(hipMemcpyAsync(o.up, dp, esz * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, esz * ng, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, esz * nh, hipMemcpyDeviceToHost, c.st));
     }
   }
   HIP_TRY(hipGetLastError());
@@ -414,6 +469,39 @@ void lfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns, const 
                &ifnear, ti, ier);
 }
 
+
+void rfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns,
+                  const double* sources, const fmm2d_int* ifcharge, const double* charge,
+                  const fmm2d_int* ifdipole, const double* dipstr, const double* dipvec,
+                  fmm2d_int* iper, const fmm2d_int* ifpgh, double* pot, double* grad,
+                  double* hess, const fmm2d_int* nt, const double* targ,
+                  const fmm2d_int* ifpghtarg, double* pottarg, double* gradtarg,
+                  double* hesstarg, const fmm2d_int* ndiv, const fmm2d_int* idivflag,
+                  const fmm2d_int* ifnear, double* timeinfo, fmm2d_int* ier) {
+  *iper = 0;
+  fmm::zero8(timeinfo);
+  LapReq r;
+  r.realk = true;
+  r.nd = *nd; r.eps = *eps; r.ns = *ns; r.nt = *nt; r.src = sources; r.trg = targ;
+  r.ifc = *ifcharge; r.ifd = *ifdipole; r.chg = charge; r.dip = dipstr; r.dipvec = dipvec;
+  r.pg = *ifpgh; r.pgt = *ifpghtarg;
+  r.pot = pot; r.grad = grad; r.hess = hess; r.pott = pottarg; r.gradt = gradtarg; r.hesst = hesstarg;
+  r.ndiv = *ndiv; r.idivflag = *idivflag; r.ifnear = *ifnear; r.timeinfo = timeinfo;
+  *ier = fmm::guarded(r);
+}
+
+void rfmm2d_(const fmm2d_int* nd, const double* eps, const fmm2d_int* ns, const double* sources,
+             const fmm2d_int* ifcharge, const double* charge, const fmm2d_int* ifdipole,
+             const double* dipstr, const double* dipvec, fmm2d_int* iper, const fmm2d_int* ifpgh,
+             double* pot, double* grad, double* hess, const fmm2d_int* nt, const double* targ,
+             const fmm2d_int* ifpghtarg, double* pottarg, double* gradtarg, double* hesstarg,
+             fmm2d_int* ier) {
+  fmm2d_int ndiv = fmm::ndiv_rule(*eps, *ns, *nt), idiv = 0, ifnear = 1;
+  double ti[8];
+  rfmm2d_ndiv_(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec, iper, ifpgh, pot,
+               grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg, &ndiv, &idiv,
+               &ifnear, ti, ier);
+}
 
 void hfmm2d_ndiv_(const fmm2d_int* nd, const double* eps, const fmm2d_cplx* zk, const fmm2d_int* ns,
                   const double* sources, const fmm2d_int* ifcharge, const fmm2d_cplx* charge,

@@ -1,7 +1,7 @@
 // direct.hip -- O(N*M) direct sums on the GPU behind the reference's
 // c2d_direct* (c_translation/src/cauchykernels2d.c:31-442), l2d_direct*
-// (lapkernels2d.c:19-494) and h2d_direct* (helmkernels2d.c:21-520) entry
-// points.  One thread per target; sources staged through LDS in tiles of
+// (lapkernels2d.c:19-494), r2d_direct* (rlapkernels2d.c:16-491) and
+// h2d_direct* (helmkernels2d.c:21-520) entry points.  One thread per target; sources staged through LDS in tiles of
 // 256.  Outputs are incremented, as in the reference; each routine keeps
 // its own skip test (rr vs thresh^2 or |z| vs thresh, <= or <).
 #include "../../include/fmm2d_hip.h"
@@ -251,6 +251,87 @@ __global__ void __launch_bounds__(TILE) k_h2d(cd zk, const double2* src, int ns,
   }
 }
 
+// Laplace log kernel with real densities and outputs (rlapkernels2d.c:16-491):
+// charge c: pot += c log(rr)/2, grad += c (dx, dy)/rr, hess += c ((rr - 2dx^2),
+// -2dxdy, (rr - 2dy^2))/rr^2; dipole d1 = q v1, d2 = q v2: pot += d1 p1 + d2 p2
+// with p = -(dx, dy)/rr, and its derivatives.  le: skip on rr <= th^2, else <.
+template <bool C, bool Q, int PG>
+__global__ void __launch_bounds__(TILE) k_r2d(const double2* src, int ns, const double* chg,
+                                              const double* dip, const double* dv,
+                                              const double2* trg, int nt, int nd, bool le,
+                                              double th2, double* pot, double* grad,
+                                              double* hess) {
+  __shared__ double2 sx[TILE];
+  __shared__ double sc[TILE];
+  __shared__ double2 sd[TILE];  // (q v1, q v2)
+  const int j = blockIdx.x * TILE + threadIdx.x;
+  const bool act = j < nt;
+  const double2 x = act ? trg[j] : make_double2(0, 0);
+  for (int d = 0; d < nd; ++d) {
+    double P = 0, Gx = 0, Gy = 0, Hxx = 0, Hxy = 0, Hyy = 0;
+    for (int s0 = 0; s0 < ns; s0 += TILE) {
+      int cnt = min(TILE, ns - s0);
+      if ((int)threadIdx.x < cnt) {
+        const size_t i = s0 + threadIdx.x;
+        sx[threadIdx.x] = src[i];
+        if (C) sc[threadIdx.x] = chg[i * nd + d];
+        if (Q) {
+          const double q = dip[i * nd + d];
+          sd[threadIdx.x] = make_double2(q * dv[(i * 2 + 0) * nd + d], q * dv[(i * 2 + 1) * nd + d]);
+        }
+      }
+      __syncthreads();
+      if (act)
+        for (int m = 0; m < cnt; ++m) {
+          const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
+          const double dx2 = dx * dx, dy2 = dy * dy, rr = dx2 + dy2;
+          if (le ? rr <= th2 : rr < th2) continue;
+          const double rr2 = rr * rr;
+          if (C) {
+            const double c = sc[m];
+            P += c * (log(rr) / 2);
+            if (PG >= 2) { Gx += c * (dx / rr); Gy += c * (dy / rr); }
+            if (PG >= 3) {
+              Hxx += c * ((rr - 2 * dx2) / rr2);
+              Hxy += c * (-2 * dx * dy / rr2);
+              Hyy += c * ((rr - 2 * dy2) / rr2);
+            }
+          }
+          if (Q) {
+            const double d1 = sd[m].x, d2 = sd[m].y;
+            P += d1 * (-dx / rr) + d2 * (-dy / rr);
+            if (PG >= 2) {
+              const double a1 = -(rr - 2 * dx2) / rr2, a2 = 2 * dx * dy / rr2;
+              const double b2 = -(rr - 2 * dy2) / rr2;
+              Gx += d1 * a1 + d2 * a2;
+              Gy += d1 * a2 + d2 * b2;
+            }
+            if (PG >= 3) {
+              const double rr3 = rr2 * rr;
+              const double e1 = -2 * dx * (dx2 - 3 * dy2) / rr3, e2 = 2 * dy * (dy2 - 3 * dx2) / rr3;
+              Hxx += d1 * e1 + d2 * e2;
+              Hxy += d1 * e2 - d2 * e1;
+              Hyy += -(d1 * e1) - d2 * e2;
+            }
+          }
+        }
+      __syncthreads();
+    }
+    if (act) {
+      pot[(size_t)j * nd + d] += P;
+      if (PG >= 2) {
+        grad[((size_t)j * 2 + 0) * nd + d] += Gx;
+        grad[((size_t)j * 2 + 1) * nd + d] += Gy;
+      }
+      if (PG >= 3) {
+        hess[((size_t)j * 3 + 0) * nd + d] += Hxx;
+        hess[((size_t)j * 3 + 1) * nd + d] += Hxy;
+        hess[((size_t)j * 3 + 2) * nd + d] += Hyy;
+      }
+    }
+  }
+}
+
 // host staging for the direct sums: copy in, run, copy the outputs back
 struct DirectIO {
   int nd, ns, nt, ncg, nch;  // gradient / Hessian components per output
@@ -323,6 +404,47 @@ int h2d(int nd, const fmm2d_cplx* zk, const double* sources, int ns, const void*
     k_h2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, st>>>(k, s, ns, cc, qq, vv, t, nt, nd, le,
                                                              th, po, go, ho);
   });
+}
+
+// real-valued direct sums: real strengths, dipole vectors and outputs
+template <bool C, bool Q, int PG>
+int r2d(int nd, const double* sources, int ns, const double* charge, const double* dipstr,
+        const double* dipvec, const double* targ, int nt, double* pot, double* grad,
+        double* hess, double th, bool le) {
+  if (nd < 1 || ns < 0 || nt < 0) return IER_ARGS;
+  if (ns == 0 || nt == 0) return IER_OK;
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  Scratch S(c.st);
+  const size_t nq = (size_t)nd * ns, no = (size_t)nd * nt;
+  double2* s = S.get<double2>(ns);
+  double2* t = S.get<double2>(nt);
+  double* cc = C ? S.get<double>(nq) : nullptr;
+  double* qq = Q ? S.get<double>(nq) : nullptr;
+  double* vv = Q ? S.get<double>(2 * nq) : nullptr;
+  double* po = S.get<double>(no);
+  double* go = PG >= 2 ? S.get<double>(2 * no) : nullptr;
+  double* ho = PG >= 3 ? S.get<double>(3 * no) : nullptr;
+  if (S.err) return S.err;
+  HIP_TRY(hipMemcpyAsync(s, sources, 16 * (size_t)ns, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(t, targ, 16 * (size_t)nt, hipMemcpyHostToDevice, c.st));
+  if (C) HIP_TRY(hipMemcpyAsync(cc, charge, 8 * nq, hipMemcpyHostToDevice, c.st));
+  if (Q) {
+    HIP_TRY(hipMemcpyAsync(qq, dipstr, 8 * nq, hipMemcpyHostToDevice, c.st));
+    HIP_TRY(hipMemcpyAsync(vv, dipvec, 16 * nq, hipMemcpyHostToDevice, c.st));
+  }
+  HIP_TRY(hipMemcpyAsync(po, pot, 8 * no, hipMemcpyHostToDevice, c.st));
+  if (go) HIP_TRY(hipMemcpyAsync(go, grad, 16 * no, hipMemcpyHostToDevice, c.st));
+  if (ho) HIP_TRY(hipMemcpyAsync(ho, hess, 24 * no, hipMemcpyHostToDevice, c.st));
+  k_r2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, c.st>>>(s, ns, cc, qq, vv, t, nt, nd, le,
+                                                             th * th, po, go, ho);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(pot, po, 8 * no, hipMemcpyDeviceToHost, c.st));
+  if (go) HIP_TRY(hipMemcpyAsync(grad, go, 16 * no, hipMemcpyDeviceToHost, c.st));
+  if (ho) HIP_TRY(hipMemcpyAsync(hess, ho, 24 * no, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
 }
 
 template <bool C, bool Q, int PG>
@@ -461,5 +583,34 @@ void l2d_directcdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, c
 }
 void h2d_directcdh_(const fmm2d_int* nd, const fmm2d_cplx* zk, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c, const fmm2d_cplx* q, const double* v, const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, const double* th) {
   fmm::h2d<true, true, 3>(*nd, zk, s, *ns, c, q, v, t, *nt, pot, grad, hess, *th, false);
+}
+// rlapkernels2d.c: skip operators per routine -- cp, dp, dg, dh, cdp: rr <= th^2
+// (:33,149,191,251,318); cg, ch, cdg, cdh: rr < th^2 (:65,106,364,434)
+void r2d_directcp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* t, const fmm2d_int* nt, double* pot, const double* th) {
+  fmm::r2d<true, false, 1>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void r2d_directcg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* t, const fmm2d_int* nt, double* pot, double* grad, const double* th) {
+  fmm::r2d<true, false, 2>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, nullptr, *th, false);
+}
+void r2d_directch_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* t, const fmm2d_int* nt, double* pot, double* grad, double* hess, const double* th) {
+  fmm::r2d<true, false, 3>(*nd, s, *ns, c, nullptr, nullptr, t, *nt, pot, grad, hess, *th, false);
+}
+void r2d_directdp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, const double* th) {
+  fmm::r2d<false, true, 1>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void r2d_directdg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, double* grad, const double* th) {
+  fmm::r2d<false, true, 2>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, grad, nullptr, *th, true);
+}
+void r2d_directdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, double* grad, double* hess, const double* th) {
+  fmm::r2d<false, true, 3>(*nd, s, *ns, nullptr, q, v, t, *nt, pot, grad, hess, *th, true);
+}
+void r2d_directcdp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, const double* th) {
+  fmm::r2d<true, true, 1>(*nd, s, *ns, c, q, v, t, *nt, pot, nullptr, nullptr, *th, true);
+}
+void r2d_directcdg_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, double* grad, const double* th) {
+  fmm::r2d<true, true, 2>(*nd, s, *ns, c, q, v, t, *nt, pot, grad, nullptr, *th, false);
+}
+void r2d_directcdh_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const double* c, const double* q, const double* v, const double* t, const fmm2d_int* nt, double* pot, double* grad, double* hess, const double* th) {
+  fmm::r2d<true, true, 3>(*nd, s, *ns, c, q, v, t, *nt, pot, grad, hess, *th, false);
 }
 }

@@ -93,7 +93,7 @@ __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
     c = max(c, x == 0 ? kMaxDepth
                       : min(kMaxDepth, (__clzll((long long)x) - (64 - 2 * kMaxDepth)) >> 1));
   }
-  // one atomic per block, and none when the block cannot raise the value
+  // the block's maximum in depth[blockIdx.x] (the host takes the maximum)
   __shared__ int wmax[4];
   for (int o = 32; o >= 1; o >>= 1) c = max(c, __shfl_xor(c, o));
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = c;
@@ -101,8 +101,7 @@ __global__ void k_depth(const uint64_t* k, int n, int ndiv, int* depth) {
   if (threadIdx.x == 0) {
     int m = wmax[0];
     for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = max(m, wmax[k]);
-    if (m >= 0 && m > __hip_atomic_load(depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(depth, m);
+    depth[blockIdx.x] = m;
   }
 }
 
@@ -638,17 +637,26 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                 const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
                 int sorted_levels, Plan& plan, double* host_s) {
   double t0 = wall();
-  // ---- 0. depth of the adaptive tree (one read-back)
-  int* d_depth = S.get<int>(1);  // deepest split level, -1 none
+  // ---- 0. depth of the adaptive tree (one read-back of per-block maxima)
+  const bool depth_s = idivflag != 1 && ns > ndiv;
+  const bool depth_t = (idivflag == 1 || idivflag == 2) && nt > ndiv;
+  const int nbs = depth_s ? std::min(1024, nblk(ns - ndiv, 256)) : 0;
+  const int nbt = depth_t ? std::min(1024, nblk(nt - ndiv, 256)) : 0;
+  int* d_depth = S.get<int>(std::max(1, nbs + nbt));  // per block: deepest split level, -1 none
   if (S.err) return S.err;
-  HIP_TRY(hipMemsetAsync(d_depth, 0xff, sizeof(int), c.st));
-  if (idivflag != 1 && ns > ndiv)
-    k_depth<<<std::min(1024, nblk(ns - ndiv, 256)), 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
-  if ((idivflag == 1 || idivflag == 2) && nt > ndiv)
-    k_depth<<<std::min(1024, nblk(nt - ndiv, 256)), 256, 0, c.st>>>(kt, nt, ndiv, d_depth);
+  if (nbs) k_depth<<<nbs, 256, 0, c.st>>>(ks, ns, ndiv, d_depth);
+  if (nbt) k_depth<<<nbt, 256, 0, c.st>>>(kt, nt, ndiv, d_depth + nbs);
   int Ls = -1;
-  int e = sync_read(c, d_depth, 1, &Ls);
-  if (e) return e;
+  int e = IER_OK;
+  if (nbs + nbt) {
+    std::vector<int> bm(nbs + nbt);
+    e = sync_read(c, d_depth, nbs + nbt, bm.data());
+    if (e) return e;
+    for (int m : bm) Ls = std::max(Ls, m);
+  } else {
+    // nothing splits: the copies of the root box still have to land
+    HIP_TRY(hipStreamSynchronize(c.st));
+  }
   if (idivflag < 0 || idivflag > 2) Ls = -1;
   // exact while every box that splits lies above the sorted levels: a box
   // at level sorted_levels holding more than ndiv points shows up as a

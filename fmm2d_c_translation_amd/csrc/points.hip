@@ -269,9 +269,9 @@ __global__ void k_unflag(uint64_t* key, int n) {
 // leaf; a leaf of at most 64 points ranks its (distinct) indices through
 // shuffles, a larger one is sorted in place by one lane
 __global__ void k_leaf_sort(int* idx, const int2* rng, const int* leaves, int nleaves) {
-  const int w = blockIdx.x;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // one wave per leaf
   if (w >= nleaves) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int2 r = rng[leaves[w]];
   const int n = r.y - r.x;
   if (n <= 1) return;
@@ -429,9 +429,9 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   const Span lss = plan.leaves_s.all(), lst = plan.leaves_t.all();
   // ---- original-index order inside leaves, then sorted coordinates
   if (lss.n)
-    k_leaf_sort<<<lss.n, 64, 0, c.st>>>(P.perm_s, D.srng, plan.dwork + lss.off, lss.n);
+    k_leaf_sort<<<nblk(lss.n, 4), 256, 0, c.st>>>(P.perm_s, D.srng, plan.dwork + lss.off, lss.n);
   if (lst.n)
-    k_leaf_sort<<<lst.n, 64, 0, c.st>>>(P.perm_t, D.trng, plan.dwork + lst.off, lst.n);
+    k_leaf_sort<<<nblk(lst.n, 4), 256, 0, c.st>>>(P.perm_t, D.trng, plan.dwork + lst.off, lst.n);
   P.src = S.get<double2>(ns);
   P.trg = S.get<double2>(nt);
   if (S.err) return S.err;

@@ -76,6 +76,30 @@ void lfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
                   fmm2d_cplx *hesstarg, const fmm2d_int *ndiv, const fmm2d_int *idivflag,
                   const fmm2d_int *ifnear, double *timeinfo, fmm2d_int *ier);
 
+/* ---- Laplace, log kernel with real densities: pot (nd,n), grad (nd,2,n),
+ * hess (nd,3,n), all float64; charge/dipstr (nd,ns), dipvec (nd,2,ns) float64 */
+
+/* replaces rfmm2d_ndiv_ (c_translation/src/rfmm2d_ndiv.c:51): the Cauchy
+ * engine on charge and dipstr * (-(v1 + i v2)), outputs Re, (Re, -Im) and
+ * (Re, -Im, -Re) of pot, d/dz and d2/dz2 (rfmm2d_ndiv.c:130-214) */
+void rfmm2d_ndiv_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns,
+                  const double *sources, const fmm2d_int *ifcharge, const double *charge,
+                  const fmm2d_int *ifdipole, const double *dipstr, const double *dipvec,
+                  fmm2d_int *iper, const fmm2d_int *ifpgh, double *pot, double *grad,
+                  double *hess, const fmm2d_int *nt, const double *targ,
+                  const fmm2d_int *ifpghtarg, double *pottarg, double *gradtarg,
+                  double *hesstarg, const fmm2d_int *ndiv, const fmm2d_int *idivflag,
+                  const fmm2d_int *ifnear, double *timeinfo, fmm2d_int *ier);
+
+/* replaces the Fortran rfmm2d (src/laplace/rfmm2d.f:27-30; ndiv by the eps
+ * table of lndiv2d, as it reaches it through cfmm2d, rfmm2d.f:155) */
+void rfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns, const double *sources,
+             const fmm2d_int *ifcharge, const double *charge, const fmm2d_int *ifdipole,
+             const double *dipstr, const double *dipvec, fmm2d_int *iper,
+             const fmm2d_int *ifpgh, double *pot, double *grad, double *hess,
+             const fmm2d_int *nt, const double *targ, const fmm2d_int *ifpghtarg,
+             double *pottarg, double *gradtarg, double *hesstarg, fmm2d_int *ier);
+
 /* ---- Helmholtz: pot = (i/4) sum c H0(k r) + dipoles, grad (nd,2,n), hess (nd,3,n) */
 
 /* replaces hfmm2d_ (c_translation/src/hfmm2d.c:1529) */
@@ -162,6 +186,19 @@ void l2d_directdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *
 void l2d_directcdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, const double *thresh);
 void l2d_directcdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
 void l2d_directcdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
+
+/* ---- direct sums, log kernel with real densities: replace r2d_direct{c,d,cd}{p,g,h}_
+ * (c_translation/src/rlapkernels2d.c:16-491); outputs incremented, all float64,
+ * grad (nd,2,nt), hess (nd,3,nt) */
+void r2d_directcp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *targ, const fmm2d_int *nt, double *pot, const double *thresh);
+void r2d_directcg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *targ, const fmm2d_int *nt, double *pot, double *grad, const double *thresh);
+void r2d_directch_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *targ, const fmm2d_int *nt, double *pot, double *grad, double *hess, const double *thresh);
+void r2d_directdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, const double *thresh);
+void r2d_directdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, double *grad, const double *thresh);
+void r2d_directdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, double *grad, double *hess, const double *thresh);
+void r2d_directcdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, const double *thresh);
+void r2d_directcdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, double *grad, const double *thresh);
+void r2d_directcdh_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns, const double *charge, const double *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, double *pot, double *grad, double *hess, const double *thresh);
 
 /* ---- direct sums, Helmholtz (i/4) H0(k r): replace h2d_direct{c,d,cd}{p,g,h}_
  * (c_translation/src/helmkernels2d.c:21-520); outputs incremented */

@@ -1,5 +1,6 @@
 """GPU direct sums behind c2d_direct*_ (cauchykernels2d.c:31-442), l2d_direct*_
-(lapkernels2d.c:19-494) and h2d_direct*_ (helmkernels2d.c:21-520): every
+(lapkernels2d.c:19-494), r2d_direct*_ (rlapkernels2d.c:16-491) and
+h2d_direct*_ (helmkernels2d.c:21-520): every
 variant against the compiled reference library where it is built, and
 potential/gradient against closed forms (log|z|, 1/z; log r and its
 derivatives; (i/4) H0(kr) and (i/4) k H1(kr) (x-y).v / r with scipy's
@@ -32,9 +33,12 @@ def _inputs(nd, ns, nt, seed):
 def _call(lib, kind, var, nd, zk, s, t, c, q, v, ncg, nch):
     ns, nt = s.shape[1], t.shape[1]
     pg = {"p": 1, "g": 2, "h": 3}[var[-1]]
-    pot = np.zeros((nd, nt), complex, order="F")
-    grad = np.zeros((nd, ncg, nt), complex, order="F")
-    hess = np.zeros((nd, nch, nt), complex, order="F")
+    dt = float if kind == "r2d" else complex
+    if kind == "r2d":
+        c, q = np.asfortranarray(c.real), np.asfortranarray(q.real)
+    pot = np.zeros((nd, nt), dt, order="F")
+    grad = np.zeros((nd, ncg, nt), dt, order="F")
+    hess = np.zeros((nd, nch, nt), dt, order="F")
     args = [C.byref(C.c_int32(nd))]
     if kind == "h2d":
         args.append((C.c_double * 2)(zk.real, zk.imag))
@@ -72,7 +76,9 @@ def _closed(kind, var, zk, s, t, c, q, v):
         pot = (np.log(r) * keep) @ cc + (keep / z) @ qq
         g = (keep / z) @ cc - (keep / z**2) @ qq
         return pot, g[None]
-    if kind == "l2d":
+    if kind in ("l2d", "r2d"):
+        if kind == "r2d":
+            cc, qq = cc.real, qq.real
         pot = (np.log(r) * keep) @ cc + ((-dx / rr) * keep) @ (qq * v1) + ((-dy / rr) * keep) @ (qq * v2)
         gx = ((dx / rr) * keep) @ cc + ((-(rr - 2 * dx**2) / rr**2) * keep) @ (qq * v1) \
             + ((2 * dx * dy / rr**2) * keep) @ (qq * v2)
@@ -91,7 +97,7 @@ def _closed(kind, var, zk, s, t, c, q, v):
     return pot, np.stack([gxc + gxd, gyc + gyd])
 
 
-@pytest.mark.parametrize("kind", ["c2d", "l2d", "h2d"])
+@pytest.mark.parametrize("kind", ["c2d", "l2d", "r2d", "h2d"])
 @pytest.mark.parametrize("var", VARIANTS)
 def test_direct_sums(eng, ref, kind, var):
     print(f"direct {kind} {var}", flush=True)
@@ -112,3 +118,29 @@ def test_direct_sums(eng, ref, kind, var):
     assert rel_l2(pot[0], cp) < 1e-12
     if pg >= 2:
         assert rel_l2(grad[0], cg) < 1e-11
+
+
+@pytest.mark.parametrize("var", ["ch", "dh", "cdh"])
+def test_r2d_hessian_finite_difference(eng, var):
+    """The r2d Hessian against central differences of the r2d gradient
+    (step 1e-6, targets kept 1e-2 away from every source: truncation error
+    ~ (h/r)^2 = 1e-8, rounding ~ 1e-16 |grad| / h ~ 1e-11 relative)."""
+    print(f"r2d hessian fd {var}", flush=True)
+    nd = 1
+    s, t, c, q, v = _inputs(nd, 300, 200, 5)
+    dmin = np.sqrt(((t[:, :, None] - s[:, None, :]) ** 2).sum(0)).min(1)
+    t = np.asfortranarray(t[:, dmin > 1e-2])
+    _, _, hess, _ = _call(eng, "r2d", var, nd, 0, s, t, c, q, v, 2, 3)
+    h = 1e-6
+    gvar = var[:-1] + "g"
+    def g_at(dx, dy):
+        tt = np.asfortranarray(t + np.array([[dx], [dy]]))
+        return _call(eng, "r2d", gvar, nd, 0, s, tt, c, q, v, 2, 3)[1][0]
+    gxp, gxm = g_at(h, 0), g_at(-h, 0)
+    gyp, gym = g_at(0, h), g_at(0, -h)
+    hxx = (gxp[0] - gxm[0]) / (2 * h)
+    hxy = (gyp[0] - gym[0]) / (2 * h)
+    hyy = (gyp[1] - gym[1]) / (2 * h)
+    assert rel_l2(hess[0, 0], hxx) < 1e-6
+    assert rel_l2(hess[0, 1], hxy) < 1e-6
+    assert rel_l2(hess[0, 2], hyy) < 1e-6

@@ -1,6 +1,7 @@
 """GPU parity of the Laplace/Cauchy FMM (cfmm2d_, cfmm2d_ndiv_, lfmm2d_,
-lfmm2d_ndiv_) against the oracle restatement on the same seeded inputs, and
-against the compiled reference where it travelled with the tree.
+lfmm2d_ndiv_, rfmm2d_, rfmm2d_ndiv_) against the oracle restatement on the
+same seeded inputs, and against the compiled reference where it travelled
+with the tree.
 
 Same tree on both sides (the tree is bit-exact), so pot -- whose imaginary
 part depends on box centres (laprouts2d.c:193-224) -- is comparable too.
@@ -82,3 +83,64 @@ def test_lfmm2d(eng, orc, ref, cd, pg):
     a = abi.lfmm2d_call(eng, "lfmm2d_", **kw)
     b = abi.lfmm2d_call(orc, "orc_lfmm2d_", **kw)
     _compare(a, b)
+
+
+@pytest.mark.parametrize("cd", ["c", "d", "cd"])
+@pytest.mark.parametrize("pg", [1, 2, 3])
+def test_rfmm2d(eng, orc, ref, cd, pg):
+    """Real densities (rfmm2d_ndiv.c:51-229).  Against the compiled
+    reference's rfmm2d_ndiv_, and against the oracle's log-kernel FMM with
+    the same strengths as complex numbers: its outputs are then real, and
+    rfmm's pot, grad and hess must equal their real parts (this pins the
+    dipole conversion -(v1 + i v2) and the unpack signs, rfmm2d_ndiv.c:130-214,
+    through an independent path)."""
+    print(f"rfmm2d cd={cd} pg={pg}", flush=True)
+    src, tg, ch, dp, dv = _inputs(13, 2500, 300, 2)
+    kw = dict(eps=1e-6, sources=src, charges=ch.real if "c" in cd else None,
+              dipstr=dp.real if "d" in cd else None, dipvec=dv, pg=pg, targets=tg, pgt=pg, nd=2)
+    a = abi.rfmm2d_call(eng, "rfmm2d_ndiv_", ndiv=15, **kw)
+    assert a.ier == 0 and a.iper == 0
+    for k in ("pot", "grad", "hess", "targ_pot", "targ_grad", "targ_hess"):
+        if k in a:
+            assert a[k].dtype == np.float64
+    b = abi.lfmm2d_call(orc, "orc_lfmm2d_ndiv_", ndiv=15, **kw)
+    for k in ("pot", "grad", "hess", "targ_pot", "targ_grad", "targ_hess"):
+        if k in b:
+            assert np.abs(b[k].imag).max() == 0.0, k
+            e = rel_l2(a[k], b[k].real)
+            assert e < TOL, (k, e)
+    if ref is not None:
+        _compare(a, abi.rfmm2d_call(ref, "rfmm2d_ndiv_", ndiv=15, **kw))
+    c = abi.rfmm2d_call(eng, "rfmm2d_", **kw)
+    assert c.ier == 0 and c.iper == 0
+    d = abi.lfmm2d_call(orc, "orc_lfmm2d_", **kw)
+    for k in ("pot", "grad", "targ_pot"):
+        if k in d:
+            assert rel_l2(c[k], d[k].real) < TOL, k
+
+
+def test_rfmm2d_vs_direct(eng):
+    """rfmm2d at eps 1e-9 against the GPU r2d direct sum (itself checked
+    against closed forms and the reference) on every target."""
+    print("rfmm2d vs r2d direct", flush=True)
+    src, tg, ch, dp, dv = _inputs(17, 4000, 500, 1)
+    kw = dict(eps=1e-9, sources=src, charges=ch.real[0], dipstr=dp.real[0], dipvec=dv, pg=0,
+              targets=tg, pgt=3, nd=1)
+    a = abi.rfmm2d_call(eng, "rfmm2d_ndiv_", ndiv=20, **kw)
+    assert a.ier == 0
+    import ctypes as C
+    P = lambda x: x.ctypes.data_as(C.c_void_p)
+    s = np.asfortranarray(src)
+    t = np.asfortranarray(tg)
+    q = np.asfortranarray(ch.real[:1])
+    d = np.asfortranarray(dp.real[:1])
+    v = np.asfortranarray(dv[:1])
+    pot = np.zeros((1, t.shape[1]), order="F")
+    grad = np.zeros((1, 2, t.shape[1]), order="F")
+    hess = np.zeros((1, 3, t.shape[1]), order="F")
+    i32 = lambda x: C.byref(C.c_int32(x))
+    eng.r2d_directcdh_(i32(1), P(s), i32(s.shape[1]), P(q), P(d), P(v), P(t), i32(t.shape[1]),
+                       P(pot), P(grad), P(hess), C.byref(C.c_double(1e-14)))
+    assert rel_l2(a.targ_pot, pot[0]) < 1e-8
+    assert rel_l2(a.targ_grad, grad[0]) < 1e-8
+    assert rel_l2(a.targ_hess, hess[0]) < 1e-8
