@@ -379,9 +379,11 @@ int solve_problem(fmm2dhip_problem& p) {
     r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
     r.ndiv = ndiv; r.device = p.device_ptrs == 1;
     e = guarded_h(r);
-  } else if (p.kernel == FMM2DHIP_CAUCHY || p.kernel == FMM2DHIP_LOG) {
+  } else if (p.kernel == FMM2DHIP_CAUCHY || p.kernel == FMM2DHIP_LOG ||
+             p.kernel == FMM2DHIP_REAL) {
     LapReq r;
     r.logk = p.kernel == FMM2DHIP_LOG;
+    r.realk = p.kernel == FMM2DHIP_REAL;
     r.nd = p.nd; r.eps = p.eps; r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
     r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
     r.pg = p.ifpgh; r.pgt = p.ifpghtarg;

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import abi
 
-CAUCHY, LOG, HELMHOLTZ = 0, 1, 2
+CAUCHY, LOG, HELMHOLTZ, REAL = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 
 
@@ -37,8 +37,9 @@ class fmm2dhip_problem(C.Structure):
 
 
 class Problem:
-    """One host-memory problem.  kernel: CAUCHY (cfmm2d), LOG (lfmm2d) or
-    HELMHOLTZ (hfmm2d); arrays as for the corresponding single call."""
+    """One host-memory problem.  kernel: CAUCHY (cfmm2d), LOG (lfmm2d),
+    HELMHOLTZ (hfmm2d) or REAL (rfmm2d: float64 strengths and outputs);
+    arrays as for the corresponding single call."""
 
     def __init__(self, kernel, eps, sources, charges=None, dipstr=None, dipvec=None, pg=1,
                  targets=None, pgt=0, nd=1, zk=0.0, ndiv=0, precision=FP64):
@@ -49,8 +50,12 @@ class Problem:
               else np.zeros((2, 1), order="F"))
         nt = tg.shape[1] if targets is not None else 0
         ifc, ifd = int(charges is not None), int(dipstr is not None)
-        ch = abi._strengths(nd, ns, charges, ifc)
-        dp = abi._strengths(nd, ns, dipstr, ifd)
+        if kernel == REAL:
+            ch = abi._farr(charges, np.float64, (nd, ns)) if ifc else np.zeros((nd, 1), order="F")
+            dp = abi._farr(dipstr, np.float64, (nd, ns)) if ifd else np.zeros((nd, 1), order="F")
+        else:
+            ch = abi._strengths(nd, ns, charges, ifc)
+            dp = abi._strengths(nd, ns, dipstr, ifd)
         vec = kernel != CAUCHY
         dv = (abi._farr(dipvec, np.float64, (nd, 2, ns)) if ifd and vec
               else np.zeros((nd, 2, 1), order="F"))
@@ -58,6 +63,9 @@ class Problem:
         self.ns, self.nt = ns, nt
         self.out = abi._outs(nd, ns, pg, ncg, nch)
         self.out_t = abi._outs(nd, nt, pgt, ncg, nch)
+        if kernel == REAL:
+            self.out = tuple(a.real.copy(order="F") for a in self.out)
+            self.out_t = tuple(a.real.copy(order="F") for a in self.out_t)
         self._keep = (src, tg, ch, dp, dv)
         p = fmm2dhip_problem()
         p.kernel, p.nd, p.eps = kernel, nd, float(eps)

@@ -311,14 +311,16 @@ int fmm2dhip_device_info(char *name, int len);
 /* ---- batched and served solves (no counterpart in the reference, whose
  * entry points solve one problem per call).
  *
- * A problem of any of the three kernels, by value.  Field meanings are those
- * of the drop-in calls (cfmm2d_ndiv_, lfmm2d_ndiv_, hfmm2d_ndiv_); arrays
+ * A problem of any of the four kernels, by value.  Field meanings are those
+ * of the drop-in calls (cfmm2d_ndiv_, lfmm2d_ndiv_, hfmm2d_ndiv_,
+ * rfmm2d_ndiv_: real strengths and outputs for FMM2DHIP_REAL); arrays
  * are host memory, or device memory in the layouts of the _dev entry points
  * when device_ptrs is 1.  Outputs are overwritten as by those calls.  The
  * caller keeps the struct and its arrays alive until the problem is done. */
 #define FMM2DHIP_CAUCHY 0    /* cfmm2d */
 #define FMM2DHIP_LOG 1       /* lfmm2d */
 #define FMM2DHIP_HELMHOLTZ 2 /* hfmm2d */
+#define FMM2DHIP_REAL 3      /* rfmm2d */
 typedef struct fmm2dhip_problem {
   int kernel;
   int nd;
@@ -330,7 +332,7 @@ typedef struct fmm2dhip_problem {
   const void *charge;
   int ifdipole;
   const void *dipstr;
-  const double *dipvec; /* log and Helmholtz kernels */
+  const double *dipvec; /* log, real and Helmholtz kernels */
   int ifpgh;
   void *pot, *grad, *hess;
   int nt;

@@ -57,6 +57,8 @@ def _alone(spec):
     with F.precision(spec["precision"]):
         if spec["kernel"] == serve.CAUCHY:
             return abi.cfmm2d_call(lib, "cfmm2d_", spec["eps"], **kw)
+        if spec["kernel"] == serve.REAL:
+            return abi.rfmm2d_call(lib, "rfmm2d_", spec["eps"], dipvec=spec.get("dipvec"), **kw)
         if spec["kernel"] == serve.LOG:
             return abi.lfmm2d_call(lib, "lfmm2d_", spec["eps"], dipvec=spec.get("dipvec"), **kw)
         return abi.hfmm2d_call(lib, "hfmm2d_", spec["eps"], spec["zk"],
@@ -80,6 +82,29 @@ def test_batch_matches_solves_alone():
     for i, (s, p) in enumerate(zip(specs, probs)):
         _same(p.result(), _alone(s), i)
         assert p.c.seconds > 0 and p.c.latency >= p.c.seconds * 0.999
+
+
+def test_batch_real_kernel_matches_alone():
+    """rfmm2d problems (real strengths) in a batch with the other kernels."""
+    specs = _problem_specs(34, 8, nmax=8000)
+    rng = np.random.default_rng(35)
+    for s in specs[::2]:
+        n = s["sources"].shape[1]
+        s["kernel"] = serve.REAL
+        s["charges"] = rng.standard_normal(n)
+        if s["dipstr"] is not None:
+            s["dipstr"] = rng.standard_normal(n)
+            a = rng.random(n) * 2 * np.pi
+            s["dipvec"] = np.stack([np.cos(a), np.sin(a)])[None]
+        s.pop("zk", None)
+    probs = [serve.Problem(**s) for s in specs]
+    print(f"batch with real kernels: {len(probs)}", flush=True)
+    assert serve.batch(F.lib(), probs, nworkers=4) == 0, [p.ier for p in probs]
+    for i, (s, p) in enumerate(zip(specs, probs)):
+        r = p.result()
+        if s["kernel"] == serve.REAL:
+            assert r["pot"].dtype == np.float64
+        _same(r, _alone(s), i)
 
 
 def test_batch_reuses_pool_and_grows_it():
