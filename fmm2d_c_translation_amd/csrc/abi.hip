@@ -309,29 +309,44 @@ int run_helm(const HelmReq& r) {
   a.pot = zeros(r.pg, 1, ns, 1); a.grad = zeros(r.pg, 2, ns, 2); a.hess = zeros(r.pg, 3, ns, 3);
   a.pott = zeros(r.pgt, 1, nt, 1); a.gradt = zeros(r.pgt, 2, nt, 2); a.hesst = zeros(r.pgt, 3, nt, 3);
   if (S.err) return S.err;
-  e = helm_run(c, S, P, plan, a);
-  if (e) return e;
-  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh; };
+  // results in original order, in device memory (caller's, or staging),
+  // written by the near field where it runs (HelmArgs::upot)
+  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh;
+               double2 *dp, *dg, *dh; };
   Out outs[2] = {{r.pg, ns, P.perm_s, a.pot, a.grad, a.hess, r.pot, r.grad, r.hess},
                  {r.pgt, nt, P.perm_t, a.pott, a.gradt, a.hesst, r.pott, r.gradt, r.hesst}};
   for (auto& o : outs) {
+    o.dp = o.dg = o.dh = nullptr;
     if (o.flag <= 0 || o.n == 0) continue;
-    size_t np = (size_t)nd * o.n;
-    double2 *dp = static_cast<double2*>(o.up), *dg = static_cast<double2*>(o.ug),
-            *dh = static_cast<double2*>(o.uh);
+    const size_t np = (size_t)nd * o.n;
+    o.dp = static_cast<double2*>(o.up);
+    o.dg = o.flag >= 2 ? static_cast<double2*>(o.ug) : nullptr;
+    o.dh = o.flag >= 3 ? static_cast<double2*>(o.uh) : nullptr;
     if (!r.device) {
-      dp = S.get<double2>(np);
-      dg = o.flag >= 2 ? S.get<double2>(2 * np) : nullptr;
-      dh = o.flag >= 3 ? S.get<double2>(3 * np) : nullptr;
+      o.dp = S.get<double2>(np);
+      o.dg = o.flag >= 2 ? S.get<double2>(2 * np) : nullptr;
+      o.dh = o.flag >= 3 ? S.get<double2>(3 * np) : nullptr;
       if (S.err) return S.err;
     }
-    scatter_channels(c.st, o.sp, dp, o.perm, o.n, nd, 1);
-    if (o.flag >= 2) scatter_channels(c.st, o.sg, dg, o.perm, o.n, nd, 2);
-    if (o.flag >= 3) scatter_channels(c.st, o.sh, dh, o.perm, o.n, nd, 3);
+  }
+  a.upot = outs[0].dp; a.ugrad = outs[0].dg; a.uhess = outs[0].dh;
+  a.upott = outs[1].dp; a.ugradt = outs[1].dg; a.uhesst = outs[1].dh;
+  e = helm_run(c, S, P, plan, a);
+  if (e) return e;
+  const bool fused[2] = {a.fused_s, a.fused_t};
+  for (int k = 0; k < 2; ++k) {
+    const Out& o = outs[k];
+    if (o.flag <= 0 || o.n == 0) continue;
+    const size_t np = (size_t)nd * o.n;
+    if (!fused[k]) {
+      scatter_channels(c.st, o.sp, o.dp, o.perm, o.n, nd, 1);
+      if (o.flag >= 2) scatter_channels(c.st, o.sg, o.dg, o.perm, o.n, nd, 2);
+      if (o.flag >= 3) scatter_channels(c.st, o.sh, o.dh, o.perm, o.n, nd, 3);
+    }
     if (!r.device) {
-      HIP_TRY(hipMemcpyAsync(o.up, dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, sizeof(double2) * 2 * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c.st));
+      HIP_TRY(hipMemcpyAsync(o.up, o.dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, o.dg, sizeof(double2) * 2 * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c.st));
     }
   }
   HIP_TRY(hipGetLastError());

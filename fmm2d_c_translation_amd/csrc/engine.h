@@ -216,6 +216,14 @@ struct HelmArgs {
   const double* dipvec = nullptr;                            // [2][nd][ns]
   double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources
   double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
+  // Outputs in input order (scatter_channels layout, ncomp 1/2/3), or null.
+  // The near field is the last step to touch a point and covers every
+  // point once (each leaf's list 1 holds the leaf), so when it runs it
+  // writes sorted value + near field there directly; fused_s / fused_t
+  // report for which set it did, and the caller scatters the others.
+  double2 *upot = nullptr, *ugrad = nullptr, *uhess = nullptr;
+  double2 *upott = nullptr, *ugradt = nullptr, *uhesst = nullptr;
+  mutable bool fused_s = false, fused_t = false;
 };
 int helm_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const HelmArgs& a);
 int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier);

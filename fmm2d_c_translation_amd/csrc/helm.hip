@@ -871,7 +871,8 @@ template <bool C, bool Q, int PG, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
        const double2* dip, const double* dvv, int NS, const int2* trng, const double2* tpts,
-       int NT, int nd, double thresh, double2* pot, double2* grad, double2* hess) {
+       int NT, int nd, double thresh, double2* pot, double2* grad, double2* hess,
+       const int* perm, double2* opot, double2* ograd, double2* ohess) {
   // staging (x, charge, dipole strength, dipole vector) and, after each
   // target group, the slice reduction share one LDS buffer
   __shared__ double2 lds[4 * HCAP];
@@ -1049,15 +1050,33 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
           for (int u = 0; u < K; ++u) s[v] += ld(&red[v * WAVE + t * K + u]);
         }
         const size_t o = t0 + t;
-        addc(pot + (size_t)d * NT + o, s[0] * mk(0, 0.25) + s[1] * ik4);
-        if (PG >= 2) {
-          addc(grad + (size_t)d * NT + o, s[2] * ik4);
-          addc(grad + (size_t)(nd + d) * NT + o, s[3] * ik4);
-        }
-        if (PG >= 3) {
-          addc(hess + (size_t)d * NT + o, s[4] * ik4);
-          addc(hess + (size_t)(nd + d) * NT + o, s[5] * ik4);
-          addc(hess + (size_t)(2 * nd + d) * NT + o, s[6] * ik4);
+        if (opot) {
+          // final value straight to the input-order output (point perm[o])
+          const size_t u = (size_t)perm[o];
+          auto put = [](double2* y, const double2* x, cd v) {
+            *y = make_double2(x->x + v.x, x->y + v.y);
+          };
+          put(opot + u * nd + d, pot + (size_t)d * NT + o, s[0] * mk(0, 0.25) + s[1] * ik4);
+          if (PG >= 2) {
+            put(ograd + (u * 2 + 0) * nd + d, grad + (size_t)d * NT + o, s[2] * ik4);
+            put(ograd + (u * 2 + 1) * nd + d, grad + (size_t)(nd + d) * NT + o, s[3] * ik4);
+          }
+          if (PG >= 3) {
+            put(ohess + (u * 3 + 0) * nd + d, hess + (size_t)d * NT + o, s[4] * ik4);
+            put(ohess + (u * 3 + 1) * nd + d, hess + (size_t)(nd + d) * NT + o, s[5] * ik4);
+            put(ohess + (u * 3 + 2) * nd + d, hess + (size_t)(2 * nd + d) * NT + o, s[6] * ik4);
+          }
+        } else {
+          addc(pot + (size_t)d * NT + o, s[0] * mk(0, 0.25) + s[1] * ik4);
+          if (PG >= 2) {
+            addc(grad + (size_t)d * NT + o, s[2] * ik4);
+            addc(grad + (size_t)(nd + d) * NT + o, s[3] * ik4);
+          }
+          if (PG >= 3) {
+            addc(hess + (size_t)d * NT + o, s[4] * ik4);
+            addc(hess + (size_t)(nd + d) * NT + o, s[5] * ik4);
+            addc(hess + (size_t)(2 * nd + d) * NT + o, s[6] * ik4);
+          }
         }
       }
       __syncthreads();
@@ -1122,11 +1141,13 @@ template <bool C, bool Q>
 void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& D, cd zk,
                  const double2* src, const double2* chg, const double2* dip, const double* dv,
                  int NS, const int2* trng, const double2* tpts, int NT, int nd, double th,
-                 double xmax, double2* pot, double2* grad, double2* hess) {
+                 double xmax, double2* pot, double2* grad, double2* hess, const int* perm,
+                 double2* opot, double2* ograd, double2* ohess) {
   const int nr = xmax <= 0.0 ? 0 : xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
 #define FMM_HP2P(PGV, NRV)                                                                  \
   k_hp2p<C, Q, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, \
-                                               NT, nd, th, pot, grad, hess)
+                                               NT, nd, th, pot, grad, hess, perm, opot, ograd, \
+                                               ohess)
 #define FMM_HP2P_NR(PGV)                                 \
   do {                                                   \
     if (nr == JY_NTERMS0) FMM_HP2P(PGV, JY_NTERMS0);     \
@@ -1460,9 +1481,15 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
                              2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
   }
   // steps 6-8 at sources and targets
-  struct Set { int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
-  Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
-                 {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
+  struct Set {
+    int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he;
+    const int* perm; double2 *uo, *ug, *uh; bool* fused;
+  };
+  Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad,
+                  a.hess, Pt.perm_s, a.upot, a.ugrad, a.uhess, &a.fused_s},
+                 {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott,
+                  a.gradt, a.hesst, Pt.perm_t, a.upott, a.ugradt, a.uhesst, &a.fused_t}};
+  a.fused_s = a.fused_t = false;
   const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
   // k r over list 1: a leaf of size s meets boxes of size 2s at most, so
   // |dx|, |dy| <= 3 s and r <= 4.25 s, s the largest such leaf
@@ -1494,11 +1521,12 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
     if (chg && dip)
-      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
+      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
     else if (chg)
-      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
+      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
     else
-      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he);
+      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
+    *s.fused = s.uo != nullptr;
   }
   tm->mark(c.st);
   HIP_TRY(hipGetLastError());

@@ -91,3 +91,21 @@ def test_hfmm2d_matches_reference_fixtures(eng, path, eps):
     assert a.ier == 0
     for k in KEYS:
         assert rel_l2(a[k], g[f"hfmm_{tag}_{k}"]) < TOL, k
+
+
+@pytest.mark.parametrize("ifnear", [0, 1])
+@pytest.mark.parametrize("pg", [1, 3])
+def test_hfmm2d_ifnear(eng, orc, ref, ifnear, pg):
+    """ifnear = 0 skips the near field (hfmm2d.c:1465): the outputs then
+    come from the far field alone and reach input order by the separate
+    scatter; with ifnear = 1 the near-field kernel writes them there.  Both
+    against the oracle and, where built, the compiled reference."""
+    print(f"hfmm2d_ndiv ifnear={ifnear} pg={pg}", flush=True)
+    src, tg, ch, dp, dv = _inputs(21, 3000, 500, 2)
+    kw = dict(eps=1e-8, zk=25.0, sources=src, charges=ch, dipstr=dp, dipvec=dv, pg=pg,
+              targets=tg, pgt=pg, nd=2, ndiv=20, ifnear=ifnear)
+    a = abi.hfmm2d_call(eng, "hfmm2d_ndiv_", **kw)
+    b = abi.hfmm2d_call(orc, "orc_hfmm2d_ndiv_", **kw)
+    _compare(a, b)
+    if ref is not None:
+        _compare(a, abi.hfmm2d_call(ref, "hfmm2d_ndiv_", **kw))
