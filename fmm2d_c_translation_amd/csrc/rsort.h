@@ -51,15 +51,15 @@ __global__ void k_rs_init(uint32_t* state, size_t nwords) {
 }
 
 // digit counts of every pass: hist[p * 256 + d]
-constexpr int kHistThreads = 1024;
-template <class K>
-__global__ void __launch_bounds__(kHistThreads)
+// TH threads per block: 1024 for large sorts (more waves per LDS histogram),
+// 256 for small ones, whose blocks then also fit beside other streams' work
+template <class K, int TH>
+__global__ void __launch_bounds__(TH)
 k_rs_hist(const K* __restrict__ key, int n, int begin, int end, int npass, uint32_t* hist) {
   __shared__ uint32_t h[8 * 256];
-  for (int i = threadIdx.x; i < npass * 256; i += kHistThreads) h[i] = 0;
+  for (int i = threadIdx.x; i < npass * 256; i += TH) h[i] = 0;
   __syncthreads();
-  for (long i = (long)blockIdx.x * kHistThreads + threadIdx.x; i < n;
-       i += (long)gridDim.x * kHistThreads) {
+  for (long i = (long)blockIdx.x * TH + threadIdx.x; i < n; i += (long)gridDim.x * TH) {
     const K k = key[i];
     for (int p = 0; p < npass; ++p) {
       const int sh = begin + 8 * p, wd = min(8, end - sh);
@@ -67,7 +67,7 @@ k_rs_hist(const K* __restrict__ key, int n, int begin, int end, int npass, uint3
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < npass * 256; i += kHistThreads)
+  for (int i = threadIdx.x; i < npass * 256; i += TH)
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
@@ -232,8 +232,12 @@ int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* v
   uint32_t* counter = look + nlook;
   uint32_t* e = err;
   k_rs_init<<<(unsigned)std::min<size_t>(1024, (nwords + 255) / 256), 256, 0, c.st>>>(state, nwords);
-  k_rs_hist<K><<<std::min(256, (n + 4 * kHistThreads - 1) / (4 * kHistThreads)), kHistThreads, 0,
-                 c.st>>>(kin, n, begin, end, npass, hist);
+  if (n >= (1 << 20))
+    k_rs_hist<K, 1024><<<std::min(256, (n + 4095) / 4096), 1024, 0, c.st>>>(kin, n, begin, end,
+                                                                          npass, hist);
+  else
+    k_rs_hist<K, 256><<<std::min(256, (n + 1023) / 1024), 256, 0, c.st>>>(kin, n, begin, end,
+                                                                        npass, hist);
   const K* ks = kin;
   const int* vs = vin;
   for (int p = 0; p < npass; ++p) {
