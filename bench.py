@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE and WRITE_SIZE of k_hp2p, each scaled by what the same
 # counter reads for a 1 GiB device copy (x2.0 and x1.0); the commands and
 # numbers are in profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md
-P2P_TRAFFIC_BYTES_1M = 746.0e6
+P2P_TRAFFIC_BYTES_1M = 780.8e6
 
 
 def make_inputs(n, nt, seed=2):
