@@ -192,40 +192,60 @@ int run_lap(const LapReq& r) {
   a.pot = zeros(r.pg, 1, ns); a.grad = zeros(r.pg, 2, ns); a.hess = zeros(r.pg, 3, ns);
   a.pott = zeros(r.pgt, 1, nt); a.gradt = zeros(r.pgt, 2, nt); a.hesst = zeros(r.pgt, 3, nt);
   if (S.err) return S.err;
-  e = lap_run(c, S, P, plan, a);
-  if (e) return e;
-
-  // results in original order, in device memory (caller's, or staging)
-  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh; };
+  // results in original order, in device memory (caller's, or staging),
+  // written by the near field where it runs (LapArgs::out_s / out_t)
+  struct Out { int flag, n; const int* perm; double2 *sp, *sg, *sh; void *up, *ug, *uh;
+               void *dp, *dg, *dh; };
   Out outs[2] = {{r.pg, ns, P.perm_s, a.pot, a.grad, a.hess, r.pot, r.grad, r.hess},
                  {r.pgt, nt, P.perm_t, a.pott, a.gradt, a.hesst, r.pott, r.gradt, r.hesst}};
-  for (auto& o : outs) {
+  LapOut* lo[2] = {&a.out_s, &a.out_t};
+  for (int k = 0; k < 2; ++k) {
+    Out& o = outs[k];
+    o.dp = o.dg = o.dh = nullptr;
     if (o.flag <= 0 || o.n == 0) continue;
-    size_t np = (size_t)nd * o.n, ng = np * ncomp_g, nh = np * ncomp_h;
-    void *dp = o.up, *dg = o.ug, *dh = o.uh;
+    const size_t np = (size_t)nd * o.n, ng = np * ncomp_g, nh = np * ncomp_h;
+    o.dp = o.up;
+    o.dg = o.flag >= 2 ? o.ug : nullptr;
+    o.dh = o.flag >= 3 ? o.uh : nullptr;
     if (!r.device) {
-      dp = S.get<char>(esz * np);
-      dg = o.flag >= 2 ? S.get<char>(esz * ng) : nullptr;
-      dh = o.flag >= 3 ? S.get<char>(esz * nh) : nullptr;
+      o.dp = S.get<char>(esz * np);
+      o.dg = o.flag >= 2 ? S.get<char>(esz * ng) : nullptr;
+      o.dh = o.flag >= 3 ? S.get<char>(esz * nh) : nullptr;
       if (S.err) return S.err;
     }
-    if (r.realk) {
+    lo[k]->kind = r.realk ? 2 : r.logk ? 1 : 0;
+    lo[k]->nd = nd;
+    lo[k]->perm = o.perm;
+    lo[k]->pot = o.dp;
+    lo[k]->grad = o.dg;
+    lo[k]->hess = o.dh;
+  }
+  e = lap_run(c, S, P, plan, a);
+  if (e) return e;
+  const bool fused[2] = {a.fused_s, a.fused_t};
+  for (int k = 0; k < 2; ++k) {
+    const Out& o = outs[k];
+    if (o.flag <= 0 || o.n == 0) continue;
+    const size_t np = (size_t)nd * o.n, ng = np * ncomp_g, nh = np * ncomp_h;
+    if (fused[k]) {
+      // written by the near field
+    } else if (r.realk) {
       k_rfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(
-          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double*>(dp),
-          static_cast<double*>(dg), static_cast<double*>(dh));
+          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double*>(o.dp),
+          static_cast<double*>(o.dg), static_cast<double*>(o.dh));
     } else if (r.logk) {
       k_lfmm_unpack<<<nblk(o.n, 256), 256, 0, c.st>>>(
-          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double2*>(dp),
-          static_cast<double2*>(dg), static_cast<double2*>(dh));
+          o.sp, o.sg, o.sh, o.perm, o.n, nd, o.flag, static_cast<double2*>(o.dp),
+          static_cast<double2*>(o.dg), static_cast<double2*>(o.dh));
     } else {
-      scatter_channels(c.st, o.sp, static_cast<double2*>(dp), o.perm, o.n, nd, 1);
-      if (o.flag >= 2) scatter_channels(c.st, o.sg, static_cast<double2*>(dg), o.perm, o.n, nd, 1);
-      if (o.flag >= 3) scatter_channels(c.st, o.sh, static_cast<double2*>(dh), o.perm, o.n, nd, 1);
+      scatter_channels(c.st, o.sp, static_cast<double2*>(o.dp), o.perm, o.n, nd, 1);
+      if (o.flag >= 2) scatter_channels(c.st, o.sg, static_cast<double2*>(o.dg), o.perm, o.n, nd, 1);
+      if (o.flag >= 3) scatter_channels(c.st, o.sh, static_cast<double2*>(o.dh), o.perm, o.n, nd, 1);
     }
     if (!r.device) {
-      HIP_TRY(hipMemcpyAsync(o.up, dp, esz * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, dg, esz * ng, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, dh, esz * nh, hipMemcpyDeviceToHost, c.st));
+      HIP_TRY(hipMemcpyAsync(o.up, o.dp, esz * np, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, o.dg, esz * ng, hipMemcpyDeviceToHost, c.st));
+      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, esz * nh, hipMemcpyDeviceToHost, c.st));
     }
   }
   HIP_TRY(hipGetLastError());

@@ -192,6 +192,18 @@ void gather_real(hipStream_t st, const double* in, double* out, const int* perm,
 void scatter_channels(hipStream_t st, const double2* in, double2* out, const int* perm, int n,
                       int nd, int ncomp);
 
+// Final outputs of one point set in input order, for the near-field kernel
+// to write (pot null: not used).  kind 0: Cauchy, complex, one component
+// each (scatter_channels layout); 1: log kernel with complex densities,
+// sorted channel 2d + h giving the real (h = 0) or imaginary (h = 1) part of
+// the caller's channel d (lfmm2d unpack); 2: log kernel with real outputs
+// (rfmm2d unpack).  nd: the caller's channels.
+struct LapOut {
+  int kind = 0, nd = 1;
+  const int* perm = nullptr;
+  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
+};
+
 // Cauchy-kernel FMM on sorted, device-resident data (cfmm2dmain semantics).
 struct LapArgs {
   int nd = 1;
@@ -201,6 +213,12 @@ struct LapArgs {
   const double2 *chg = nullptr, *dip = nullptr;      // sorted, channel-major
   double2 *pot = nullptr, *grad = nullptr, *hess = nullptr;   // sorted sources
   double2 *pott = nullptr, *gradt = nullptr, *hesst = nullptr;  // sorted targets
+  // The near field is the last step to touch a point and covers every
+  // point once (each leaf's list 1 holds the leaf): where it runs it writes
+  // sorted value + near field straight to out_s / out_t, and fused_s /
+  // fused_t say so; the caller unpacks the other sets.
+  LapOut out_s, out_t;
+  mutable bool fused_s = false, fused_t = false;
 };
 int lap_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const LapArgs& a);
 

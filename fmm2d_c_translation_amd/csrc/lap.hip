@@ -158,6 +158,46 @@ int get_tables(Ctx& c, int p, LapTables& t) {
 __device__ inline void addc(double2* o, cd v) { *o = make_double2(o->x + v.x, o->y + v.y); }
 __device__ inline cd ldc(const double2* p) { const double2 v = *p; return mk(v.x, v.y); }
 
+// final value of sorted point i, channel dd, to the caller's arrays (LapOut)
+template <int PG>
+__device__ inline void lap_final(const LapOut& u, int dd, size_t i, cd P, cd G, cd H) {
+  const size_t t = (size_t)u.perm[i];
+  const size_t nd = (size_t)u.nd;
+  if (u.kind == 0) {
+    static_cast<double2*>(u.pot)[t * nd + dd] = make_double2(P.x, P.y);
+    if (PG >= 2) static_cast<double2*>(u.grad)[t * nd + dd] = make_double2(G.x, G.y);
+    if (PG >= 3) static_cast<double2*>(u.hess)[t * nd + dd] = make_double2(H.x, H.y);
+  } else if (u.kind == 1) {  // lfmm2d_ndiv.c:171-243
+    const size_t d = dd >> 1, h = dd & 1;
+    double* p = static_cast<double*>(u.pot);
+    p[2 * (t * nd + d) + h] = P.x;
+    if (PG >= 2) {
+      double* g = static_cast<double*>(u.grad);
+      g[2 * ((t * 2 + 0) * nd + d) + h] = G.x;
+      g[2 * ((t * 2 + 1) * nd + d) + h] = -G.y;
+    }
+    if (PG >= 3) {
+      double* q = static_cast<double*>(u.hess);
+      q[2 * ((t * 3 + 0) * nd + d) + h] = H.x;
+      q[2 * ((t * 3 + 1) * nd + d) + h] = -H.y;
+      q[2 * ((t * 3 + 2) * nd + d) + h] = -H.x;
+    }
+  } else {  // rfmm2d_ndiv.c:156-214
+    static_cast<double*>(u.pot)[t * nd + dd] = P.x;
+    if (PG >= 2) {
+      double* g = static_cast<double*>(u.grad);
+      g[(t * 2 + 0) * nd + dd] = G.x;
+      g[(t * 2 + 1) * nd + dd] = -G.y;
+    }
+    if (PG >= 3) {
+      double* q = static_cast<double*>(u.hess);
+      q[(t * 3 + 0) * nd + dd] = H.x;
+      q[(t * 3 + 1) * nd + dd] = -H.y;
+      q[(t * 3 + 2) * nd + dd] = -H.x;
+    }
+  }
+}
+
 // sum over the 16 lanes of a group; every lane receives the sum
 __device__ inline cd gsum16(cd v) {
 #pragma unroll
@@ -382,7 +422,7 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
                                               const double2* chg, const double2* dip, int NS,
                                               const int2* trng, const double2* tpts, int NT,
                                               int nd, double th2, double2* pot, double2* grad,
-                                              double2* hess) {
+                                              double2* hess, LapOut out) {
   // staging (x, charge, dipole) and the slice reduction share one buffer
   __shared__ double2 lds[3 * LCAP];
   double2* sx = lds;
@@ -450,9 +490,14 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
           if (PG >= 3) H += mk(red[2 * WAVE + l].x, red[2 * WAVE + l].y);
         }
         const size_t o = (size_t)d * NT + t0 + t;
-        addc(pot + o, P);
-        if (PG >= 2) addc(grad + o, G);
-        if (PG >= 3) addc(hess + o, H);
+        if (out.pot) {
+          lap_final<PG>(out, d, t0 + t, ldc(pot + o) + P, PG >= 2 ? ldc(grad + o) + G : G,
+                        PG >= 3 ? ldc(hess + o) + H : H);
+        } else {
+          addc(pot + o, P);
+          if (PG >= 2) addc(grad + o, G);
+          if (PG >= 3) addc(hess + o, H);
+        }
       }
       __syncthreads();
     }
@@ -477,7 +522,7 @@ __global__ void __launch_bounds__(WAVE) k_p2p32(const int* work, DTree D, const 
                                                 const double2* chg, const double2* dip, int NS,
                                                 const int2* trng, const double2* tpts, int NT,
                                                 int nd, double th2, double sc, double2* pot,
-                                                double2* grad) {
+                                                double2* grad, LapOut out) {
   // staging: x (FP64), charge and dipole strengths (FP32); the slice
   // reduction reuses the x buffer
   __shared__ double2 lds[2 * LCAP];
@@ -561,8 +606,12 @@ __global__ void __launch_bounds__(WAVE) k_p2p32(const int* work, DTree D, const 
           if (PG >= 2) G += mk(red[WAVE + l].x, red[WAVE + l].y);
         }
         const size_t o = (size_t)d * NT + t0 + t;
-        addc(pot + o, P);
-        if (PG >= 2) addc(grad + o, G);
+        if (out.pot) {
+          lap_final<PG>(out, d, t0 + t, ldc(pot + o) + P, PG >= 2 ? ldc(grad + o) + G : G, G);
+        } else {
+          addc(pot + o, P);
+          if (PG >= 2) addc(grad + o, G);
+        }
       }
       __syncthreads();
     }
@@ -573,10 +622,10 @@ template <bool C, bool Q>
 void launch_p2p32(int pg, int mode, int nbx, hipStream_t st, const int* work, const DTree& D,
                   const double2* src, const double2* chg, const double2* dip, int NS,
                   const int2* trng, const double2* tpts, int NT, int nd, double th2, double sc,
-                  double2* pot, double2* grad) {
+                  double2* pot, double2* grad, const LapOut& out) {
 #define FMM_P2P32(PGV, MV)                                                                       \
   k_p2p32<C, Q, PGV, MV><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, \
-                                               sc, pot, grad)
+                                               sc, pot, grad, out)
   if (pg == 1) {
     if (mode == 0) FMM_P2P32(1, 0);
     else FMM_P2P32(1, 1);
@@ -591,21 +640,21 @@ template <bool C, bool Q, int PG>
 void launch_p2p_mode(int mode, int nbx, hipStream_t st, const int* work, const DTree& D,
                      const double2* src, const double2* chg, const double2* dip, int NS,
                      const int2* trng, const double2* tpts, int NT, int nd, double th2,
-                     double2* pot, double2* grad, double2* hess) {
+                     double2* pot, double2* grad, double2* hess, const LapOut& out) {
   if (mode == 0)
-    k_p2p<C, Q, PG, 0><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess);
+    k_p2p<C, Q, PG, 0><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
   else
-    k_p2p<C, Q, PG, 1><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess);
+    k_p2p<C, Q, PG, 1><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
 }
 
 template <bool C, bool Q>
 void launch_p2p_pg(int pg, int mode, int nbx, hipStream_t st, const int* work, const DTree& D,
                    const double2* src, const double2* chg, const double2* dip, int NS,
                    const int2* trng, const double2* tpts, int NT, int nd, double th2,
-                   double2* pot, double2* grad, double2* hess) {
-  if (pg == 1) launch_p2p_mode<C, Q, 1>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess);
-  if (pg == 2) launch_p2p_mode<C, Q, 2>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess);
-  if (pg == 3) launch_p2p_mode<C, Q, 3>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess);
+                   double2* pot, double2* grad, double2* hess, const LapOut& out) {
+  if (pg == 1) launch_p2p_mode<C, Q, 1>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
+  if (pg == 2) launch_p2p_mode<C, Q, 2>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
+  if (pg == 3) launch_p2p_mode<C, Q, 3>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
 }
 
 }  // namespace
@@ -697,9 +746,13 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 1, TL, P, P, lo, per, lo, per, nd, 0.0));
   }
   // steps 6 + 7: list 3 and local expansions at sources / targets
-  struct Set { int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he; };
-  Set sets[2] = {{a.ifpgh, plan.list3_s.all(), plan.leaves_s.all(), plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess},
-                 {a.ifpghtarg, plan.list3_t.all(), plan.leaves_t.all(), plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst}};
+  struct Set {
+    int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he;
+    const LapOut* out; bool* fused;
+  };
+  Set sets[2] = {{a.ifpgh, plan.list3_s.all(), plan.leaves_s.all(), plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad, a.hess, &a.out_s, &a.fused_s},
+                 {a.ifpghtarg, plan.list3_t.all(), plan.leaves_t.all(), plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott, a.gradt, a.hesst, &a.out_t, &a.fused_t}};
+  a.fused_s = a.fused_t = false;
   // reference skip tests: charges (with or without dipoles) use rr vs thresh^2,
   // '<=' for potential only and '<' otherwise; dipole-only routines compare |z|
   const double thresh = Pt.bs0 * 0x1p-51;
@@ -724,20 +777,22 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
         // FP32 pair terms (the Hessian's 1/z^3 can leave the FP32 range,
         // so pg = 3 stays FP64)
         const double sc = Pt.bs0 > 0.0 ? std::ldexp(1.0, -std::ilogb(Pt.bs0)) : 1.0;
+        *s.fused = s.out->pot != nullptr;
         if (chg && dip)
-          launch_p2p32<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+          launch_p2p32<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr, *s.out);
         else if (chg)
-          launch_p2p32<true, false>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+          launch_p2p32<true, false>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr, *s.out);
         else
-          launch_p2p32<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr);
+          launch_p2p32<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, sc, s.po, s.gr, *s.out);
         continue;
       }
+      *s.fused = s.out->pot != nullptr;
       if (chg && dip)
-        launch_p2p_pg<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he);
+        launch_p2p_pg<true, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he, *s.out);
       else if (chg)
-        launch_p2p_pg<true, false>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he);
+        launch_p2p_pg<true, false>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he, *s.out);
       else
-        launch_p2p_pg<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he);
+        launch_p2p_pg<false, true>(s.pg, mode, s.nn.n, c.st, W + s.nn.off, D, Pt.src, chg, dip, Pt.ns, s.rng, s.pts, s.N, nd, th2, s.po, s.gr, s.he, *s.out);
     }
   }
   tm->mark(c.st);
