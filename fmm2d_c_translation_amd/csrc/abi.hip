@@ -315,9 +315,7 @@ int run_helm(const HelmReq& r) {
   double2* qs = r.ifd ? S.get<double2>((size_t)nd * ns) : nullptr;
   double* vs = r.ifd ? S.get<double>(2 * (size_t)nd * ns) : nullptr;
   if (S.err) return S.err;
-  if (cs) gather_channels(c.st, chg, cs, P.perm_s, ns, nd, 1);
-  if (qs) gather_channels(c.st, dip, qs, P.perm_s, ns, nd, 1);
-  if (vs) gather_real(c.st, dv, vs, P.perm_s, ns, nd, 2);
+  gather_sources(c.st, chg, cs, dip, qs, dv, vs, P.perm_s, ns, nd);
   a.chg = cs; a.dip = qs; a.dipvec = vs;
   auto zeros = [&](int flag, int need, int n, int ncomp) -> double2* {
     if (flag < need) return nullptr;

@@ -125,7 +125,7 @@ inline int sort_status(const Plan& p) { return p.sort_err && *p.sort_err ? IER_H
 // the root box {centre x, centre y, size}, written by a copy already queued
 // on the stream (read after the first read-back).  host_s receives the wall
 // time of refinement, level restriction and boxes+lists.
-constexpr int kSortLevels = 19;  // with the target flag, 39 bits: five 8-bit passes
+constexpr int kSortLevels = 15;  // with the target flag, 31 bits: four 8-bit passes
 constexpr int IER_RESORT = -1;  // internal: sort on more key bits and retry
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
                 const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
@@ -188,6 +188,10 @@ void gather_channels(hipStream_t st, const double2* in, double2* out, const int*
                      int nd, int ncomp);
 void gather_real(hipStream_t st, const double* in, double* out, const int* perm, int n, int nd,
                  int ncomp);
+// Charges, dipole strengths and dipole vectors (each may be null) in one
+// pass: chg1/dip1 as gather_channels(ncomp 1), dv1 as gather_real(ncomp 2).
+void gather_sources(hipStream_t st, const double2* chg, double2* chg1, const double2* dip,
+                    double2* dip1, const double* dv, double* dv1, const int* perm, int n, int nd);
 // Inverse: out[(perm[i]*ncomp + c)*nd + d] = in[(c*nd + d)*n + i].
 void scatter_channels(hipStream_t st, const double2* in, double2* out, const int* perm, int n,
                       int nd, int ncomp);

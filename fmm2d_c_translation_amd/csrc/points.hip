@@ -268,9 +268,15 @@ __global__ void k_unflag(uint64_t* key, int n) {
 // original indices of each leaf's points in increasing order: a wave per
 // leaf; a leaf of at most 64 points ranks its (distinct) indices through
 // shuffles, a larger one is sorted in place by one lane
-__global__ void k_leaf_sort(int* idx, const int2* rng, const int* leaves, int nleaves) {
-  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // one wave per leaf
-  if (w >= nleaves) return;
+__global__ void k_leaf_sort(int* idx_s, const int2* rng_s, const int* leaves_s, int nls,
+                            int* idx_t, const int2* rng_t, const int* leaves_t, int nlt) {
+  int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // one wave per leaf
+  if (w >= nls + nlt) return;
+  const bool tg = w >= nls;  // source leaves first, then target leaves
+  if (tg) w -= nls;
+  int* idx = tg ? idx_t : idx_s;
+  const int2* rng = tg ? rng_t : rng_s;
+  const int* leaves = tg ? leaves_t : leaves_s;
   const int lane = threadIdx.x & 63;
   const int2 r = rng[leaves[w]];
   const int n = r.y - r.x;
@@ -306,6 +312,29 @@ __global__ void k_gather_ch(const double2* in, double2* out, const int* perm, in
   for (int c = 0; c < ncomp; ++c)
     for (int d = 0; d < nd; ++d)
       out[(size_t)(c * nd + d) * n + i] = in[((size_t)src * ncomp + c) * nd + d];
+}
+
+// Helmholtz source strengths in one pass over the permutation: charges and
+// dipole strengths (complex, nd per point) and dipole vectors (2 nd reals),
+// each to channel-major sorted order (as k_gather_ch / k_gather_re)
+__global__ void k_gather_src(const double2* chg, double2* chg1, const double2* dip, double2* dip1,
+                             const double* dv, double* dv1, const int* perm, int n, int nd) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t s = (size_t)perm[i];
+  for (int d = 0; d < nd; ++d) {
+    if (chg1) chg1[(size_t)d * n + i] = chg[s * nd + d];
+    if (dip1) dip1[(size_t)d * n + i] = dip[s * nd + d];
+  }
+  if (dv1) {
+    if (nd == 1 && (reinterpret_cast<uintptr_t>(dv) & 15) == 0) {  // one 16-byte load
+      const double2 v = reinterpret_cast<const double2*>(dv)[s];
+      dv1[i] = v.x;
+      dv1[(size_t)n + i] = v.y;
+    } else {
+      for (int c = 0; c < 2 * nd; ++c) dv1[(size_t)c * n + i] = dv[s * 2 * nd + c];
+    }
+  }
 }
 
 __global__ void k_gather_re(const double* in, double* out, const int* perm, int n, int nd,
@@ -357,6 +386,11 @@ int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* t
 void gather_channels(hipStream_t st, const double2* in, double2* out, const int* perm, int n,
                      int nd, int ncomp) {
   if (n > 0) k_gather_ch<<<nblk(n, 256), 256, 0, st>>>(in, out, perm, n, nd, ncomp);
+}
+void gather_sources(hipStream_t st, const double2* chg, double2* chg1, const double2* dip,
+                    double2* dip1, const double* dv, double* dv1, const int* perm, int n,
+                    int nd) {
+  if (n > 0) k_gather_src<<<nblk(n, 256), 256, 0, st>>>(chg, chg1, dip, dip1, dv, dv1, perm, n, nd);
 }
 void gather_real(hipStream_t st, const double* in, double* out, const int* perm, int n, int nd,
                  int ncomp) {
@@ -428,10 +462,10 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   th[5] = wall();
   const Span lss = plan.leaves_s.all(), lst = plan.leaves_t.all();
   // ---- original-index order inside leaves, then sorted coordinates
-  if (lss.n)
-    k_leaf_sort<<<nblk(lss.n, 4), 256, 0, c.st>>>(P.perm_s, D.srng, plan.dwork + lss.off, lss.n);
-  if (lst.n)
-    k_leaf_sort<<<nblk(lst.n, 4), 256, 0, c.st>>>(P.perm_t, D.trng, plan.dwork + lst.off, lst.n);
+  if (lss.n + lst.n)
+    k_leaf_sort<<<nblk(lss.n + lst.n, 4), 256, 0, c.st>>>(
+        P.perm_s, D.srng, plan.dwork + lss.off, lss.n, P.perm_t, D.trng, plan.dwork + lst.off,
+        lst.n);
   P.src = S.get<double2>(ns);
   P.trg = S.get<double2>(nt);
   if (S.err) return S.err;

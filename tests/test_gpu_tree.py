@@ -41,7 +41,9 @@ CASES = [("unif", 1, 3000, 8, 0), ("unif", 2, 3000, 8, 1), ("unif", 3, 3000, 8, 
          ("circle", 4, 8000, 10, 0), ("blob", 5, 6000, 20, 0), ("circle", 6, 2500, 3, 0),
          ("grid", 7, 4096, 20, 0), ("twoblob", 8, 4000, 5, 0), ("line", 9, 3000, 4, 0),
          ("twoblob", 10, 3000, 1, 2), ("unif", 11, 1, 1, 0), ("unif", 12, 40, 100, 0),
-         ("unif", 13, 200000, 20, 0), ("circle", 14, 100000, 15, 2)]
+         ("unif", 13, 200000, 20, 0), ("circle", 14, 100000, 15, 2),
+         # deeper than the first point sort covers: the keys are sorted again on every bit
+         ("deep", 15, 3000, 10, 0), ("deeper", 16, 3000, 10, 2)]
 
 
 @pytest.mark.parametrize("kind,seed,n,ndiv,idiv", CASES)

@@ -28,6 +28,12 @@ def make_points(kind, seed, n):
     if kind == "line":
         t = rng.random(n)
         return np.stack([t, 0.3 + 1e-4 * t])
+    if kind in ("deep", "deeper"):  # uniform plus a tight cluster: trees 17-25 levels deep
+        src = rng.random((2, n))
+        sig = 1e-5 if kind == "deep" else 1e-7
+        k = n // 6
+        src[:, :k] = np.array([[0.3], [0.7]]) + sig * rng.standard_normal((2, k))
+        return src
     if kind == "grid":
         g = np.linspace(0, 1, int(np.sqrt(n)))
         X, Y = np.meshgrid(g, g)
