@@ -198,32 +198,35 @@ __device__ inline void lap_final(const LapOut& u, int dd, size_t i, cd P, cd G, 
   }
 }
 
-// sum over the 16 lanes of a group; every lane receives the sum
-__device__ inline cd gsum16(cd v) {
-#pragma unroll
-  for (int o = 8; o >= 1; o >>= 1) {
-    v.x += __shfl_xor(v.x, o, 16);
-    v.y += __shfl_xor(v.y, o, 16);
-  }
-  return v;
-}
 
 // Leaf multipoles (MP) from the leaf's sources, or local expansions from
-// the sources of the list-4 boxes (!MP).  Lane u forms its point's term of
-// coefficient n, the group sums it, and lane n % 16 keeps it (p <= 63):
+// the sources of the list-4 boxes (!MP).  Four boxes per wave, a 16-lane
+// group per box, lane u on point u of each chunk of 16 sources: the lane
+// forms its point's terms of 16 coefficients at a time into a 16 x 17 LDS
+// tile, and lane u then sums coefficient u's 16 terms in point order (p <= 63):
 //   multipole (laprouts2d.c:41-182):  mp_0 += c,
 //     mp_n += -c (z/r)^n / n + d z^(n-1) / r^n
 //   local (laprouts2d.c:358-500):     lo_0 += c log|z| - d / z,
 //     lo_n += -c (r/z)^n / n - d (r/z)^n / z
 // with z = source - centre and r the box size.
+constexpr int FT_PAD = 17;  // tile row length (16 terms + 1: no bank conflicts)
+
+__device__ inline void lap_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool MP>
 __global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree D,
                                                const double2* pts, const double2* chg,
                                                const double2* dip, int N, int nd, int p,
                                                double2* out) {
-  const int u = threadIdx.x & 15;
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 4);
-  if (w >= nwork) return;  // whole groups leave; sums never cross groups
+  __shared__ cd tile[4 * 16 * FT_PAD];
+  const int grp = (int)threadIdx.x >> 4, u = (int)threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + grp;
+  if (w >= nwork) return;  // whole groups leave; only wave-level syncs below
+  cd* T = tile + grp * 16 * FT_PAD;
   const int b = work[w];
   const int P = p + 1;
   const double2 ctr = D.ctr[b];
@@ -242,29 +245,26 @@ __global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree
         const cd c = ok && chg ? ldc(&chg[(size_t)d * N + j]) : mk(0, 0);
         const cd q = ok && dip ? ldc(&dip[(size_t)d * N + j]) : mk(0, 0);
         const cd z = mk(s.x - ctr.x, s.y - ctr.y);
-        // running powers: MP w = (z/r)^n, qw = d z^(n-1) / r^n;
-        // local w = (r/z)^n, qw = -d (r/z)^n / z
-        cd w0, step, qw;
-        cd term0;
+        // running powers: MP pw = (z/r)^n, qw = d z^(n-1) / r^n;
+        // local pw = (r/z)^n, qw = -d (r/z)^n / z
+        cd step, qw, term0;
         if (MP) {
-          w0 = mk(1, 0);
           step = z * ri;
           qw = q * ri;
           term0 = c;
         } else {
           const cd zi = inv(z);
-          w0 = mk(1, 0);
           step = r * zi;
           qw = -(q * zi);
           term0 = c * (0.5 * log_pos(norm2(z))) + qw;
         }
-        cd pw = w0;
+        cd pw = mk(1, 0);
 #pragma unroll
         for (int hi = 0; hi < 4; ++hi) {
           if (16 * hi > p) break;
-          for (int lo = 0; lo < 16; ++lo) {
+          const int nfill = min(16, P - 16 * hi);
+          for (int lo = 0; lo < nfill; ++lo) {
             const int n = 16 * hi + lo;
-            if (n > p) break;
             cd t;
             if (n == 0) {
               t = term0;
@@ -279,9 +279,15 @@ __global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree
                 t = -(c * pw) * inv_n + qw;
               }
             }
-            t = gsum16(t);
-            if (lo == u) acc[hi] += t;
+            T[lo * FT_PAD + u] = t;
           }
+          lap_wave_sync();
+          if (u < nfill) {
+            cd sum = mk(0, 0);
+            for (int k = 0; k < 16; ++k) sum += T[u * FT_PAD + k];
+            acc[hi] += sum;
+          }
+          lap_wave_sync();
         }
       }
     }
