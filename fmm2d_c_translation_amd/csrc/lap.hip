@@ -302,8 +302,11 @@ __global__ void __launch_bounds__(WAVE) k_form(const int* work, int nwork, DTree
 
 // list 3: multipoles of finer far boxes at this box's points
 // (laprouts2d.c:193-344): pot += m_0 log|z| + sum m_n t^n, t = r/z;
-// grad = (m_0 - sum n m_n t^n) / z; hess = (sum n(n+1) m_n t^n - m_0) / z^2
-template <int PG>
+// grad = (m_0 - sum n m_n t^n) / z; hess = (sum n(n+1) m_n t^n - m_0) / z^2.
+// NC channels at a time share each box's z, 1/z, log|z| and powers t^n (the
+// log kernel's complex densities always come as channel pairs); each
+// channel's sums are formed exactly as alone.
+template <int PG, int NC>
 __global__ void __launch_bounds__(WAVE) k_eval_mp(const int* work, int nwork, DTree D,
                                                   const int2* rng, const double2* pts, int N,
                                                   int nd, int p, const double2* mp, double2* pot,
@@ -317,31 +320,49 @@ __global__ void __launch_bounds__(WAVE) k_eval_mp(const int* work, int nwork, DT
   const int k0 = D.o3[b], k1 = D.o3[b + 1];
   for (int i = rg.x + u; i < rg.y; i += 16) {
     const double2 x = pts[i];
-    for (int d = 0; d < nd; ++d) {
-      cd sp = mk(0, 0), sg = mk(0, 0), sh = mk(0, 0);
+    for (int d0 = 0; d0 < nd; d0 += NC) {
+      cd sp[NC], sg[NC], sh[NC];
+#pragma unroll
+      for (int e = 0; e < NC; ++e) sp[e] = sg[e] = sh[e] = mk(0, 0);
       for (int k = k0; k < k1; ++k) {
         const int s = D.l3[k];
         const double2 c = D.ctr[s];
         const double r = D.bs[D.level[s]];
         const cd z = mk(x.x - c.x, x.y - c.y);
         const cd zi = inv(z), t = r * zi;
-        const double2* m = mp + ((size_t)s * nd + d) * P;
-        const cd m0 = ldc(m);
-        cd s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0), tn = mk(1, 0);
+        const double lg = 0.5 * log_pos(norm2(z));
+        const double2* m = mp + ((size_t)s * nd + d0) * P;
+        cd m0[NC], s1[NC], s2[NC], s3[NC];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+          m0[e] = ldc(m + (size_t)e * P);
+          s1[e] = s2[e] = s3[e] = mk(0, 0);
+        }
+        cd tn = mk(1, 0);
         for (int n = 1; n <= p; ++n) {
           tn = tn * t;
-          const cd v = ldc(m + n) * tn;
-          s1 += v;
-          if (PG >= 2) s2 += v * (double)n;
-          if (PG >= 3) s3 += v * (double)(n * (n + 1));
+#pragma unroll
+          for (int e = 0; e < NC; ++e) {
+            const cd v = ldc(m + (size_t)e * P + n) * tn;
+            s1[e] += v;
+            if (PG >= 2) s2[e] += v * (double)n;
+            if (PG >= 3) s3[e] += v * (double)(n * (n + 1));
+          }
         }
-        sp += m0 * (0.5 * log_pos(norm2(z))) + s1;
-        if (PG >= 2) sg += (m0 - s2) * zi;
-        if (PG >= 3) sh += (s3 - m0) * (zi * zi);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+          sp[e] += m0[e] * lg + s1[e];
+          if (PG >= 2) sg[e] += (m0[e] - s2[e]) * zi;
+          if (PG >= 3) sh[e] += (s3[e] - m0[e]) * (zi * zi);
+        }
       }
-      addc(pot + (size_t)d * N + i, sp);
-      if (PG >= 2) addc(grad + (size_t)d * N + i, sg);
-      if (PG >= 3) addc(hess + (size_t)d * N + i, sh);
+#pragma unroll
+      for (int e = 0; e < NC; ++e) {
+        const int d = d0 + e;
+        addc(pot + (size_t)d * N + i, sp[e]);
+        if (PG >= 2) addc(grad + (size_t)d * N + i, sg[e]);
+        if (PG >= 3) addc(hess + (size_t)d * N + i, sh[e]);
+      }
     }
   }
 }
@@ -385,10 +406,14 @@ void launch_eval(bool mpk, int nw, hipStream_t st, const int* work, const DTree&
                  const int2* rng, const double2* pts, int N, int nd, int p, const double2* ex,
                  double2* pot, double2* grad, double2* hess) {
   const int nbx = (nw + 3) / 4;
-  if (mpk)
-    k_eval_mp<PG><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
-  else
+  if (mpk) {
+    if (nd % 2 == 0)
+      k_eval_mp<PG, 2><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
+    else
+      k_eval_mp<PG, 1><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
+  } else {
     k_eval_lo<PG><<<nbx, WAVE, 0, st>>>(work, nw, D, rng, pts, N, nd, p, ex, pot, grad, hess);
+  }
 }
 
 // ---------------------------------------------------------------------
