@@ -448,19 +448,23 @@ __device__ inline int list1_source(const int* boff, const int* bsrc, int g) {
   return bsrc[kb] + (g - boff[kb]);
 }
 
-template <bool C, bool Q, int PG, int MODE>
+// NC channels at a time (2 for the log kernel's channel pairs): each pair's
+// skip test, 1/z, -1/z^2 and log|z| serve every channel, whose sums are
+// formed exactly as alone.
+template <bool C, bool Q, int PG, int MODE, int NC>
 __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const double2* src,
                                               const double2* chg, const double2* dip, int NS,
                                               const int2* trng, const double2* tpts, int NT,
                                               int nd, double th2, double2* pot, double2* grad,
                                               double2* hess, LapOut out) {
-  // staging (x, charge, dipole) and the slice reduction share one buffer
-  __shared__ double2 lds[3 * LCAP];
+  // staging (x, charges, dipoles of NC channels) and the slice reduction
+  // share one buffer
+  __shared__ double2 lds[(1 + 2 * NC) * LCAP];
   double2* sx = lds;
-  double2* sc = lds + LCAP;
-  double2* sq = lds + 2 * LCAP;
+  double2* sc = lds + LCAP;             // [NC][LCAP]
+  double2* sq = lds + (1 + NC) * LCAP;  // [NC][LCAP]
   double2* red = lds;
-  static_assert(3 * WAVE <= 3 * LCAP, "reduction buffer");
+  static_assert(3 * NC * WAVE <= (1 + 2 * NC) * LCAP, "reduction buffer");
   __shared__ int boff[14], bsrc[14];
   const int b = work[blockIdx.x];
   const int lane = threadIdx.x;
@@ -472,15 +476,20 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
     const int t = lane / K, k = lane % K;
     const bool act = t < Tc;
     const double2 x = act ? tpts[t0 + t] : make_double2(0, 0);
-    for (int d = 0; d < nd; ++d) {
-      cd ap = mk(0, 0), ag = mk(0, 0), ah = mk(0, 0);
+    for (int d0 = 0; d0 < nd; d0 += NC) {
+      cd ap[NC], ag[NC], ah[NC];
+#pragma unroll
+      for (int e = 0; e < NC; ++e) ap[e] = ag[e] = ah[e] = mk(0, 0);
       for (int g0 = 0; g0 < S; g0 += LCAP) {
         const int cnt = min(LCAP, S - g0);
         for (int m = lane; m < cnt; m += WAVE) {
           const int si = list1_source(boff, bsrc, g0 + m);
           sx[m] = src[si];
-          if (C) sc[m] = chg[(size_t)d * NS + si];
-          if (Q) sq[m] = dip[(size_t)d * NS + si];
+#pragma unroll
+          for (int e = 0; e < NC; ++e) {
+            if (C) sc[e * LCAP + m] = chg[(size_t)(d0 + e) * NS + si];
+            if (Q) sq[e * LCAP + m] = dip[(size_t)(d0 + e) * NS + si];
+          }
         }
         __syncthreads();
         if (act) {
@@ -492,42 +501,54 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
             const double rinv = 1.0 / rr;
             const cd zi = mk(dx * rinv, -dy * rinv);
             const cd z2 = -(zi * zi);
-            if (C) {
-              const cd cc = mk(sc[m].x, sc[m].y);
-              ap += cc * (0.5 * log_pos(rr));
-              if (PG >= 2) ag += cc * zi;
-              if (PG >= 3) ah += cc * z2;
-            }
-            if (Q) {
-              const cd qq = mk(sq[m].x, sq[m].y);
-              ap += qq * zi;
-              if (PG >= 2) ag += qq * z2;
-              if (PG >= 3) ah += qq * ((z2 * zi) * -2.0);
+            const double lg = C ? 0.5 * log_pos(rr) : 0.0;
+            const cd z3 = (z2 * zi) * -2.0;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+              if (C) {
+                const cd cc = mk(sc[e * LCAP + m].x, sc[e * LCAP + m].y);
+                ap[e] += cc * lg;
+                if (PG >= 2) ag[e] += cc * zi;
+                if (PG >= 3) ah[e] += cc * z2;
+              }
+              if (Q) {
+                const cd qq = mk(sq[e * LCAP + m].x, sq[e * LCAP + m].y);
+                ap[e] += qq * zi;
+                if (PG >= 2) ag[e] += qq * z2;
+                if (PG >= 3) ah[e] += qq * z3;
+              }
             }
           }
         }
         __syncthreads();
       }
-      red[lane] = make_double2(ap.x, ap.y);
-      if (PG >= 2) red[WAVE + lane] = make_double2(ag.x, ag.y);
-      if (PG >= 3) red[2 * WAVE + lane] = make_double2(ah.x, ah.y);
+#pragma unroll
+      for (int e = 0; e < NC; ++e) {
+        red[(3 * e + 0) * WAVE + lane] = make_double2(ap[e].x, ap[e].y);
+        if (PG >= 2) red[(3 * e + 1) * WAVE + lane] = make_double2(ag[e].x, ag[e].y);
+        if (PG >= 3) red[(3 * e + 2) * WAVE + lane] = make_double2(ah[e].x, ah[e].y);
+      }
       __syncthreads();
       if (act && k == 0) {
-        cd P = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
-        for (int u = 0; u < K; ++u) {
-          const int l = t * K + u;
-          P += mk(red[l].x, red[l].y);
-          if (PG >= 2) G += mk(red[WAVE + l].x, red[WAVE + l].y);
-          if (PG >= 3) H += mk(red[2 * WAVE + l].x, red[2 * WAVE + l].y);
-        }
-        const size_t o = (size_t)d * NT + t0 + t;
-        if (out.pot) {
-          lap_final<PG>(out, d, t0 + t, ldc(pot + o) + P, PG >= 2 ? ldc(grad + o) + G : G,
-                        PG >= 3 ? ldc(hess + o) + H : H);
-        } else {
-          addc(pot + o, P);
-          if (PG >= 2) addc(grad + o, G);
-          if (PG >= 3) addc(hess + o, H);
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+          cd P = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
+          for (int u = 0; u < K; ++u) {
+            const int l = t * K + u;
+            P += ldc(&red[(3 * e + 0) * WAVE + l]);
+            if (PG >= 2) G += ldc(&red[(3 * e + 1) * WAVE + l]);
+            if (PG >= 3) H += ldc(&red[(3 * e + 2) * WAVE + l]);
+          }
+          const int d = d0 + e;
+          const size_t o = (size_t)d * NT + t0 + t;
+          if (out.pot) {
+            lap_final<PG>(out, d, t0 + t, ldc(pot + o) + P, PG >= 2 ? ldc(grad + o) + G : G,
+                          PG >= 3 ? ldc(hess + o) + H : H);
+          } else {
+            addc(pot + o, P);
+            if (PG >= 2) addc(grad + o, G);
+            if (PG >= 3) addc(hess + o, H);
+          }
         }
       }
       __syncthreads();
@@ -672,10 +693,17 @@ void launch_p2p_mode(int mode, int nbx, hipStream_t st, const int* work, const D
                      const double2* src, const double2* chg, const double2* dip, int NS,
                      const int2* trng, const double2* tpts, int NT, int nd, double th2,
                      double2* pot, double2* grad, double2* hess, const LapOut& out) {
-  if (mode == 0)
-    k_p2p<C, Q, PG, 0><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
-  else
-    k_p2p<C, Q, PG, 1><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
+#define FMM_P2P(MV, NCV)                                                                 \
+  k_p2p<C, Q, PG, MV, NCV><<<nbx, WAVE, 0, st>>>(work, D, src, chg, dip, NS, trng, tpts, NT, nd, \
+                                                 th2, pot, grad, hess, out)
+  if (nd % 2 == 0) {
+    if (mode == 0) FMM_P2P(0, 2);
+    else FMM_P2P(1, 2);
+  } else {
+    if (mode == 0) FMM_P2P(0, 1);
+    else FMM_P2P(1, 1);
+  }
+#undef FMM_P2P
 }
 
 template <bool C, bool Q>
