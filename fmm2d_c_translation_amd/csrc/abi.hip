@@ -873,13 +873,15 @@ int fmm2dhip_get_precision(void) { return fmm::thread_precision(); }
 
 int fmm2dhip_last_stats(double* out, int n) {
   const fmm::SolveStats& s = fmm::last_stats();
-  double v[24] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
+  double v[26] = {s.p2p_pairs_src, s.p2p_pairs_trg, s.m2l_pairs, s.m2l_flops, s.nboxes,
                   s.nlevels, s.pmax};
   for (int k = 0; k < 8; ++k) v[7 + k] = s.step_s[k];
   for (int k = 0; k < 8; ++k) v[15 + k] = s.host_s[k];
   v[23] = s.workspace_bytes;
-  for (int k = 0; k < n && k < 24; ++k) out[k] = v[k];
-  return 24;
+  v[24] = s.p2p_reads_src;
+  v[25] = s.p2p_reads_trg;
+  for (int k = 0; k < n && k < 26; ++k) out[k] = v[k];
+  return 26;
 }
 
 int fmm2dhip_device_info(char* name, int len) {

@@ -1,8 +1,9 @@
 // direct.hip -- O(N*M) direct sums on the GPU behind the reference's
 // c2d_direct* (c_translation/src/cauchykernels2d.c:31-442), l2d_direct*
 // (lapkernels2d.c:19-494), r2d_direct* (rlapkernels2d.c:16-491) and
-// h2d_direct* (helmkernels2d.c:21-520) entry points.  One thread per target; sources staged through LDS in tiles of
-// 256.  Outputs are incremented, as in the reference; each routine keeps
+// h2d_direct* (helmkernels2d.c:21-520) entry points.  One thread per target,
+// sources staged through LDS in tiles of 256; when there are few targets
+// the sources are also split into chunks across blocks (launch_chunked).  Outputs are incremented, as in the reference; each routine keeps
 // its own skip test (rr vs thresh^2 or |z| vs thresh, <= or <).
 #include "../../include/fmm2d_hip.h"
 #include "engine.h"
@@ -19,7 +20,12 @@ template <bool C, bool Q, int PG>
 __global__ void __launch_bounds__(TILE) k_c2d(const double2* src, int ns, const double2* chg,
                                               const double2* dip, const double2* trg, int nt,
                                               int nd, int mode, double th, double2* pot,
-                                              double2* grad, double2* hess) {
+                                              double2* grad, double2* hess, int len, size_t sp, size_t sg, size_t sh) {
+  // sources of this block's chunk; outputs of chunk y go to slice y
+  const int sb = blockIdx.y * len, se = min(ns, sb + len);
+  pot += blockIdx.y * sp;
+  if (PG >= 2) grad += blockIdx.y * sg;
+  if (PG >= 3) hess += blockIdx.y * sh;
   __shared__ double2 sx[TILE];
   __shared__ double2 sc[TILE];
   __shared__ double2 sq[TILE];
@@ -28,8 +34,8 @@ __global__ void __launch_bounds__(TILE) k_c2d(const double2* src, int ns, const 
   const double2 x = act ? trg[j] : make_double2(0, 0);
   for (int d = 0; d < nd; ++d) {
     cd P = mk(0, 0), G = mk(0, 0), H = mk(0, 0);
-    for (int s0 = 0; s0 < ns; s0 += TILE) {
-      int cnt = min(TILE, ns - s0);
+    for (int s0 = sb; s0 < se; s0 += TILE) {
+      int cnt = min(TILE, se - s0);
       if ((int)threadIdx.x < cnt) {
         sx[threadIdx.x] = src[s0 + threadIdx.x];
         if (C) sc[threadIdx.x] = chg[(size_t)(s0 + threadIdx.x) * nd + d];
@@ -79,7 +85,12 @@ __global__ void __launch_bounds__(TILE) k_l2d(const double2* src, int ns, const 
                                               const double2* dip, const double* dv,
                                               const double2* trg, int nt, int nd, bool le,
                                               double th2, double2* pot, double2* grad,
-                                              double2* hess) {
+                                              double2* hess, int len, size_t sp, size_t sg, size_t sh) {
+  // sources of this block's chunk; outputs of chunk y go to slice y
+  const int sb = blockIdx.y * len, se = min(ns, sb + len);
+  pot += blockIdx.y * sp;
+  if (PG >= 2) grad += blockIdx.y * sg;
+  if (PG >= 3) hess += blockIdx.y * sh;
   __shared__ double2 sx[TILE];
   __shared__ double2 sc[TILE];
   __shared__ double2 sq[TILE];
@@ -89,8 +100,8 @@ __global__ void __launch_bounds__(TILE) k_l2d(const double2* src, int ns, const 
   const double2 x = act ? trg[j] : make_double2(0, 0);
   for (int d = 0; d < nd; ++d) {
     cd P = mk(0, 0), Gx = mk(0, 0), Gy = mk(0, 0), Hxx = mk(0, 0), Hxy = mk(0, 0), Hyy = mk(0, 0);
-    for (int s0 = 0; s0 < ns; s0 += TILE) {
-      int cnt = min(TILE, ns - s0);
+    for (int s0 = sb; s0 < se; s0 += TILE) {
+      int cnt = min(TILE, se - s0);
       if ((int)threadIdx.x < cnt) {
         const size_t i = s0 + threadIdx.x;
         sx[threadIdx.x] = src[i];
@@ -161,7 +172,12 @@ __global__ void __launch_bounds__(TILE) k_h2d(cd zk, const double2* src, int ns,
                                               const double2* chg, const double2* dip,
                                               const double* dv, const double2* trg, int nt,
                                               int nd, bool le, double th, double2* pot,
-                                              double2* grad, double2* hess) {
+                                              double2* grad, double2* hess, int len, size_t sp, size_t sg, size_t sh) {
+  // sources of this block's chunk; outputs of chunk y go to slice y
+  const int sb = blockIdx.y * len, se = min(ns, sb + len);
+  pot += blockIdx.y * sp;
+  if (PG >= 2) grad += blockIdx.y * sg;
+  if (PG >= 3) hess += blockIdx.y * sh;
   __shared__ double2 sx[TILE];
   __shared__ double2 sc[TILE];
   __shared__ double2 sq[TILE];
@@ -173,8 +189,8 @@ __global__ void __launch_bounds__(TILE) k_h2d(cd zk, const double2* src, int ns,
   for (int d = 0; d < nd; ++d) {
     cd Pc = mk(0, 0), Pd = mk(0, 0), Gx = mk(0, 0), Gy = mk(0, 0);
     cd Hxx = mk(0, 0), Hxy = mk(0, 0), Hyy = mk(0, 0);
-    for (int s0 = 0; s0 < ns; s0 += TILE) {
-      int cnt = min(TILE, ns - s0);
+    for (int s0 = sb; s0 < se; s0 += TILE) {
+      int cnt = min(TILE, se - s0);
       if ((int)threadIdx.x < cnt) {
         const size_t i = s0 + threadIdx.x;
         sx[threadIdx.x] = src[i];
@@ -260,7 +276,12 @@ __global__ void __launch_bounds__(TILE) k_r2d(const double2* src, int ns, const 
                                               const double* dip, const double* dv,
                                               const double2* trg, int nt, int nd, bool le,
                                               double th2, double* pot, double* grad,
-                                              double* hess) {
+                                              double* hess, int len, size_t sp, size_t sg, size_t sh) {
+  // sources of this block's chunk; outputs of chunk y go to slice y
+  const int sb = blockIdx.y * len, se = min(ns, sb + len);
+  pot += blockIdx.y * sp;
+  if (PG >= 2) grad += blockIdx.y * sg;
+  if (PG >= 3) hess += blockIdx.y * sh;
   __shared__ double2 sx[TILE];
   __shared__ double sc[TILE];
   __shared__ double2 sd[TILE];  // (q v1, q v2)
@@ -269,8 +290,8 @@ __global__ void __launch_bounds__(TILE) k_r2d(const double2* src, int ns, const 
   const double2 x = act ? trg[j] : make_double2(0, 0);
   for (int d = 0; d < nd; ++d) {
     double P = 0, Gx = 0, Gy = 0, Hxx = 0, Hxy = 0, Hyy = 0;
-    for (int s0 = 0; s0 < ns; s0 += TILE) {
-      int cnt = min(TILE, ns - s0);
+    for (int s0 = sb; s0 < se; s0 += TILE) {
+      int cnt = min(TILE, se - s0);
       if ((int)threadIdx.x < cnt) {
         const size_t i = s0 + threadIdx.x;
         sx[threadIdx.x] = src[i];
@@ -332,6 +353,61 @@ __global__ void __launch_bounds__(TILE) k_r2d(const double2* src, int ns, const 
   }
 }
 
+// Sources are split into chunks over the grid's y dimension when the
+// targets alone give too few blocks (a few targets against many sources):
+// chunk y writes its partial sums to slice y of zeroed buffers, which
+// k_dsum then adds to the outputs in chunk order (deterministic).
+struct Chunks {
+  int n = 1, len = 0;
+};
+inline Chunks chunks_for(int ns, int nt) {
+  const int tb = (nt + TILE - 1) / TILE;
+  const int n = std::min((ns + 1023) / 1024, (2048 + tb - 1) / tb);
+  if (n <= 1) return {1, ns};
+  const int len = ((ns + n - 1) / n + TILE - 1) / TILE * TILE;
+  return {(ns + len - 1) / len, len};
+}
+__global__ void k_dsum(const double* part, int nchunk, size_t n, double* out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double a = out[e];
+  for (int y = 0; y < nchunk; ++y) a += part[y * n + e];
+  out[e] = a;
+}
+// Launches kern(grid, pot, grad, hess, len, slice strides) over the chunks
+// of ns sources; the outputs are no (pot), no*ncg (grad) and no*nch (hess)
+// elements of T (double2 or double), incremented.
+template <class T, class K>
+hipError_t launch_chunked(hipStream_t st, Scratch& S, int ns, int nt, size_t no, int ncg,
+                          int nch, int PG, T* po, T* go, T* ho, K&& kern) {
+  const Chunks ch = chunks_for(ns, nt);
+  const unsigned tb = (unsigned)((nt + TILE - 1) / TILE);
+  if (ch.n == 1) {
+    kern(dim3(tb, 1), po, go, ho, ns, size_t(0), size_t(0), size_t(0));
+    return hipGetLastError();
+  }
+  const size_t ng = PG >= 2 ? no * ncg : 0, nh = PG >= 3 ? no * nch : 0;
+  T* pp = S.get<T>(ch.n * no);
+  T* pg = ng ? S.get<T>(ch.n * ng) : nullptr;
+  T* ph = nh ? S.get<T>(ch.n * nh) : nullptr;
+  if (S.err) return hipErrorOutOfMemory;
+  hipError_t e = hipMemsetAsync(pp, 0, sizeof(T) * ch.n * no, st);
+  if (e == hipSuccess && pg) e = hipMemsetAsync(pg, 0, sizeof(T) * ch.n * ng, st);
+  if (e == hipSuccess && ph) e = hipMemsetAsync(ph, 0, sizeof(T) * ch.n * nh, st);
+  if (e != hipSuccess) return e;
+  kern(dim3(tb, (unsigned)ch.n), pp, pg, ph, ch.len, no, ng, nh);
+  constexpr int w = sizeof(T) / sizeof(double);
+  auto sum = [&](const T* part, size_t n, T* out) {
+    const size_t m = n * w;
+    k_dsum<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(reinterpret_cast<const double*>(part),
+                                                       ch.n, m, reinterpret_cast<double*>(out));
+  };
+  sum(pp, no, po);
+  if (pg) sum(pg, ng, go);
+  if (ph) sum(ph, nh, ho);
+  return hipGetLastError();
+}
+
 // host staging for the direct sums: copy in, run, copy the outputs back
 struct DirectIO {
   int nd, ns, nt, ncg, nch;  // gradient / Hessian components per output
@@ -369,8 +445,10 @@ int run_direct(const DirectIO& io, bool C, bool Q, int PG, Launch&& launch) {
   HIP_TRY(hipMemcpyAsync(po, io.pot, 16 * no, hipMemcpyHostToDevice, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(go, io.grad, 16 * no * io.ncg, hipMemcpyHostToDevice, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(ho, io.hess, 16 * no * io.nch, hipMemcpyHostToDevice, c.st));
-  launch(c.st, s, cc, qq, vv, t, po, go, ho);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_chunked(c.st, S, ns, nt, no, io.ncg, io.nch, PG, po, go, ho,
+                         [&](dim3 grid, double2* p, double2* g, double2* h, int len, size_t a,
+                             size_t b, size_t d) { launch(c.st, grid, s, cc, qq, vv, t, p, g, h,
+                                                          len, a, b, d); }));
   HIP_TRY(hipMemcpyAsync(io.pot, po, 16 * no, hipMemcpyDeviceToHost, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(io.grad, go, 16 * no * io.ncg, hipMemcpyDeviceToHost, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(io.hess, ho, 16 * no * io.nch, hipMemcpyDeviceToHost, c.st));
@@ -383,11 +461,12 @@ int l2d(int nd, const double* sources, int ns, const void* charge, const void* d
         const double* dipvec, const double* targ, int nt, void* pot, void* grad, void* hess,
         double th, bool le) {
   DirectIO io{nd, ns, nt, 2, 3, sources, targ, dipvec, charge, dipstr, pot, grad, hess};
-  return run_direct(io, C, Q, PG, [&](hipStream_t st, const double2* s, const double2* cc,
-                                      const double2* qq, const double* vv, const double2* t,
-                                      double2* po, double2* go, double2* ho) {
-    k_l2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, st>>>(s, ns, cc, qq, vv, t, nt, nd, le,
-                                                             th * th, po, go, ho);
+  return run_direct(io, C, Q, PG, [&](hipStream_t st, dim3 grid, const double2* s,
+                                      const double2* cc, const double2* qq, const double* vv,
+                                      const double2* t, double2* po, double2* go, double2* ho,
+                                      int len, size_t a, size_t b, size_t d) {
+    k_l2d<C, Q, PG><<<grid, TILE, 0, st>>>(s, ns, cc, qq, vv, t, nt, nd, le, th * th, po, go, ho,
+                                           len, a, b, d);
   });
 }
 
@@ -398,11 +477,12 @@ int h2d(int nd, const fmm2d_cplx* zk, const double* sources, int ns, const void*
   const double* z = reinterpret_cast<const double*>(zk);
   const cd k = mk(z[0], z[1]);
   DirectIO io{nd, ns, nt, 2, 3, sources, targ, dipvec, charge, dipstr, pot, grad, hess};
-  return run_direct(io, C, Q, PG, [&](hipStream_t st, const double2* s, const double2* cc,
-                                      const double2* qq, const double* vv, const double2* t,
-                                      double2* po, double2* go, double2* ho) {
-    k_h2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, st>>>(k, s, ns, cc, qq, vv, t, nt, nd, le,
-                                                             th, po, go, ho);
+  return run_direct(io, C, Q, PG, [&](hipStream_t st, dim3 grid, const double2* s,
+                                      const double2* cc, const double2* qq, const double* vv,
+                                      const double2* t, double2* po, double2* go, double2* ho,
+                                      int len, size_t a, size_t b, size_t d) {
+    k_h2d<C, Q, PG><<<grid, TILE, 0, st>>>(k, s, ns, cc, qq, vv, t, nt, nd, le, th, po, go, ho,
+                                           len, a, b, d);
   });
 }
 
@@ -437,9 +517,13 @@ int r2d(int nd, const double* sources, int ns, const double* charge, const doubl
   HIP_TRY(hipMemcpyAsync(po, pot, 8 * no, hipMemcpyHostToDevice, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(go, grad, 16 * no, hipMemcpyHostToDevice, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(ho, hess, 24 * no, hipMemcpyHostToDevice, c.st));
-  k_r2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, c.st>>>(s, ns, cc, qq, vv, t, nt, nd, le,
-                                                             th * th, po, go, ho);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_chunked(c.st, S, ns, nt, no, 2, 3, PG, po, go, ho,
+                         [&](dim3 grid, double* p, double* g, double* h, int len, size_t a,
+                             size_t b, size_t d) {
+                           k_r2d<C, Q, PG><<<grid, TILE, 0, c.st>>>(s, ns, cc, qq, vv, t, nt, nd,
+                                                                    le, th * th, p, g, h, len, a,
+                                                                    b, d);
+                         }));
   HIP_TRY(hipMemcpyAsync(pot, po, 8 * no, hipMemcpyDeviceToHost, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(grad, go, 16 * no, hipMemcpyDeviceToHost, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(hess, ho, 24 * no, hipMemcpyDeviceToHost, c.st));
@@ -472,9 +556,12 @@ int c2d(int nd, const double* sources, int ns, const void* charge, const void* d
   HIP_TRY(hipMemcpyAsync(po, pot, 16 * no, hipMemcpyHostToDevice, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(go, grad, 16 * no, hipMemcpyHostToDevice, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(ho, hess, 16 * no, hipMemcpyHostToDevice, c.st));
-  k_c2d<C, Q, PG><<<(nt + TILE - 1) / TILE, TILE, 0, c.st>>>(s, ns, cc, qq, t, nt, nd, mode, th,
-                                                             po, go, ho);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_chunked(c.st, S, ns, nt, no, 1, 1, PG, po, go, ho,
+                         [&](dim3 grid, double2* p, double2* g, double2* h, int len, size_t a,
+                             size_t b, size_t d) {
+                           k_c2d<C, Q, PG><<<grid, TILE, 0, c.st>>>(s, ns, cc, qq, t, nt, nd, mode,
+                                                                    th, p, g, h, len, a, b, d);
+                         }));
   HIP_TRY(hipMemcpyAsync(pot, po, 16 * no, hipMemcpyDeviceToHost, c.st));
   if (go) HIP_TRY(hipMemcpyAsync(grad, go, 16 * no, hipMemcpyDeviceToHost, c.st));
   if (ho) HIP_TRY(hipMemcpyAsync(hess, ho, 16 * no, hipMemcpyDeviceToHost, c.st));

@@ -450,7 +450,10 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
     unsigned long long bal = __ballot((m >> j) & 1);
     if (lane == 0 && bal) atomicAdd(&sc[j], __popcll(bal));
   }
-  double ps = 0, pt = 0;
+  // pairs[0], [1]: list-1 pairs at source and at target leaves; [2], [3]:
+  // list-1 source reads (sum over list 1 of the source counts) of the
+  // leaves holding sources and of those holding targets
+  double acc[4] = {0, 0, 0, 0};
   if (b < a.nb) {
     if ((m >> WL_M2L) & 1) atomicAdd(&sm2l[a.level[b]], (unsigned long long)a.cnt[b].y);
     if (a.cnt[b].x > 0) {
@@ -459,21 +462,21 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
         int2 r = a.srng[l1[k]];
         srcs += r.y - r.x;
       }
-      ps = srcs * (a.srng[b].y - a.srng[b].x);
-      pt = srcs * (a.trng[b].y - a.trng[b].x);
+      const int nsb = a.srng[b].y - a.srng[b].x, ntb = a.trng[b].y - a.trng[b].x;
+      acc[0] = srcs * nsb;
+      acc[1] = srcs * ntb;
+      acc[2] = nsb > 0 ? srcs : 0.0;
+      acc[3] = ntb > 0 ? srcs : 0.0;
     }
   }
-  for (int w = 32; w >= 1; w >>= 1) {
-    ps += __shfl_xor(ps, w);
-    pt += __shfl_xor(pt, w);
-  }
-  __shared__ double sp[2][WLB / 64];
-  if (lane == 0) {
-    sp[0][threadIdx.x >> 6] = ps;
-    sp[1][threadIdx.x >> 6] = pt;
+  __shared__ double sp[4][WLB / 64];
+  for (int q = 0; q < 4; ++q) {
+    double v = acc[q];
+    for (int w = 32; w >= 1; w >>= 1) v += __shfl_xor(v, w);
+    if (lane == 0) sp[q][threadIdx.x >> 6] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {  // one atomic per block and sum
+  if (threadIdx.x < 4) {  // one atomic per block and sum
     double t = 0;
     for (int w = 0; w < WLB / 64; ++w) t += sp[threadIdx.x][w];
     if (t != 0) atomicAdd(&pairs[threadIdx.x], t);
@@ -622,13 +625,16 @@ bool debug_enabled() {
   return on;
 }
 
-int sync_read(Ctx& c, const int* d, int n, int* h) {
+// Reads n ints back; the stream is then synchronised, so a sort whose
+// look-back wait gave up is known here, before its output is used as
+// indices: that is reported as IER_HIP.
+int sync_read(Ctx& c, const int* d, int n, int* h, const uint32_t* sort_err) {
   int* pin = static_cast<int*>(pinned(sizeof(int) * n, 4));
   if (!pin) return IER_ALLOC;
   HIP_TRY(hipMemcpyAsync(pin, d, sizeof(int) * n, hipMemcpyDeviceToHost, c.st));
   HIP_TRY(hipStreamSynchronize(c.st));
   std::memcpy(h, pin, sizeof(int) * n);
-  return IER_OK;
+  return sort_err && *sort_err ? IER_HIP : IER_OK;
 }
 
 }  // namespace
@@ -650,19 +656,24 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   int e = IER_OK;
   if (nbs + nbt) {
     std::vector<int> bm(nbs + nbt);
-    e = sync_read(c, d_depth, nbs + nbt, bm.data());
+    e = sync_read(c, d_depth, nbs + nbt, bm.data(), plan.sort_err);
     if (e) return e;
     for (int m : bm) Ls = std::max(Ls, m);
   } else {
     // nothing splits: the copies of the root box still have to land
     HIP_TRY(hipStreamSynchronize(c.st));
+    if (*plan.sort_err) return IER_HIP;
   }
   if (idivflag < 0 || idivflag > 2) Ls = -1;
   // exact while every box that splits lies above the sorted levels: a box
   // at level sorted_levels holding more than ndiv points shows up as a
   // split at that level or below, and then the keys need a full sort
   if (Ls >= sorted_levels && sorted_levels < kMaxDepth) return IER_RESORT;
-  if (Ls >= kMaxDepth) return IER_DEPTH;
+  // The keys resolve kMaxDepth levels: refinement stops there and the
+  // deepest leaves keep more than ndiv points, as the reference's leaves do
+  // when it stops at nlmax (pts_tree2d.c:191; nlmax = 51, cfmm2d.c:1120).
+  // Coincident points (more than ndiv of them) reach this.
+  if (Ls >= kMaxDepth) Ls = kMaxDepth - 1;
   const int L = Ls + 1;
 
   // ---- 1. adaptive refinement in one pass over the points (split_span):
@@ -727,7 +738,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   std::vector<int> nR(L + 1, 0);
   {
     std::vector<int> ro(L + 2);
-    e = sync_read(c, d_roff, L + 2, ro.data());
+    e = sync_read(c, d_roff, L + 2, ro.data(), plan.sort_err);
     if (e) return e;
     for (int l = 0; l <= L; ++l) nR[l] = ro[l + 1] - ro[l];
   }
@@ -774,7 +785,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     k_split_levels<<<1, 64, 0, c.st>>>(Rall, d_nu, L, d_loff);
     k_strip_level<<<nblk(std::max<long>(nc, 1), 256), 256, 0, c.st>>>(Rall, d_nu, L);
     std::vector<int> lo(L + 2);
-    e = sync_read(c, d_loff, L + 2, lo.data());
+    e = sync_read(c, d_loff, L + 2, lo.data(), plan.sort_err);
     if (e) return e;
     for (int l = 0; l <= L; ++l) nR[l] = lo[l + 1] - lo[l];
   }
@@ -825,7 +836,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   HIP_TRY(hipcub::DeviceScan::ExclusiveScan(tmp, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
   int tot[4];
   HIP_TRY(hipMemcpyAsync(tot4, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
-  e = sync_read(c, tot4, 4, tot);
+  e = sync_read(c, tot4, 4, tot, plan.sort_err);
   if (e) return e;
   lo.l1 = S.get<int>(std::max(tot[0], 1));
   lo.l2 = S.get<int>(std::max(tot[1], 1));
@@ -839,8 +850,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   const int W = L + 2;
   const int ninfo = 2 * WL_N + 1;
   // small results read back in one copy: info, level offsets, m2l pairs
-  // (as 2 ints each), list-1 pairs (2 doubles as 4 ints)
-  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 4;
+  // (as 2 ints each), list-1 pairs and source reads (4 doubles as 8 ints)
+  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 8;
   int* small = S.get<int>(nsmall + 2);
   int* bcnt = S.get<int>((size_t)WL_N * nbk);
   if (S.err) return S.err;
@@ -860,7 +871,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   k_wl_kids<<<nblk(nb, 256), 256, 0, c.st>>>(info, lev, W, child, work);
   HIP_TRY(hipGetLastError());
   std::vector<int> hs(nsmall + 2);
-  e = sync_read(c, small, nsmall + 2, hs.data());
+  e = sync_read(c, small, nsmall + 2, hs.data(), plan.sort_err);
   if (e) return e;
 
   // ---- host view
@@ -896,6 +907,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     const int kb = kM2lKeyBits;
     e = rsort::radix_sort(c, S, k0, k1, v0, work + plan.m2l.off, n, 0, kb, plan.sort_err);
     if (e) return e;
+    rsort::fallback_unsorted(c.st, plan.sort_err, v0, work + plan.m2l.off, n);
   }
   // list-3 and leaf work in decreasing cost within each level (the four
   // lists are adjacent in the work buffer)
@@ -916,6 +928,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                                                   k0, v0);
       e = rsort::radix_sort(c, S, k0, k1, v0, work + first, n, 0, 15, plan.sort_err);
       if (e) return e;
+      rsort::fallback_unsorted(c.st, plan.sort_err, v0, work + first, n);
     }
   }
   const int m2l_at = (ninfo + WL_N * W + 1) & ~1;
@@ -949,6 +962,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   SolveStats& st = last_stats();
   st.p2p_pairs_src = hp[0];
   st.p2p_pairs_trg = hp[1];
+  st.p2p_reads_src = hp[2];
+  st.p2p_reads_trg = hp[3];
   st.nboxes = nb;
   st.nlevels = L;
   double t3 = wall();

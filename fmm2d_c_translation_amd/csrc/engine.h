@@ -164,6 +164,9 @@ inline bool mixed_for(double eps) { return thread_precision() == PREC_MIXED && e
 // Work counts of the last solve on this thread (for roofline reporting).
 struct SolveStats {
   double p2p_pairs_src = 0, p2p_pairs_trg = 0;  // sum over targets of list-1 sources
+  // sum over the leaves holding sources (targets) of their list-1 source
+  // counts: the source reads of the near field's algorithmic traffic
+  double p2p_reads_src = 0, p2p_reads_trg = 0;
   double m2l_pairs = 0, m2l_flops = 0;          // list-2 pairs, 8 * ncoef^2 per pair
   double nboxes = 0, nlevels = 0, pmax = 0;
   double step_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};

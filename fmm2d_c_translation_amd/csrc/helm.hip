@@ -1290,8 +1290,10 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   }
   if (pmax > 2000) return IER_P_TOO_LARGE;
   // the diagonal (high-frequency) form of boxes above 16 wavelengths is not built
+  // (levels 0 and 1 form no expansions: the reference's level-1 multipoles
+  // feed nothing, so a root box of any size is fine)
   double dlam = 1.0 / (a.zk.real() / (2 * kPi));
-  for (int l = 1; l <= L; ++l)
+  for (int l = 2; l <= L; ++l)
     if (bs[l] / dlam > 16.0 && gate[l]) return IER_HIGHFREQ;
   const double2* chg = a.ifcharge ? a.chg : nullptr;
   const double2* dip = a.ifdipole ? a.dip : nullptr;

@@ -260,6 +260,20 @@ int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* v
   return hipGetLastError() == hipSuccess ? IER_OK : IER_HIP;
 }
 
+// After a sort whose values are used as indices before the host can read
+// *err: when a wait gave up, the unsorted values (vin is left intact) replace
+// the output, so consumers still see valid indices in a valid (unsorted)
+// order and the failure is reported when the host reads *err.
+__global__ void k_rs_fallback(const uint32_t* err, const int* vin, int* vout, int n) {
+  if (!ld_relaxed(err)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    vout[i] = vin[i];
+}
+inline void fallback_unsorted(hipStream_t st, const uint32_t* err, const int* vin, int* vout,
+                              int n) {
+  if (n > 0) k_rs_fallback<<<std::min(64, (n + 255) / 256), 256, 0, st>>>(err, vin, vout, n);
+}
+
 }  // namespace
 }  // namespace rsort
 }  // namespace fmm

@@ -115,16 +115,22 @@ class Server {
     return t;
   }
 
+  // A ticket is forgotten once a wait has seen it done, so a long-running
+  // server holds state only for requests still queued, running, or done
+  // and not yet waited for.
   int wait(long long ticket, double timeout_s) {
     std::unique_lock<std::mutex> lk(mu_);
-    auto it = state_.find(ticket);
-    if (it == state_.end()) return -1;
-    auto done = [&] { return state_[ticket] == 1; };
+    if (state_.find(ticket) == state_.end()) return -1;
+    auto done = [&] {
+      auto it = state_.find(ticket);
+      return it == state_.end() || it->second == 1;
+    };
     if (timeout_s < 0) {
       cv_done_.wait(lk, done);
     } else if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
       return 1;
     }
+    state_.erase(ticket);
     return 0;
   }
 
@@ -138,11 +144,12 @@ class Server {
       return lat[std::min(i, lat.size() - 1)];
     };
     const double span = t_last_ > t_first_ ? t_last_ - t_first_ : 0.0;
-    double v[10] = {(double)lat.size(), (double)failed_, points_, span,
+    double v[11] = {(double)lat.size(), (double)failed_, points_, span,
                     span > 0 ? points_ / span / 1e6 : 0.0, q(0.50), q(0.99),
-                    lat.empty() ? 0.0 : lat.back(), peak_bytes_, budget_};
-    for (int k = 0; k < n && k < 10; ++k) out[k] = v[k];
-    return 10;
+                    lat.empty() ? 0.0 : lat.back(), peak_bytes_, budget_,
+                    (double)state_.size()};
+    for (int k = 0; k < n && k < 11; ++k) out[k] = v[k];
+    return 11;
   }
 
  private:
@@ -155,7 +162,7 @@ class Server {
 
   double ratio(int kernel) {
     std::lock_guard<std::mutex> g(ratio_mu_);
-    return kernel >= 0 && kernel < 3 ? ratio_[kernel] : 1.0;
+    return kernel >= 0 && kernel < kKernels ? ratio_[kernel] : 1.0;
   }
 
   // workspace a worker holding cap ends up with after a solve that takes
@@ -198,7 +205,7 @@ class Server {
           const double cap = (double)workspace_capacity();
           const double used = last_stats().workspace_bytes;
           const double est0 = estimate_bytes(*it.p);
-          if (est0 > 0 && it.p->kernel >= 0 && it.p->kernel < 3) {
+          if (est0 > 0 && it.p->kernel >= 0 && it.p->kernel < kKernels) {
             std::lock_guard<std::mutex> g(ratio_mu_);
             ratio_[it.p->kernel] = std::max(ratio_[it.p->kernel], 1.1 * used / est0);
           }
@@ -237,7 +244,7 @@ class Server {
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::deque<Item> queue_;
-  std::unordered_map<long long, int> state_;  // 0 queued or running, 1 done
+  std::unordered_map<long long, int> state_;  // 0 queued or running, 1 done, not waited for
   std::vector<std::thread> threads_;
   std::vector<double> cap_, hold_;
   std::vector<char> busy_;
@@ -249,7 +256,10 @@ class Server {
   double points_ = 0, peak_bytes_ = 0, t_first_ = -1, t_last_ = 0;
   long long failed_ = 0;
   std::mutex ratio_mu_;
-  double ratio_[3] = {1.0, 1.0, 1.0};
+  // per kernel (FMM2DHIP_CAUCHY .. FMM2DHIP_REAL): largest measured /
+  // estimated workspace so far
+  static constexpr int kKernels = FMM2DHIP_REAL + 1;
+  double ratio_[kKernels] = {1.0, 1.0, 1.0, 1.0};
 };
 
 namespace {

@@ -106,7 +106,7 @@ def batch(lib, problems, nworkers=8):
 
 
 STAT_KEYS = ("done", "failed", "points", "span_s", "mpts_s", "p50_s", "p99_s", "max_s",
-             "peak_workspace_bytes", "budget_bytes")
+             "peak_workspace_bytes", "budget_bytes", "tracked_tickets")
 
 
 class Server:

@@ -281,7 +281,9 @@ int fmm2dhip_test_devtree(const double *src, int ns, const double *targ, int nt,
  * keys and sort, adaptive refinement, level restriction, boxes + lists +
  * work lists, leaf order and gather, operator setup, whole call including
  * the end-of-solve synchronisation), [23] device workspace bytes the solve
- * took.  Returns the number of values available (24). */
+ * took, [24] and [25] the near field's source reads: over the leaves
+ * holding sources ([24]) and targets ([25]), the sum of the source counts
+ * of their list 1.  Returns the number of values available (26). */
 int fmm2dhip_last_stats(double *out, int n);
 
 /* frees the calling thread's device workspace on the current device (no
@@ -363,11 +365,13 @@ int fmm2dhip_batch(int n, fmm2dhip_problem *probs, int nworkers);
  * queue would not fit).  Work of concurrent requests overlaps on the
  * device.  submit returns a ticket >= 0, or -1 when the request alone
  * exceeds the budget or the server is shutting down; wait returns 0 when
- * the ticket is done, 1 on time-out, -1 for an unknown ticket.  stats
- * writes up to n of: requests done, failed, points solved, seconds from the
- * first submission to the last completion, Mpts/s over that span, p50, p99
- * and max latency (s), peak workspace bytes held, budget; returns the count
- * available.  destroy finishes admitted and queued work, then stops the
+ * the ticket is done, 1 on time-out, -1 for an unknown ticket; a ticket
+ * is forgotten once a wait has returned 0 for it (a later wait returns -1).
+ * stats writes up to n of: requests done, failed, points solved, seconds
+ * from the first submission to the last completion, Mpts/s over that span,
+ * p50, p99 and max latency (s), peak workspace bytes held, budget, tickets
+ * still tracked (queued, running, or done and not yet waited for); returns
+ * the count available.  destroy finishes admitted and queued work, then stops the
  * workers and frees their workspaces. */
 typedef struct fmm2dhip_server fmm2dhip_server;
 fmm2dhip_server *fmm2dhip_server_create(int nworkers, double hbm_budget_bytes);

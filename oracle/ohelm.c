@@ -234,25 +234,29 @@ static void translate(int kind, int nd, cplx zk, double r1, const double *c1, co
         a *= zmul;
         b = -b * zinv;
     }
-    /* powers r1^k, r2^k, k = 0..2P */
-    double *p1w = malloc(sizeof(double) * (2 * P + 2)), *p2w = malloc(sizeof(double) * (2 * P + 2));
-    p1w[0] = p2w[0] = 1.0;
-    for (int k = 1; k <= 2 * P + 1; k++) p1w[k] = p1w[k - 1] * r1, p2w[k] = p2w[k - 1] * r2;
-#define R1(k) ((k) >= 0 ? p1w[(k)] : 1.0 / p1w[-(k)])
+    /* The scale factors F, written with non-negative exponents only (the
+     * orders satisfy |i| <= |j| + |i-j| and its permutations), so that no
+     * power of a tiny scale (deep boxes: rscale ~ |zk| bs) is inverted:
+     *   M2M    r1^(|j|+|m|-|i|) (r1/r2)^|i|
+     *   L2L    r1^(|i|+|m|-|j|) (r2/r1)^|i|
+     *   M2L    r1^(|i|+|j|-|m|) (r2/r1)^|i|       (m = i - j)          */
+    double *pr = malloc(sizeof(double) * (2 * P + 2)), *pq = malloc(sizeof(double) * (2 * P + 2));
+    double q = kind == T_MPMP ? r1 / r2 : r2 / r1;
+    pr[0] = pq[0] = 1.0;
+    for (int k = 1; k <= 2 * P + 1; k++) pr[k] = pr[k - 1] * r1, pq[k] = pq[k - 1] * q;
     for (int i = -p2; i <= p2; i++) {
         int ai = abs(i);
         for (int j = -p1; j <= p1; j++) {
             int aj = abs(j), am = abs(i - j);
             double F;
-            if (kind == T_MPMP) F = R1(aj + am - ai) * (R1(ai) / p2w[ai]);
-            else if (kind == T_LOCLOC) F = p2w[ai] * R1(am - aj);
-            else F = R1(aj - am) * p2w[ai];
+            if (kind == T_MPMP) F = pr[aj + am - ai] * pq[ai];
+            else if (kind == T_LOCLOC) F = pr[ai + am - aj] * pq[ai];
+            else F = pr[ai + aj - am] * pq[ai];
             cplx t = B[P + i - j] * F;
             for (int d = 0; d < nd; d++) EX(out, d, i, p2, nd) += EX(in, d, j, p1, nd) * t;
         }
     }
-#undef R1
-    free(v); free(B); free(p1w); free(p2w);
+    free(v); free(B); free(pr); free(pq);
 }
 
 /* ------------------------------------------------------------------ */
@@ -281,9 +285,10 @@ static int helm_main(const otree *t, const olists *L, const helm_job *J, const i
     const double *bs = t->boxsize;
     int pmax = 0;
     for (int l = 0; l <= nl; l++) pmax = nterms[l] > pmax ? nterms[l] : pmax;
-    /* boxes of more than 16 wavelengths use the diagonal form in the reference */
+    /* boxes of more than 16 wavelengths use the diagonal form in the
+     * reference; levels 0 and 1 form no expansion that any step reads */
     double dlam = 1.0 / (creal(zk) / (2 * PI));
-    for (int l = 1; l <= nl; l++)
+    for (int l = 2; l <= nl; l++)
         if (bs[l] / dlam > 16.0 && zi * bs[l] < 40.0) return 105;
     size_t per = (size_t)nd * (2 * pmax + 1);
     cplx *mp = calloc(per * (nb + 1), sizeof(cplx)), *lo = calloc(per * (nb + 1), sizeof(cplx));

@@ -191,3 +191,48 @@ def test_oracle_hfmm_vs_direct(orc, eps):
                           C.c_double(1e-14))
     assert rel_l2(r.pot, po) < 10 * eps
     assert rel_l2(r.grad.ravel(order="F"), gr) < 10 * eps
+
+
+def test_oracle_hfmm_deep_coincident_tree(orc, ref):
+    """More than ndiv coincident sources refine one box to the level cap
+    (51 levels): box scales near |zk| bs0 2^-51 must not overflow the
+    oracle's scale powers.  Against the compiled reference."""
+    if ref is None:
+        pytest.skip("reference library not built here")
+    rng = np.random.default_rng(53)
+    ndiv, n = 10, 2000
+    src = rng.random((2, n))
+    src[:, 100:100 + ndiv + 5] = np.array([[0.3141], [0.2718]])
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dp = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    th = rng.random(n) * 2 * np.pi
+    dv = np.stack([np.cos(th), np.sin(th)])[None]
+    kw = dict(eps=1e-9, zk=20.0, sources=src, charges=ch, dipstr=dp, dipvec=dv, pg=2,
+              targets=rng.random((2, 300)), pgt=2, ndiv=ndiv)
+    a = abi.hfmm2d_call(orc, "orc_hfmm2d_ndiv_", **kw)
+    b = abi.hfmm2d_call(ref, "hfmm2d_ndiv_", **kw)
+    assert a.ier == 0 and b.ier == 0
+    for k in ("pot", "grad", "targ_pot", "targ_grad"):
+        assert rel_l2(a[k], b[k]) < 1e-12, k
+
+
+def test_oracle_hfmm_root_box_above_16_wavelengths(orc, ref):
+    """zk = 250 on the unit square: the root and level-1 boxes exceed 16
+    wavelengths, where the reference switches to the diagonal form; no step
+    reads those levels' expansions, so the oracle's low-frequency
+    translations give the reference's result."""
+    if ref is None:
+        pytest.skip("reference library not built here")
+    rng = np.random.default_rng(59)
+    n, nt = 8000, 1000
+    src = rng.random((2, n))
+    src[:, 0] = (0.0, 0.0)
+    src[:, 1] = (1.0, 1.0)
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    kw = dict(eps=1e-6, zk=250.0, sources=src, charges=ch, pg=2, targets=rng.random((2, nt)),
+              pgt=2)
+    a = abi.hfmm2d_call(orc, "orc_hfmm2d_", **kw)
+    b = abi.hfmm2d_call(ref, "hfmm2d_", **kw)
+    assert a.ier == 0 and b.ier == 0
+    for k in ("pot", "grad", "targ_pot", "targ_grad"):
+        assert rel_l2(a[k], b[k]) < 1e-11, k
