@@ -814,6 +814,27 @@ int fmm2dhip_test_devtree(const double* src, int ns, const double* targ, int nt,
   }
 }
 
+int fmm2dhip_test_fft(int n, int nrows, int sign, const double* in, double* out) {
+  try {
+    if (n < 1 || nrows < 1 || (sign != 1 && sign != -1) || fmm::next235(n) != n)
+      return fmm::IER_ARGS;
+    fmm::Ctx c;
+    int e = fmm::get_ctx(c);
+    if (e) return e;
+    fmm::Scratch S(c.st);
+    const size_t m = (size_t)n * nrows;
+    double2* d = S.get<double2>(m);
+    if (S.err) return S.err;
+    HIP_TRY(hipMemcpyAsync(d, in, sizeof(double2) * m, hipMemcpyHostToDevice, c.st));
+    if ((e = fmm::fft_rows_device(c.st, S, d, nrows, n, sign))) return e;
+    HIP_TRY(hipMemcpyAsync(out, d, sizeof(double2) * m, hipMemcpyDeviceToHost, c.st));
+    HIP_TRY(hipStreamSynchronize(c.st));
+    return fmm::IER_OK;
+  } catch (...) {
+    return fmm::IER_ALLOC;
+  }
+}
+
 int fmm2dhip_test_sort(const void* keys, int n, int key_bytes, int begin, int end,
                        void* keys_out, int* vals_out) {
   try {

@@ -253,6 +253,32 @@ struct HelmArgs {
 int helm_run(Ctx& c, Scratch& S, const SortedPoints& P, const Plan& plan, const HelmArgs& a);
 int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier);
 
+// Diagonal-form (high-frequency) translations of one level (wideband.hip):
+// Y[work[w]] += sum over slots s with map[w][s] >= 0 of the translation of
+// X[map[w][s]], through signatures of length nsig.  kind 0: M2M from the
+// children (slot = quadrant) into their parents at box size bs; 1: M2L at
+// box size bs (49 list-2 offsets); 2: L2L from parents of box size bs into
+// the children (slot = the child's quadrant).  Input boxes are the id range
+// in_first .. in_first + in_count - 1 (one level).  Expansions as
+// everywhere: [box][channel][order + p], scale 1.
+struct HfJob {
+  int kind = 0;
+  int pin = 0, pout = 0, nsig = 0;
+  const int* work = nullptr;
+  int nwork = 0;
+  const int* map = nullptr;
+  int nslot = 0;
+  int in_first = 0, in_count = 0;
+  double bs = 0;
+};
+int hf_translate(hipStream_t st, Scratch& S, std::complex<double> zk, const HfJob& j,
+                 const double2* X, size_t xs, double2* Y, size_t ys, int nd);
+// the reference's FFT length rule (next235.c:10-32)
+int next235(double base);
+// unnormalised DFTs of nrows rows of n (2^a 3^b 5^c) values in place; sign
+// -1 forward (FFTPACK zfftf), +1 backward (zfftb)
+int fft_rows_device(hipStream_t st, Scratch& S, double2* rows, int nrows, int n, int sign);
+
 int l2dterms(double eps);
 int ndiv_rule(double eps, int ns, int nt);
 

@@ -1289,12 +1289,25 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     if (gate[l] && l >= 2) pmax = std::max(pmax, lev[l].p);
   }
   if (pmax > 2000) return IER_P_TOO_LARGE;
-  // the diagonal (high-frequency) form of boxes above 16 wavelengths is not built
-  // (levels 0 and 1 form no expansions: the reference's level-1 multipoles
-  // feed nothing, so a root box of any size is fine)
-  double dlam = 1.0 / (a.zk.real() / (2 * kPi));
+  // Levels whose boxes exceed 16 wavelengths translate in the diagonal form
+  // (hfmm2d.c:946-960: M2M into them, M2L at them and L2L out of them;
+  // levels 0 and 1 form no expansion that any step reads).
+  std::vector<char> hf(L + 2, 0);
   for (int l = 2; l <= L; ++l)
-    if (bs[l] / dlam > 16.0 && gate[l]) return IER_HIGHFREQ;
+    hf[l] = gate[l] && bs[l] * a.zk.real() / (2 * kPi) > 16.0;
+  // the largest order the per-point steps meet: formation, list 4 and local
+  // evaluation at their own level, list 3 at the level below (the orders of
+  // the diagonal-form levels are far above those of the leaves)
+  int pmax_pt = 0;
+  for (int l = 0; l <= L; ++l) {
+    if (!gate[l] || l < 2) continue;
+    if (plan.formmp.at(l).n || plan.list4.at(l).n || plan.leaves_s.at(l).n ||
+        plan.leaves_t.at(l).n)
+      pmax_pt = std::max(pmax_pt, lev[l].p);
+  }
+  for (int l = 0; l < L; ++l)
+    if (gate[l + 1] && l + 1 >= 2 && (plan.list3_s.at(l).n || plan.list3_t.at(l).n))
+      pmax_pt = std::max(pmax_pt, lev[l + 1].p);
   const double2* chg = a.ifcharge ? a.chg : nullptr;
   const double2* dip = a.ifdipole ? a.dip : nullptr;
   if (!chg && !dip) return IER_OK;
@@ -1315,7 +1328,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   int vmax = 0;
   for (int l = 0; l <= L; ++l) {
     if (!gate[l]) continue;
-    if (l >= 2 && l <= L - 1) {
+    if (l >= 2 && l <= L - 1 && !hf[l]) {
       double h = bs[l + 1] / 2;
       size_t sz1 = (size_t)(2 * lev[l + 1].p + 1) * (2 * lev[l].p + 1);
       size_t base_m = toff, base_l = toff + 4 * sz1;
@@ -1331,7 +1344,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       tab[l].l2l[0] = base_l;
       vmax = std::max(vmax, lev[l].p + lev[l + 1].p + 4);
     }
-    if (l >= 2) {
+    if (l >= 2 && !hf[l]) {
       size_t sz = (size_t)(2 * lev[l].p + 1) * (2 * lev[l].p + 1);
       size_t base = toff;
       toff += 49 * sz;  // near slots stay unused (no list-2 pair maps to them)
@@ -1366,12 +1379,17 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // formation: points staged CH at a time, Z (complex) and V (J real for a
   // real wavenumber, else complex) for orders 0..p+2; 16 points keep the
   // footprint near 12-16 KB so several waves share a CU
-  const int CH = 16;
+  // (fewer points per stage for the high orders of leaves above a few
+  // wavelengths, so that a block stays within 64 KB of LDS)
   const bool rz = zk.y == 0.0 && zk.x > 0.0;
-  const size_t shm = (size_t)(pmax + 3) * CH * (sizeof(double2) + sizeof(double2)) +
-                     (size_t)6 * CH * sizeof(double2);
-  const size_t shm_mp = (size_t)(pmax + 3) * CH * (sizeof(double2) + (rz ? sizeof(double) : sizeof(double2))) +
-                        (size_t)6 * CH * sizeof(double2);
+  auto shm_for = [&](int ch, size_t vsz) {
+    return (size_t)(pmax_pt + 3) * ch * (sizeof(double2) + vsz) + (size_t)6 * ch * sizeof(double2);
+  };
+  int CH = 16;
+  while (CH > 1 && shm_for(CH, sizeof(double2)) > 65536) CH /= 2;
+  if (shm_for(CH, sizeof(double2)) > 65536) return IER_P_TOO_LARGE;
+  const size_t shm = shm_for(CH, sizeof(double2));
+  const size_t shm_mp = shm_for(CH, rz ? sizeof(double) : sizeof(double2));
 
   // levels with gate[l] false (complex zk, boxes far above the decay length)
   // form no expansions; gate[l] only switches on as boxes shrink, so the
@@ -1404,14 +1422,15 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       double np = plan.m2l_pairs[l];
       double nc = 2 * lev[l].p + 1;
       st.m2l_pairs += np;
-      st.m2l_flops += np * 8.0 * nc * nc;
+      // a diagonal-form pair is one complex multiply-add per signature point
+      st.m2l_flops += hf[l] ? np * 8.0 * next235(4.0 * lev[l].p + 1) : np * 8.0 * nc * nc;
     }
     st.pmax = pmax;
   }
   tm->mark(c.st);
   // step 1: leaf multipoles
   // all-lanes formation for a real wavenumber up to order 63
-  const bool fg = rz && pmax <= 63;
+  const bool fg = rz && pmax_pt <= 63;
   if (s_form.n) {
     if (fg)
       k_hform_g<true, 1><<<nblk(s_form.n, 4), WAVE, 0, c.st>>>(wptr(s_form), s_form.n, D, dlev,
@@ -1438,6 +1457,23 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2m.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* map = map_m2m + (size_t)(sp.off - plan.m2m.off) * 4;
+    if (hf[l]) {  // hfmm2d.c:993-1051
+      HfJob j;
+      j.kind = 0;
+      j.pin = lev[l + 1].p;
+      j.pout = lev[l].p;
+      j.nsig = next235(2.0 * (lev[l].p + lev[l + 1].p) + 1);
+      j.work = W + sp.off;
+      j.nwork = sp.n;
+      j.map = map;
+      j.nslot = 4;
+      j.in_first = plan.loff[l + 1];
+      j.in_count = plan.loff[l + 2] - plan.loff[l + 1];
+      j.bs = bs[l];
+      const int e = hf_translate(c.st, S, a.zk, j, mp, per, mp, per, nd);
+      if (e) return e;
+      continue;
+    }
     HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 4, dT + tab[l].m2m[0], 2 * lev[l + 1].p + 1,
                              2 * lev[l].p + 1, mp, per, mp, per, nd, 0.0));
   }
@@ -1469,6 +1505,22 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
+    if (hf[l]) {  // hfmm2d.c:1057-1160, the boxlam > 16 branch
+      HfJob j;
+      j.kind = 1;
+      j.pin = j.pout = lev[l].p;
+      j.nsig = next235(4.0 * lev[l].p + 1);
+      j.work = W + sp.off;
+      j.nwork = sp.n;
+      j.map = ml;
+      j.nslot = 49;
+      j.in_first = plan.loff[l];
+      j.in_count = plan.loff[l + 1] - plan.loff[l];
+      j.bs = bs[l];
+      const int e = hf_translate(c.st, S, a.zk, j, mp, per, lo, per, nd);
+      if (e) return e;
+      continue;
+    }
     m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0,
                           dT32 ? dT32 + tab[l].m2l[0] : nullptr});
   }
@@ -1479,6 +1531,23 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     const Span sp = plan.l2l_kids.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* map = map_l2l + (size_t)(sp.off - plan.l2l_kids.off) * 4;
+    if (hf[l]) {  // hfmm2d.c:1165-1254, the boxlam > 16 branch
+      HfJob j;
+      j.kind = 2;
+      j.pin = lev[l].p;
+      j.pout = lev[l + 1].p;
+      j.nsig = next235(2.0 * (lev[l].p + lev[l + 1].p) + 1);
+      j.work = W + sp.off;
+      j.nwork = sp.n;
+      j.map = map;
+      j.nslot = 4;
+      j.in_first = plan.loff[l];
+      j.in_count = plan.loff[l + 1] - plan.loff[l];
+      j.bs = bs[l];
+      const int e = hf_translate(c.st, S, a.zk, j, lo, per, lo, per, nd);
+      if (e) return e;
+      continue;
+    }
     HIP_TRY(launch_xlate(c.st, S, W + sp.off, sp.n, map, 4, 1, dT + tab[l].l2l[0], 2 * lev[l].p + 1,
                              2 * lev[l + 1].p + 1, lo, per, lo, per, nd, 0.0));
   }
@@ -1514,11 +1583,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.l3.n)
-      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, pmax, s.po, s.gr, s.he);
+      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, pmax_pt, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.lv.n)
-      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, 0.0, pmax, s.po, s.gr, s.he);
+      launch_heval<false>(s.pg, s.lv.n, c.st, wptr(s.lv), D, dlev, zk, s.rng, s.pts, s.N, nd, per, lo, 0.0, pmax_pt, s.po, s.gr, s.he);
   tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;

@@ -11,10 +11,12 @@
  *   - *ier: 0 ok, 4 device allocation failure, 13 Helmholtz expansion order
  *     above 10000 (reference meanings), and this engine's own codes
  *       101 HIP runtime error          102 unsupported argument
- *       103 tree deeper than 29 levels 104 no usable device
- *       105 Helmholtz boxes above 16 wavelengths (high-frequency diagonal
- *           form not built)            106 expansion order beyond the
- *           kernels' limits (Laplace 63, Helmholtz 2000).
+ *       104 no usable device
+ *       106 expansion order beyond the kernels' limits (Laplace 63,
+ *           Helmholtz 2000, and the order of a Helmholtz leaf whose
+ *           formation tile would exceed 64 KB of LDS).  (103 and 105 are
+ *           no longer returned: refinement stops at the keys' 29 levels, and
+ *           boxes above 16 wavelengths take the diagonal form.)
  *     The library never aborts or prints.
  * Calls are synchronous and thread-safe (one HIP stream per calling thread).
  */
@@ -258,6 +260,13 @@ int fmm2dhip_test_tree(const double *src, int ns, const double *targ, int nt, in
  * 102 for bad arguments, or 101 when the sort reports a failed wait. */
 int fmm2dhip_test_sort(const void *keys, int n, int key_bytes, int begin, int end,
                        void *keys_out, int *vals_out);
+
+/* ---- test hook (no counterpart in the reference): the diagonal-form
+ * translations' FFT.  Transforms nrows rows of n complex values (interleaved
+ * re, im; n an even product of powers of 2, 3 and 5, as next235 returns),
+ * unnormalised: sign -1 sum_j x_j e^{-2 pi i jk/n} (FFTPACK zfftf), +1 with
+ * e^{+...} (zfftb).  Returns 0, or 102 for bad arguments. */
+int fmm2dhip_test_fft(int n, int nrows, int sign, const double *in, double *out);
 
 /* ---- test hook (no counterpart in the reference): the solvers' device
  * tree for the given points (copied to the device first; pg = pgt = 1 work

@@ -13,11 +13,15 @@
  *                                          scale factors F implied by the scaled
  *                                          conventions (mp_n = M_n / rscale^|n|,
  *                                          lo_n = L_n rscale^|n|)
- *   8-step engine                          c_translation/src/hfmm2d.c:619-1523 (branches
- *                                          with boxlam <= 16 only)
+ *   8-step engine                          c_translation/src/hfmm2d.c:619-1523
  *   drivers                                hfmm2d.c:1529-1907, hfmm2d_ndiv.c:68-351
- * Boxes larger than 16 wavelengths (the diagonal-form branches,
- * wideband2d.c) are not restated: the call returns ier = 105.
+ * Boxes larger than 16 wavelengths: the reference translates them in the
+ * diagonal form (wideband2d.c:33-290, hfmm2d.c:946-1254), signatures of
+ * nsig = next235(2 P + 1) points multiplied pointwise by the transformed
+ * kernel.  With nsig >= 2 P + 1 the circular convolution has no aliasing,
+ * so the result is the dense translation sum_n a_n B(j - n) exactly; the
+ * oracle computes that dense sum at every level (pinned against the
+ * compiled reference's diagonal form at zk = 500 in test_oracle_helm.py).
  */
 #include "oracle.h"
 #include <math.h>
@@ -285,11 +289,11 @@ static int helm_main(const otree *t, const olists *L, const helm_job *J, const i
     const double *bs = t->boxsize;
     int pmax = 0;
     for (int l = 0; l <= nl; l++) pmax = nterms[l] > pmax ? nterms[l] : pmax;
-    /* boxes of more than 16 wavelengths use the diagonal form in the
-     * reference; levels 0 and 1 form no expansion that any step reads */
-    double dlam = 1.0 / (creal(zk) / (2 * PI));
-    for (int l = 2; l <= nl; l++)
-        if (bs[l] / dlam > 16.0 && zi * bs[l] < 40.0) return 105;
+    /* Boxes of more than 16 wavelengths use the diagonal form in the
+     * reference (wideband2d.c).  Its signatures of nsig >= 2 P + 1 points
+     * make the circular convolution exact, so it equals the dense
+     * translation below in exact arithmetic: the oracle translates every
+     * level densely. */
     size_t per = (size_t)nd * (2 * pmax + 1);
     cplx *mp = calloc(per * (nb + 1), sizeof(cplx)), *lo = calloc(per * (nb + 1), sizeof(cplx));
     const cplx *c = J->ifcharge ? J->chg : NULL, *q = J->ifdipole ? J->dip : NULL;

@@ -236,3 +236,28 @@ def test_oracle_hfmm_root_box_above_16_wavelengths(orc, ref):
     assert a.ier == 0 and b.ier == 0
     for k in ("pot", "grad", "targ_pot", "targ_grad"):
         assert rel_l2(a[k], b[k]) < 1e-11, k
+
+
+@pytest.mark.parametrize("zk,n,eps", [(500.0, 8000, 1e-6), (500.0, 3000, 1e-9),
+                                      (450.0 + 2.0j, 5000, 1e-6)])
+def test_oracle_dense_equals_reference_diagonal_form(orc, ref, zk, n, eps):
+    """Level-2 boxes of the unit square are 18-20 wavelengths across at these
+    wavenumbers: the reference translates them in the diagonal form
+    (wideband2d.c), the oracle densely.  Signatures of nsig >= 2 P + 1
+    points convolve without aliasing, so the two agree to rounding."""
+    if ref is None:
+        pytest.skip("reference library not built here")
+    rng = np.random.default_rng(7)
+    src = rng.random((2, n))
+    tg = rng.random((2, 1000))
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dp = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    th = rng.random(n) * 2 * np.pi
+    dv = np.stack([np.cos(th), np.sin(th)])[None]
+    kw = dict(eps=eps, zk=zk, sources=src, charges=ch, dipstr=dp, dipvec=dv, pg=2, targets=tg,
+              pgt=2)
+    a = abi.hfmm2d_call(orc, "orc_hfmm2d_", **kw)
+    b = abi.hfmm2d_call(ref, "hfmm2d_", **kw)
+    assert a.ier == 0 and b.ier == 0
+    for k in ("pot", "grad", "targ_pot", "targ_grad"):
+        assert rel_l2(a[k], b[k]) < 1e-12, k
