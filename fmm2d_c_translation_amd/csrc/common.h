@@ -24,7 +24,8 @@ enum : int {
   IER_DEPTH = 103,      // tree would need more than kMaxDepth levels
   IER_NOGPU = 104,      // no usable gfx950 device
   IER_HIGHFREQ = 105,   // Helmholtz box size above 16 wavelengths (diagonal form)
-  IER_P_TOO_LARGE = 106 // expansion order beyond the kernels' register/LDS budget
+  IER_P_TOO_LARGE = 106, // expansion order beyond the kernels' register/LDS budget
+  IER_SCRATCH = 107      // beyond the allocation-free entry point's fixed scratch
 };
 
 // Keys hold 2 bits per level below the root: kMaxDepth levels in 58 bits.

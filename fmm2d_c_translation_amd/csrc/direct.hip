@@ -570,6 +570,13 @@ int c2d(int nd, const double* sources, int ns, const void* charge, const void* d
 }
 
 }  // namespace
+
+int h2d_direct_cdg(int nd, const fmm2d_cplx* zk, const double* s, int ns, const fmm2d_cplx* c,
+                   const fmm2d_cplx* q, const double* v, const double* t, int nt,
+                   fmm2d_cplx* pot, fmm2d_cplx* grad, double th) {
+  return h2d<true, true, 2>(nd, zk, s, ns, c, q, v, t, nt, pot, grad, nullptr, th, false);
+}
+
 }  // namespace fmm
 
 extern "C" {

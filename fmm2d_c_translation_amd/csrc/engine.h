@@ -54,6 +54,26 @@ struct Ctx {
 };
 int get_ctx(Ctx& c);
 
+// While a FixedArena is alive on a thread, that thread's Scratch takes its
+// device memory from [base, base + cap) by pointer bump and never calls
+// hipMalloc (a request beyond cap fails as IER_ALLOC), and get_ctx hands
+// out `stream` instead of creating the thread's own.  For callers that
+// cannot allocate; the arena's previous contents are garbage.
+class FixedArena {
+ public:
+  FixedArena(void* base, size_t cap, hipStream_t stream);
+  ~FixedArena();
+  FixedArena(const FixedArena&) = delete;
+  FixedArena& operator=(const FixedArena&) = delete;
+  size_t used() const;      // high-water mark of the bump pointer, bytes
+  bool overflowed() const;  // a request did not fit
+};
+
+// h2d_directcdg_ (helmkernels2d.c) with its return code (direct.hip)
+int h2d_direct_cdg(int nd, const fmm2d_cplx* zk, const double* s, int ns, const fmm2d_cplx* c,
+                   const fmm2d_cplx* q, const double* v, const double* t, int nt,
+                   fmm2d_cplx* pot, fmm2d_cplx* grad, double th);
+
 // Tree arrays on the device (0-based box ids, lists in CSR form).
 struct DTree {
   int nb = 0;

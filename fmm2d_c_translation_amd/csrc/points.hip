@@ -69,10 +69,32 @@ struct Pinned {
 thread_local Pinned g_pinned;
 }  // namespace
 
+namespace {
+struct ArenaOverride {
+  char* base = nullptr;
+  size_t cap = 0, used = 0, high = 0;
+  hipStream_t st = nullptr;
+  bool on = false, overflow = false;
+};
+thread_local ArenaOverride g_fixed;
+}  // namespace
+
+FixedArena::FixedArena(void* base, size_t cap, hipStream_t stream) {
+  g_fixed = ArenaOverride{static_cast<char*>(base), cap, 0, 0, stream, true, false};
+}
+FixedArena::~FixedArena() { g_fixed = ArenaOverride{}; }
+size_t FixedArena::used() const { return g_fixed.high; }
+bool FixedArena::overflowed() const { return g_fixed.overflow; }
+
 int get_ctx(Ctx& c) {
   ThreadStream& T = g_stream;
   int d = 0;
   if (hipGetDevice(&d) != hipSuccess) return IER_NOGPU;
+  if (g_fixed.on) {
+    c.st = g_fixed.st;
+    c.device = d;
+    return IER_OK;
+  }
   if (!T.st || T.dev != d) {
     if (hipStreamCreateWithFlags(&T.st, hipStreamNonBlocking) != hipSuccess) return IER_NOGPU;
     T.dev = d;
@@ -100,8 +122,19 @@ Scratch::Scratch(hipStream_t s) : st_(s) {
 }
 
 void* Scratch::take(size_t bytes) {
-  Arena& A = g_arena[dev_];
   bytes = (bytes + 255) & ~size_t(255);
+  if (g_fixed.on) {  // a FixedArena: bump only, never allocate
+    ArenaOverride& F = g_fixed;
+    if (bytes > F.cap - F.used) {
+      F.overflow = true;
+      return nullptr;
+    }
+    void* p = F.base + F.used;
+    F.used += bytes;
+    F.high = std::max(F.high, F.used);
+    return p;
+  }
+  Arena& A = g_arena[dev_];
   A.total += bytes;
   if (!A.blocks.empty() && A.used + bytes <= A.blocks.back().second) {
     void* p = A.blocks.back().first + A.used;
@@ -120,6 +153,11 @@ Scratch::~Scratch() {
   Arena& A = g_arena[dev_];
   if (--A.depth > 0) return;
   (void)hipStreamSynchronize(st_);
+  if (g_fixed.on) {
+    last_stats().workspace_bytes = (double)g_fixed.high;
+    g_fixed.used = 0;
+    return;
+  }
   if (A.blocks.size() > 1) {
     size_t want = std::max(A.total + A.total / 4, kBlockMin);
     for (auto& b : A.blocks) (void)hipFree(b.first);

@@ -214,6 +214,25 @@ void h2d_directcdp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sou
 void h2d_directcdg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
 void h2d_directcdh_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
 
+/* ---- allocation-free direct sum.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Arguments and
+ * result as h2d_directcdg_ (helmkernels2d.c), run on `stream` (a
+ * hipStream_t; NULL for the null stream), with all device memory -- staged
+ * copies of the host arrays and the per-chunk partial sums -- taken from a
+ * fixed 64 MiB array the library holds on each GPU until the process ends;
+ * no hipMalloc, hipFree or other allocating call is made.  Returns 0; 107
+ * when the call needs more than that array (about 1e6 sources with 1e3
+ * targets), the outputs then untouched; 101 / 104 for HIP / device errors.
+ * Every call on one GPU shares its array: calls that overlap on one GPU
+ * (from several threads or streams) race. */
+int fmm2dhip_h2d_directcdg_noalloc(const fmm2d_int *nd, const fmm2d_cplx *zk,
+                                   const double *sources, const fmm2d_int *ns,
+                                   const fmm2d_cplx *charge, const fmm2d_cplx *dipstr,
+                                   const double *dipvec, const double *targ, const fmm2d_int *nt,
+                                   fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh,
+                                   void *stream);
+
 /* ---- device-resident variants (no counterpart in the reference).
  * Same argument meaning as cfmm2d_ndiv_ / lfmm2d_ndiv_, but every array is a
  * device pointer, and `stream` (a hipStream_t, or NULL for the library's
