@@ -658,7 +658,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     std::vector<int> bm(nbs + nbt);
     e = sync_read(c, d_depth, nbs + nbt, bm.data(), plan.sort_err);
     if (e) return e;
-    for (int m : bm) Ls = std::max(Ls, m);
+    for (int m : bm) {
+      if (m < -1 || m > kMaxDepth) return IER_HIP;  // read-back out of range
+      Ls = std::max(Ls, m);
+    }
   } else {
     // nothing splits: the copies of the root box still have to land
     HIP_TRY(hipStreamSynchronize(c.st));
@@ -741,6 +744,11 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     e = sync_read(c, d_roff, L + 2, ro.data(), plan.sort_err);
     if (e) return e;
     for (int l = 0; l <= L; ++l) nR[l] = ro[l + 1] - ro[l];
+    // counts read back are checked before they size anything: at most cap
+    // split boxes in all, none negative
+    if (ro[0] != 0 || ro[L + 1] > cap) return IER_HIP;
+    for (int l = 0; l <= L; ++l)
+      if (nR[l] < 0) return IER_HIP;
   }
   if (Ls >= 0 && nR[Ls] == 0) return IER_ARGS;  // the depth and the refinement disagree
   double t1 = wall();
@@ -787,7 +795,13 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     std::vector<int> lo(L + 2);
     e = sync_read(c, d_loff, L + 2, lo.data(), plan.sort_err);
     if (e) return e;
-    for (int l = 0; l <= L; ++l) nR[l] = lo[l + 1] - lo[l];
+    if (lo[0] != 0 || lo[L + 1] > nc) return IER_HIP;
+    for (int l = 0; l <= L; ++l) {
+      nR[l] = lo[l + 1] - lo[l];
+      // split boxes of level l have split parents at level l - 1, and a box
+      // has 4 children: level l holds at most 4 nR[l-1] of them
+      if (nR[l] < 0 || (l > 0 && nR[l] > 4L * nR[l - 1])) return IER_HIP;
+    }
   }
   double t2 = wall();
   // ---- 3. boxes
@@ -838,6 +852,13 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   HIP_TRY(hipMemcpyAsync(tot4, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
   e = sync_read(c, tot4, 4, tot, plan.sort_err);
   if (e) return e;
+  // list lengths per box are bounded (13, 27, 20 and 5 after level restriction;
+  // twice that is a generous cap on any tree this code builds)
+  {
+    const long lim[4] = {26L * nb, 54L * nb, 40L * nb, 10L * nb};
+    for (int k = 0; k < 4; ++k)
+      if (tot[k] < 0 || tot[k] > lim[k]) return IER_HIP;
+  }
   lo.l1 = S.get<int>(std::max(tot[0], 1));
   lo.l2 = S.get<int>(std::max(tot[1], 1));
   lo.l3 = S.get<int>(std::max(tot[2], 1));
@@ -887,6 +908,12 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     wl[j]->off = hs[j];
     wl[j]->n = hs[WL_N + 1 + j];
     wl[j]->lev.assign(hs.begin() + ninfo + j * W, hs.begin() + ninfo + (j + 1) * W);
+    // every list holds at most nb boxes, in level order
+    if (wl[j]->n < 0 || wl[j]->n > nb || wl[j]->off < 0 || wl[j]->lev.front() != 0 ||
+        wl[j]->lev.back() != wl[j]->n)
+      return IER_HIP;
+    for (int l = 0; l + 1 < W; ++l)
+      if (wl[j]->lev[l] > wl[j]->lev[l + 1]) return IER_HIP;
   }
   plan.l2l_kids.off = hs[WL_N];
   plan.l2l_kids.n = 4 * plan.l2l.n;

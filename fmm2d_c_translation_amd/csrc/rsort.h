@@ -264,14 +264,19 @@ int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* v
 // *err: when a wait gave up, the unsorted values (vin is left intact) replace
 // the output, so consumers still see valid indices in a valid (unsorted)
 // order and the failure is reported when the host reads *err.
-__global__ void k_rs_fallback(const uint32_t* err, const int* vin, int* vout, int n) {
-  if (!ld_relaxed(err)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    vout[i] = vin[i];
+// One block: the error word (pinned host memory) is read once, by one
+// thread, and the copy only runs when it is set.
+__global__ void __launch_bounds__(256) k_rs_fallback(const uint32_t* err, const int* vin,
+                                                     int* vout, int n) {
+  __shared__ uint32_t flag;
+  if (threadIdx.x == 0) flag = ld_relaxed(err);
+  __syncthreads();
+  if (!flag) return;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) vout[i] = vin[i];
 }
 inline void fallback_unsorted(hipStream_t st, const uint32_t* err, const int* vin, int* vout,
                               int n) {
-  if (n > 0) k_rs_fallback<<<std::min(64, (n + 255) / 256), 256, 0, st>>>(err, vin, vout, n);
+  if (n > 0) k_rs_fallback<<<1, 256, 0, st>>>(err, vin, vout, n);
 }
 
 }  // namespace
