@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstddef>
 
+#include "log_table.h"
+
 #define FMM_HD __host__ __device__ inline
 
 namespace fmm {
@@ -111,6 +113,53 @@ FMM_HD double log_pos(double x) {
   const double v = log_pos_core(x);
   const bool normal = x > 0.0 && x <= 1.7976931348623157e308;
   return normal ? v : x == 0.0 ? -HUGE_VAL : x > 0.0 ? x : NAN;
+}
+
+// 1/sqrt(x) for a positive normal finite x: the hardware estimate refined
+// by one third-order step, y + y e (1/2 + 3/8 e) with e = 1 - x y^2
+// (Householder's method for x y^2 = 1), without the special-value checks of
+// the library rsqrt.
+FMM_HD double rsqrt_pos(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+#else
+  return 1.0 / std::sqrt(x);
+#endif
+}
+
+// Natural log of a positive normal finite double by table: x = 2^e m with
+// m in [1, 2) rounded to the nearest c_j = 1 + j/256, whose table row holds
+// r_j ~ 1/c_j and L_j = -ln r_j exactly rounded (log_table.h; from c_j >=
+// sqrt 2 the row holds L_j - ln 2 and e counts one more), so ln x =
+// e ln 2 + L_j + log1p(u) with u = m r_j - 1 (one FMA, |u| <= 2^-9) and
+// log1p(u) = u - u^2/2 + ... + u^7/7 (the first omitted term is below
+// 1e-23); e ln 2 in two parts as in log_pos_core.  Next to x = 1 the table
+// term is exactly 0.  About 18 instructions and one table load, against
+// about 30 for log_pos_core; no special cases.
+struct LogRow {
+  double r, l;
+};
+static constexpr LogRow kLogTab[LOG_TAB_N] = {LOG_TAB_ROWS};
+FMM_HD double log_tab(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint64_t frac = b & 0x000fffffffffffffull;
+  const int j = (int)((frac + (uint64_t(1) << 43)) >> 44);  // 0 .. 256
+  const int e = (int)((b >> 52) & 0x7ff) - 1023 + (j >= LOG_TAB_SHIFT);
+  const double m = __builtin_bit_cast(double, frac | 0x3ff0000000000000ull);
+  const LogRow t = kLogTab[j];
+  const double u = fma(m, t.r, -1.0);
+  double p = 1.0 / 7.0;
+  p = fma(p, u, -1.0 / 6.0);
+  p = fma(p, u, 1.0 / 5.0);
+  p = fma(p, u, -1.0 / 4.0);
+  p = fma(p, u, 1.0 / 3.0);
+  p = fma(p, u, -0.5);
+  const double l1p = fma(p * u, u, u);
+  const double ln2_hi = 0x1.62e42p-1, ln2_lo = 0x1.fdf473de6af28p-22;
+  const double ed = (double)e;
+  return fma(ed, ln2_hi, fma(ed, ln2_lo, t.l + l1p));
 }
 
 // Quadrant digit of a point relative to a box centre: bit 0 set for the

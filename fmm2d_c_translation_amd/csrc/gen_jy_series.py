@@ -6,7 +6,8 @@ w = x^2/4, exactly rounded from rational arithmetic:
             Q(w) = sum_k YC[k] w^k,  YC[k] = -(-1)^k H_k / (k!)^2
 with H_k the k-th harmonic number; J1 = -J0' and Y1 = -Y0' follow from
 P'(w) and Q'(w) (dw/dx = x/2).  JY_NTERMS<i> is the number of terms that
-leaves the truncated tails of P, P', Q and Q' below 1e-18 for x <= JY_XMAX<i>.
+leaves the truncated tails of P, P', Q and Q' below 1e-18 for x <= JY_XMAX<i>;
+jy_nterms picks the shortest series for a bound on x.
 Usage:
     python gen_jy_series.py > jy_series.h"""
 from fractions import Fraction
@@ -19,7 +20,7 @@ for k in range(1, K + 1):
 JA = [Fraction((-1) ** k, factorial(k) ** 2) for k in range(K)]
 YC = [-Fraction((-1) ** k) * H[k] / factorial(k) ** 2 for k in range(K)]
 
-XMAX = [1.0, 4.0]
+XMAX = [0.35, 1.0, 4.0]
 NT = []
 for xm in XMAX:
     w = Fraction(xm) ** 2 / 4
@@ -46,5 +47,10 @@ print(f"#define JY_K {K}")
 print(row("JY_JA", JA))
 print(row("JY_YC", YC))
 for i, (xm, n) in enumerate(zip(XMAX, NT)):
-    print(f"constexpr double JY_XMAX{i} = {xm:.1f};")
+    print(f"constexpr double JY_XMAX{i} = {xm!r};")
     print(f"constexpr int JY_NTERMS{i} = {n};")
+print("// series length for arguments up to xmax (0: beyond the series)")
+print("inline int jy_nterms(double xmax) {")
+print("  return xmax <= 0.0 ? 0 : xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1")
+print("       : xmax <= JY_XMAX2 ? JY_NTERMS2 : 0;")
+print("}")

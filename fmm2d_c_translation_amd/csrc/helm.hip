@@ -902,6 +902,7 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
   const cd ik4 = zk * mk(0, 0.25);
   const bool cp_mode = C && !Q && PG == 1;
   const double izk = NR > 0 ? 1.0 / zk.x : 0.0;
+  const double kk4 = zk.x * zk.x * 0.25;  // w = (k r)^2 / 4 = kk4 r^2
   for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
     const int Tc = min(WAVE, tr.y - t0);
     const int K = WAVE / Tc;
@@ -932,25 +933,28 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
           // b = q g, and the dipole potential is b (d.v).
           for (int m = k; m < cnt; m += K) {
             const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
-            const double rr = dx * dx + dy * dy;
-            const double ri0 = rsqrt(fmax(rr, 0x1p-1022));
+            const double rr = fmax(dx * dx + dy * dy, 0x1p-1022);
+            const double ri0 = rsqrt_pos(rr);
             const double az0 = zk.x * (rr * ri0);
             const bool skip = cp_mode ? az0 <= thresh : az0 < thresh;
-            const double az = skip ? 1.0 : az0, ri = skip ? 1.0 : ri0;
-            const double wt = skip ? 0.0 : 1.0;  // strengths are finite
+            // a skipped pair: the series at x = 1 (w = 1/4), then H0 and
+            // 1/r set to 0, which zeroes every term of the pair
+            const double az = skip ? 1.0 : az0, ri = skip ? 0.0 : ri0;
+            const double w = skip ? 0.25 : kk4 * rr;
             cd h0, h1;
-            hank_series_real<(NR > 0 ? NR : 1), true>(az, ri * izk, h0, h1);
+            hank_series_real_w<(NR > 0 ? NR : 1), true>(az, ri * izk, w, h0, h1);
+            if (skip) h0 = cd{0.0, 0.0};
             const cd g = h1 * ri;
             cd a = mk(0, 0);
             if (C) {
               const double2 cv = sc[m];
-              const cd c = cd{cv.x * wt, cv.y * wt};
+              const cd c = cd{cv.x, cv.y};
               A.Pc += h0 * c;
               if (PG >= 2) a = g * c;
             }
             if (Q) {
               const double2 qv = sq[m];
-              const cd q = cd{qv.x * wt, qv.y * wt};
+              const cd q = cd{qv.x, qv.y};
               const double v1 = sv[m].x, v2 = sv[m].y;
               const double dv = dx * v1 + dy * v2;
               const cd bq = q * g;
@@ -1143,7 +1147,7 @@ void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& 
                  int NS, const int2* trng, const double2* tpts, int NT, int nd, double th,
                  double xmax, double2* pot, double2* grad, double2* hess, const int* perm,
                  double2* opot, double2* ograd, double2* ohess) {
-  const int nr = xmax <= 0.0 ? 0 : xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
+  const int nr = jy_nterms(xmax);
 #define FMM_HP2P(PGV, NRV)                                                                  \
   k_hp2p<C, Q, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, D, zk, src, chg, dip, dv, NS, trng, tpts, \
                                                NT, nd, th, pot, grad, hess, perm, opot, ograd, \
@@ -1152,6 +1156,7 @@ void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& 
   do {                                                   \
     if (nr == JY_NTERMS0) FMM_HP2P(PGV, JY_NTERMS0);     \
     else if (nr == JY_NTERMS1) FMM_HP2P(PGV, JY_NTERMS1); \
+    else if (nr == JY_NTERMS2) FMM_HP2P(PGV, JY_NTERMS2); \
     else FMM_HP2P(PGV, 0);                               \
   } while (0)
   if (pg == 1) FMM_HP2P_NR(1);
@@ -1172,7 +1177,7 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
   int nr = 0;
   if (rz) {
     if (!MPK) nr = 1;
-    else nr = xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1 : 0;
+    else nr = jy_nterms(xmax);
   }
   const int nbx = (nw + 3) / 4;
   // staged expansions (value and gradient, NQ entries per order) while
@@ -1197,6 +1202,7 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
     } else {                                                             \
       if (nr == JY_NTERMS0) FMM_HEVAL(PGV, JY_NTERMS0);                  \
       else if (nr == JY_NTERMS1) FMM_HEVAL(PGV, JY_NTERMS1);             \
+      else if (nr == JY_NTERMS2) FMM_HEVAL(PGV, JY_NTERMS2);             \
       else FMM_HEVAL(PGV, 0);                                            \
     }                                                                    \
   } while (0)
@@ -1411,7 +1417,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     for (int l = 0; l <= L && lmin > L; ++l)
       if (plan.list4.at(l).n) lmin = l;
     const double xmax4 = lmin <= L ? zk.x * 4.95 * bs[lmin] : 0.0;
-    nr4 = xmax4 <= JY_XMAX0 ? JY_NTERMS0 : xmax4 <= JY_XMAX1 ? JY_NTERMS1 : 0;
+    nr4 = jy_nterms(xmax4);
   }
   StepTimer* tm = step_timer();
   {
@@ -1486,8 +1492,14 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     else if (fg && nr4 == JY_NTERMS1)
       k_hform_g<false, JY_NTERMS1><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
           wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+    else if (fg && nr4 == JY_NTERMS2)
+      k_hform_g<false, JY_NTERMS2><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
+          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else
-      (nr4 == JY_NTERMS0 ? k_hform<false, JY_NTERMS0> : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1> : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(
+      (nr4 == JY_NTERMS0   ? k_hform<false, JY_NTERMS0>
+       : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1>
+       : nr4 == JY_NTERMS2 ? k_hform<false, JY_NTERMS2>
+                           : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(
           wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
   }
   tm->mark(c.st);

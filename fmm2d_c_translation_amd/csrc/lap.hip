@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(WAVE) k_p2p(const int* work, DTree D, const do
             const double rinv = 1.0 / rr;
             const cd zi = mk(dx * rinv, -dy * rinv);
             const cd z2 = -(zi * zi);
-            const double lg = C ? 0.5 * log_pos(rr) : 0.0;
+            const double lg = C ? 0.5 * log_tab(rr) : 0.0;  // rr > th2 > 0
             const cd z3 = (z2 * zi) * -2.0;
 #pragma unroll
             for (int e = 0; e < NC; ++e) {

@@ -73,25 +73,32 @@ FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
 // Horner with their derivatives, J1 = -P'(w) x/2 and
 // Y1 = -(2/pi) [J0/x - (ln(x/2) + gamma) J1 + Q'(w) x/2]
 // (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
-// Core = true: x/2 must be a positive normal double (log without the
-// special cases).
+// Core = true: x/2 must be a positive normal double (the table log,
+// log_tab, without special cases).
 template <int N, bool Core = false>
-FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
-  const double w = x * x * 0.25;
-  double p = JY_JA[N - 1], dp = 0.0, q = JY_YC[N - 1], dq = 0.0;
+FMM_HD void hank_series_real_w(double x, double xi, double w, cd& h0, cd& h1) {
+  // Horner for P and P' (and Q, Q'), the first step of the derivative
+  // chain peeled (it starts from the top coefficient)
+  double dp = JY_JA[N - 1], p = fma(JY_JA[N - 1], w, JY_JA[N - 2]);
+  double dq = JY_YC[N - 1], q = fma(JY_YC[N - 1], w, JY_YC[N - 2]);
 #pragma unroll
-  for (int k = N - 2; k >= 0; --k) {
+  for (int k = N - 3; k >= 0; --k) {
     dp = fma(dp, w, p);
     p = fma(p, w, JY_JA[k]);
     dq = fma(dq, w, q);
     q = fma(q, w, JY_YC[k]);
   }
-  const double hx = 0.5 * x, lg = (Core ? log_pos_core(hx) : log_pos(hx)) + kEuler;
+  const double hx = 0.5 * x, lg = (Core ? log_tab(hx) : log_pos(hx)) + kEuler;
   const double j1 = -dp * hx;
   const double y0 = (lg * p + q) * (2.0 / kPi);
   const double y1 = -(2.0 / kPi) * (p * xi - lg * j1 + dq * hx);
   h0 = cd{p, y0};
   h1 = cd{j1, y1};
+}
+// w = x^2/4 computed here
+template <int N, bool Core = false>
+FMM_HD void hank_series_real(double x, double xi, cd& h0, cd& h1) {
+  hank_series_real_w<N, Core>(x, xi, x * x * 0.25, h0, h1);
 }
 template <int N>
 FMM_HD void hank_series_real(double x, cd& h0, cd& h1) {
