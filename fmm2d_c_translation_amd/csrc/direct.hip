@@ -408,6 +408,144 @@ hipError_t launch_chunked(hipStream_t st, Scratch& S, int ns, int nt, size_t no,
   return hipGetLastError();
 }
 
+// Stokes (stokkernels2d.c:11-173): Stokeslets f and stresslets (mu, nu) at
+// the sources, real outputs velocity pot (nd,2,nt), pressure pre (nd,nt)
+// and velocity gradient grad (nd,2,2,nt), each incremented; pairs with
+// r^2 < thresh^2 skipped.  With d = t - s, r2 = |d|^2, rt = d1^2 - d2^2:
+//   Stokeslet: pot -= f log(r2)/4 - d (d.f)/(2 r2); pre += (d.f)/r2;
+//     grad11 -= rt (d.f)/(2 r2^2), grad22 += the same,
+//     grad21 += (pl2 rt - 4 f1 d2 d1^2)/(2 r2^2),
+//     grad12 -= (pl2 rt + 4 f2 d1 d2^2)/(2 r2^2), pl2 = d1 f2 + d2 f1;
+//   stresslet, pl = d.mu, pl2 = d.nu, pv = mu.nu:
+//     pot -= 2 d pl pl2 / r2^2; pre -= 4 pl pl2 / r2^2 - 2 pv / r2;
+//     grad(a,b) -= 2 d_b (mu_a pl2 + nu_a pl)/r2^2 - 8 d_a d_b pl pl2/r2^3
+//       (plus 2 pl pl2 / r2^2 on the diagonal).
+// grad(a, b) = d u_b / d x_a is grad[((t*2 + b)*2 + a)*nd + d] (column-major
+// (nd,2,2,nt)).
+// Chunk slices as the other direct sums: pre is the first output, pot the
+// second, grad the third (launch_chunked's pot / grad / hess).
+template <bool STOK, bool STRS>
+__global__ void __launch_bounds__(TILE) k_st2d(const double2* src, int ns, const double* stok,
+                                               const double* mu, const double* nu,
+                                               const double2* trg, int nt, int nd, double th2,
+                                               double* pre, double* pot, double* grad, int len,
+                                               size_t sp, size_t sg, size_t sh) {
+  const int sb = blockIdx.y * len, se = min(ns, sb + len);
+  pre += blockIdx.y * sp;
+  pot += blockIdx.y * sg;
+  grad += blockIdx.y * sh;
+  __shared__ double2 sx[TILE];
+  __shared__ double2 sf[TILE];
+  __shared__ double2 smu[TILE];
+  __shared__ double2 snu[TILE];
+  const int j = blockIdx.x * TILE + threadIdx.x;
+  const bool act = j < nt;
+  const double2 x = act ? trg[j] : make_double2(0, 0);
+  for (int d = 0; d < nd; ++d) {
+    double u1 = 0, u2 = 0, pr = 0, g11 = 0, g21 = 0, g12 = 0, g22 = 0;
+    for (int s0 = sb; s0 < se; s0 += TILE) {
+      const int cnt = min(TILE, se - s0);
+      if ((int)threadIdx.x < cnt) {
+        const size_t i = s0 + threadIdx.x;
+        sx[threadIdx.x] = src[i];
+        if (STOK) sf[threadIdx.x] = make_double2(stok[(i * 2 + 0) * nd + d], stok[(i * 2 + 1) * nd + d]);
+        if (STRS) {
+          smu[threadIdx.x] = make_double2(mu[(i * 2 + 0) * nd + d], mu[(i * 2 + 1) * nd + d]);
+          snu[threadIdx.x] = make_double2(nu[(i * 2 + 0) * nd + d], nu[(i * 2 + 1) * nd + d]);
+        }
+      }
+      __syncthreads();
+      if (act)
+        for (int m = 0; m < cnt; ++m) {
+          const double d1 = x.x - sx[m].x, d2 = x.y - sx[m].y;
+          const double r2 = d1 * d1 + d2 * d2;
+          if (r2 < th2) continue;
+          const double rt = d1 * d1 - d2 * d2, r4 = r2 * r2;
+          if (STOK) {
+            const double f1 = sf[m].x, f2 = sf[m].y;
+            const double lg = log_pos(r2) / 4.0;
+            const double pl = d1 * f1 + d2 * f2, pl2 = d1 * f2 + d2 * f1;
+            u1 += d1 * pl / r2 / 2.0 - f1 * lg;
+            u2 += d2 * pl / r2 / 2.0 - f2 * lg;
+            g11 -= rt * pl / r4 / 2.0;
+            g22 += rt * pl / r4 / 2.0;
+            g21 += (pl2 * rt - f1 * 4.0 * d2 * (d1 * d1)) / 2.0 / r4;
+            g12 -= (pl2 * rt + f2 * 4.0 * d1 * (d2 * d2)) / 2.0 / r4;
+            pr += pl / r2;
+          }
+          if (STRS) {
+            const double m1 = smu[m].x, m2 = smu[m].y, n1 = snu[m].x, n2 = snu[m].y;
+            const double pl = d1 * m1 + d2 * m2, pl2 = d1 * n1 + d2 * n2;
+            const double pv = m1 * n1 + m2 * n2, r6 = r4 * r2;
+            const double q = pl * pl2;
+            u1 -= 2 * d1 * q / r4;
+            u2 -= 2 * d2 * q / r4;
+            g11 -= 2 * q / r4 + 2 * d1 * m1 * pl2 / r4 + 2 * d1 * n1 * pl / r4 - 8 * d1 * d1 * q / r6;
+            g21 -= 2 * d1 * m2 * pl2 / r4 + 2 * d1 * n2 * pl / r4 - 8 * d1 * d2 * q / r6;
+            g12 -= 2 * d2 * m1 * pl2 / r4 + 2 * d2 * n1 * pl / r4 - 8 * d1 * d2 * q / r6;
+            g22 -= 2 * q / r4 + 2 * d2 * m2 * pl2 / r4 + 2 * d2 * n2 * pl / r4 - 8 * d2 * d2 * q / r6;
+            pr -= 4 * q / r4 - 2 * pv / r2;
+          }
+        }
+      __syncthreads();
+    }
+    if (act) {
+      const size_t t = (size_t)j;
+      pot[(t * 2 + 0) * nd + d] += u1;
+      pot[(t * 2 + 1) * nd + d] += u2;
+      pre[t * nd + d] += pr;
+      grad[((t * 2 + 0) * 2 + 0) * nd + d] += g11;
+      grad[((t * 2 + 0) * 2 + 1) * nd + d] += g21;
+      grad[((t * 2 + 1) * 2 + 0) * nd + d] += g12;
+      grad[((t * 2 + 1) * 2 + 1) * nd + d] += g22;
+    }
+  }
+}
+
+// Stokes direct sums through the host staging of the other routines
+template <bool STOK, bool STRS>
+int st2d(int nd, const double* sources, int ns, const double* stok, const double* mu,
+         const double* nu, const double* targ, int nt, double* pot, double* pre, double* grad,
+         double th) {
+  if (nd < 1 || ns < 0 || nt < 0) return IER_ARGS;
+  if (ns == 0 || nt == 0) return IER_OK;
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  Scratch S(c.st);
+  const size_t nq = (size_t)2 * nd * ns, no = (size_t)nd * nt;
+  double2* s = S.get<double2>(ns);
+  double2* t = S.get<double2>(nt);
+  double* fs = STOK ? S.get<double>(nq) : nullptr;
+  double* ms = STRS ? S.get<double>(nq) : nullptr;
+  double* vs = STRS ? S.get<double>(nq) : nullptr;
+  double* pr = S.get<double>(no);
+  double* po = S.get<double>(2 * no);
+  double* go = S.get<double>(4 * no);
+  if (S.err) return S.err;
+  HIP_TRY(hipMemcpyAsync(s, sources, 16 * (size_t)ns, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(t, targ, 16 * (size_t)nt, hipMemcpyHostToDevice, c.st));
+  if (STOK) HIP_TRY(hipMemcpyAsync(fs, stok, 8 * nq, hipMemcpyHostToDevice, c.st));
+  if (STRS) {
+    HIP_TRY(hipMemcpyAsync(ms, mu, 8 * nq, hipMemcpyHostToDevice, c.st));
+    HIP_TRY(hipMemcpyAsync(vs, nu, 8 * nq, hipMemcpyHostToDevice, c.st));
+  }
+  HIP_TRY(hipMemcpyAsync(pr, pre, 8 * no, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(po, pot, 16 * no, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(go, grad, 32 * no, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(launch_chunked(c.st, S, ns, nt, no, 2, 4, 3, pr, po, go,
+                         [&](dim3 grid, double* a, double* b, double* g, int len, size_t x,
+                             size_t y, size_t z) {
+                           k_st2d<STOK, STRS><<<grid, TILE, 0, c.st>>>(
+                               s, ns, fs, ms, vs, t, nt, nd, th * th, a, b, g, len, x, y, z);
+                         }));
+  HIP_TRY(hipMemcpyAsync(pre, pr, 8 * no, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipMemcpyAsync(pot, po, 16 * no, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipMemcpyAsync(grad, go, 32 * no, hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
+}
+
 // host staging for the direct sums: copy in, run, copy the outputs back
 struct DirectIO {
   int nd, ns, nt, ncg, nch;  // gradient / Hessian components per output
@@ -580,6 +718,28 @@ int h2d_direct_cdg(int nd, const fmm2d_cplx* zk, const double* s, int ns, const 
 }  // namespace fmm
 
 extern "C" {
+void st2ddirectstokg_(const fmm2d_int* nd, const double* sources, const double* stoklet,
+                      const fmm2d_int* ns, const double* targ, const fmm2d_int* nt, double* pot,
+                      double* pre, double* grad, const double* thresh) {
+  fmm::st2d<true, false>(*nd, sources, *ns, stoklet, nullptr, nullptr, targ, *nt, pot, pre, grad,
+                         *thresh);
+}
+void st2ddirectstokstrsg_(const fmm2d_int* nd, const double* sources, const fmm2d_int* ifstoklet,
+                          const double* stoklet, const fmm2d_int* istress, const double* strslet,
+                          const double* strsvec, const fmm2d_int* ns, const double* targ,
+                          const fmm2d_int* nt, double* pot, double* pre, double* grad,
+                          const double* thresh) {
+  const bool f = *ifstoklet == 1, m = *istress == 1;
+  if (f && m)
+    fmm::st2d<true, true>(*nd, sources, *ns, stoklet, strslet, strsvec, targ, *nt, pot, pre, grad,
+                          *thresh);
+  else if (f)
+    fmm::st2d<true, false>(*nd, sources, *ns, stoklet, nullptr, nullptr, targ, *nt, pot, pre,
+                           grad, *thresh);
+  else if (m)
+    fmm::st2d<false, true>(*nd, sources, *ns, nullptr, strslet, strsvec, targ, *nt, pot, pre,
+                           grad, *thresh);
+}
 void c2d_directcp_(const fmm2d_int* nd, const double* s, const fmm2d_int* ns, const fmm2d_cplx* c,
                    const double* t, const fmm2d_int* nt, fmm2d_cplx* pot, const double* th) {
   fmm::c2d<true, false, 1>(*nd, s, *ns, c, nullptr, t, *nt, pot, nullptr, nullptr, *th, 0);

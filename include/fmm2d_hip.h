@@ -214,6 +214,22 @@ void h2d_directcdp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sou
 void h2d_directcdg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
 void h2d_directcdh_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
 
+/* ---- direct sums, Stokes: replace st2ddirectstokg_ and st2ddirectstokstrsg_
+ * (c_translation/src/stokkernels2d.c:11-173, MEX-called through st2ddir,
+ * matlab/fmm2d.mw): Stokeslets stoklet (nd,2,ns) and stresslets
+ * strslet/strsvec (nd,2,ns) to velocity pot (nd,2,nt), pressure pre (nd,nt)
+ * and velocity gradient grad (nd,2,2,nt), all float64 and incremented;
+ * pairs with r^2 < thresh^2 skipped.  The Stokes FMM itself (stfmm2d) is
+ * not built. */
+void st2ddirectstokg_(const fmm2d_int *nd, const double *sources, const double *stoklet,
+                      const fmm2d_int *ns, const double *targ, const fmm2d_int *nt, double *pot,
+                      double *pre, double *grad, const double *thresh);
+void st2ddirectstokstrsg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ifstoklet,
+                          const double *stoklet, const fmm2d_int *istress, const double *strslet,
+                          const double *strsvec, const fmm2d_int *ns, const double *targ,
+                          const fmm2d_int *nt, double *pot, double *pre, double *grad,
+                          const double *thresh);
+
 /* ---- allocation-free direct sum.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Arguments and

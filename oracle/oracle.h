@@ -57,4 +57,9 @@ void o_h2dall(int nterms, cplx z, double rscale, cplx *hvec);
 void o_jbessel(int nterms, cplx z, double rscale, cplx *fjs);
 int  o_h2dterms(double bsize, cplx zk, double eps, int *ier);
 
+/* Stokes direct sums, stokkernels2d.c:11-173 (ostokes.c) */
+void oracle_st2d_direct(int nd, const double *src, int ns, int ifstok, const double *stok,
+                        int istress, const double *mu, const double *nu, const double *targ,
+                        int nt, double *pot, double *pre, double *grad, double thresh);
+
 #endif
