@@ -1,0 +1,71 @@
+"""The reference's MEX gateway (matlab/fmm2d.c with matlab/numbl/
+mex_shim.cpp) linked against libfmm2d_hip.so, without a GPU: every
+identifier the Python driver sends reaches its mwrap stub (a wrong dimension
+gives that stub's "Bad argument size" error before any library call, an
+unknown identifier gives "Unknown identifier"), hndiv2d through the gateway
+equals the library's hndiv2d_ (host-only), the stfmm2d stub raises, and
+MATLAB's squeeze.  The gateway is built only where the reference's matlab/
+sources are present."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fmm2d_c_translation_amd import mex
+
+pytestmark = pytest.mark.skipif(not mex.available(), reason="MEX gateway not built")
+
+
+def _dummy(spec):
+    toks = spec.split()
+    args = [np.zeros((1, 1), complex if "dcomplex" in t else float) for t in toks]
+    dims = []
+    for t in toks:
+        dims += [1, 1] if t.endswith("2") else [1]
+    return args, dims
+
+
+@pytest.mark.parametrize("name", sorted(mex.SPECS))
+def test_identifier_reaches_its_stub(name):
+    spec = mex.SPECS[name]
+    args, dims = _dummy(spec)
+    dims[0] = 2  # first argument's declared size no longer matches
+    with pytest.raises(mex.MexError, match="Bad argument size"):
+        mex.gateway().call(name, spec, *args, dims=dims)
+
+
+def test_unknown_identifier():
+    with pytest.raises(mex.MexError, match="Unknown identifier"):
+        mex.gateway().call("cfmm2d_nodiv", "int", 1)
+
+
+def test_mwrap_id_form():
+    assert mex.mwrap_id("hndiv2d", mex.SPECS["hndiv2d"]) == (
+        "hndiv2d(i double[x], i int[x], i int[x], i int[x], i int[x], i int[x], i int[x], "
+        "io int[x], io int[x])")
+
+
+@pytest.mark.parametrize("eps,ns,nt", [(1e-3, 5000, 0), (0.51e-6, 200000, 3), (1e-9, 10, 10),
+                                       (1e-12, 1000000, 1000000), (1e-15, 7, 0)])
+def test_hndiv2d_through_gateway(eng, eps, ns, nt):
+    nd, idf = mex.gateway().call("hndiv2d", mex.SPECS["hndiv2d"], eps, ns, nt, 1, 0, 1, 0, 20, 0)
+    a, b = C.c_int32(0), C.c_int32(5)
+    i32 = lambda v: C.byref(C.c_int32(v))
+    eng.hndiv2d_(C.byref(C.c_double(eps)), i32(ns), i32(nt), i32(1), i32(0), i32(1), i32(0),
+                 C.byref(a), C.byref(b))
+    assert (int(nd[0, 0]), int(idf[0, 0])) == (a.value, b.value)
+
+
+def test_stfmm2d_is_rejected():
+    src = {"sources": np.random.default_rng(0).random((2, 10)),
+           "stoklet": np.ones((2, 10))}
+    with pytest.raises(mex.MexError, match="Stokes FMM is not provided"):
+        mex.stfmm2d(1e-6, src, 1)
+
+
+def test_squeeze_matches_matlab():
+    assert mex.squeeze(np.zeros((1, 5))).shape == (1, 5)
+    assert mex.squeeze(np.zeros((1, 2, 5))).shape == (2, 5)
+    assert mex.squeeze(np.zeros((3, 2, 5))).shape == (3, 2, 5)
+    assert mex.squeeze(np.zeros((1, 1, 5))).shape == (5, 1)
+    assert mex.squeeze(np.zeros((1, 2, 2, 5))).shape == (2, 2, 5)
