@@ -408,6 +408,8 @@ struct WlArgs {
   const int2* trng;
   const int4* cnt;  // list lengths n1..n4
   int nb, pg, pgt;
+  const double2* ctr;  // box centres
+  const double* bs;    // box size per level
 };
 
 __device__ inline unsigned wl_mask(const WlArgs& a, int b) {
@@ -434,7 +436,10 @@ __device__ inline unsigned wl_mask(const WlArgs& a, int b) {
 }
 
 // per-block list counts; also the solve's work counts (list-1 pairs, list-2
-// pairs per level)
+// pairs per level) and, in pairs[4] (as the bits of a non-negative double,
+// whose order is that of the integers), the square of the largest
+// point-to-point distance over list 1: per axis |centre difference| + the
+// two half sizes, over leaves with points and list-1 boxes with sources
 __global__ void __launch_bounds__(WLB)
 k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, double* pairs,
            unsigned long long* m2l_lev) {
@@ -454,15 +459,25 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
   // list-1 source reads (sum over list 1 of the source counts) of the
   // leaves holding sources and of those holding targets
   double acc[4] = {0, 0, 0, 0};
+  double r2max = 0.0;
   if (b < a.nb) {
     if ((m >> WL_M2L) & 1) atomicAdd(&sm2l[a.level[b]], (unsigned long long)a.cnt[b].y);
     if (a.cnt[b].x > 0) {
+      const int nsb = a.srng[b].y - a.srng[b].x, ntb = a.trng[b].y - a.trng[b].x;
+      const double2 cb = a.ctr[b];
+      const double hb = 0.5 * a.bs[a.level[b]];
       double srcs = 0;
       for (int k = o1[b]; k < o1[b + 1]; ++k) {
-        int2 r = a.srng[l1[k]];
+        const int j = l1[k];
+        int2 r = a.srng[j];
         srcs += r.y - r.x;
+        if (r.y > r.x && nsb + ntb > 0) {
+          const double2 cj = a.ctr[j];
+          const double h = hb + 0.5 * a.bs[a.level[j]];
+          const double ex = fabs(cb.x - cj.x) + h, ey = fabs(cb.y - cj.y) + h;
+          r2max = fmax(r2max, ex * ex + ey * ey);
+        }
       }
-      const int nsb = a.srng[b].y - a.srng[b].x, ntb = a.trng[b].y - a.trng[b].x;
       acc[0] = srcs * nsb;
       acc[1] = srcs * ntb;
       acc[2] = nsb > 0 ? srcs : 0.0;
@@ -481,6 +496,10 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
     for (int w = 0; w < WLB / 64; ++w) t += sp[threadIdx.x][w];
     if (t != 0) atomicAdd(&pairs[threadIdx.x], t);
   }
+  for (int w = 32; w >= 1; w >>= 1) r2max = fmax(r2max, __shfl_xor(r2max, w));
+  if (lane == 0 && r2max > 0.0)
+    atomicMax(reinterpret_cast<unsigned long long*>(pairs + 4),
+              (unsigned long long)__double_as_longlong(r2max));
   if (threadIdx.x < WL_N) bcnt[threadIdx.x * nblocks + blockIdx.x] = sc[threadIdx.x];
   if (threadIdx.x <= kMaxDepth && sm2l[threadIdx.x])
     atomicAdd(&m2l_lev[threadIdx.x], sm2l[threadIdx.x]);
@@ -871,8 +890,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   const int W = L + 2;
   const int ninfo = 2 * WL_N + 1;
   // small results read back in one copy: info, level offsets, m2l pairs
-  // (as 2 ints each), list-1 pairs and source reads (4 doubles as 8 ints)
-  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 8;
+  // (as 2 ints each), list-1 pairs, source reads and the squared list-1
+  // distance bound (5 doubles as 10 ints)
+  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 10;
   int* small = S.get<int>(nsmall + 2);
   int* bcnt = S.get<int>((size_t)WL_N * nbk);
   if (S.err) return S.err;
@@ -881,7 +901,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   unsigned long long* m2l_lev = reinterpret_cast<unsigned long long*>(small + ((ninfo + WL_N * W + 1) & ~1));
   double* pairs = reinterpret_cast<double*>(m2l_lev + kMaxDepth + 1);
   HIP_TRY(hipMemsetAsync(small, 0, sizeof(int) * (nsmall + 2), c.st));
-  WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg};
+  WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg, ctr, dbs};
   k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, bcnt, nbk, pairs, m2l_lev);
   k_wl_scan<<<1, 64 * WL_N, 0, c.st>>>(bcnt, nbk, info);
   // every list holds at most nb entries; the L2L children at most 4 nb
@@ -991,6 +1011,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   st.p2p_pairs_trg = hp[1];
   st.p2p_reads_src = hp[2];
   st.p2p_reads_trg = hp[3];
+  plan.p2p_rmax = hp[4] >= 0.0 && hp[4] < 1e300 ? std::sqrt(hp[4]) : 0.0;
   st.nboxes = nb;
   st.nlevels = L;
   double t3 = wall();

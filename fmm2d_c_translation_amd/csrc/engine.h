@@ -131,6 +131,10 @@ struct Plan {
   WorkList l2l_kids;            // the children of those (lev[] by parent level)
   WorkList list4, list3_s, list3_t, leaves_s, leaves_t, p2p_s, p2p_t;
   std::vector<double> m2l_pairs;  // list-2 pairs of the m2l boxes, per level
+  // largest distance between a point of a leaf with points and a point of
+  // a box with sources in its list 1 (from box centres and sizes): bounds
+  // r over every near-field pair
+  double p2p_rmax = 0;
   // pinned host word, cleared before the front end's sorts and set by a sort
   // whose look-back wait gave up; read after the stream synchronises
   uint32_t* sort_err = nullptr;

@@ -20,7 +20,7 @@ for k in range(1, K + 1):
 JA = [Fraction((-1) ** k, factorial(k) ** 2) for k in range(K)]
 YC = [-Fraction((-1) ** k) * H[k] / factorial(k) ** 2 for k in range(K)]
 
-XMAX = [0.35, 1.0, 4.0]
+XMAX = [0.35, 0.6, 1.2, 4.0]
 NT = []
 for xm in XMAX:
     w = Fraction(xm) ** 2 / 4
@@ -52,5 +52,5 @@ for i, (xm, n) in enumerate(zip(XMAX, NT)):
 print("// series length for arguments up to xmax (0: beyond the series)")
 print("inline int jy_nterms(double xmax) {")
 print("  return xmax <= 0.0 ? 0 : xmax <= JY_XMAX0 ? JY_NTERMS0 : xmax <= JY_XMAX1 ? JY_NTERMS1")
-print("       : xmax <= JY_XMAX2 ? JY_NTERMS2 : 0;")
+print("       : xmax <= JY_XMAX2 ? JY_NTERMS2 : xmax <= JY_XMAX3 ? JY_NTERMS3 : 0;")
 print("}")

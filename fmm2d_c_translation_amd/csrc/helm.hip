@@ -927,7 +927,9 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
         __syncthreads();
         if (act && NR > 0 && PG <= 2) {
           // real k, branch-free: a pair under the threshold (a coincident one
-          // included) runs at r = 1 with zero strengths, so adds exactly 0.
+          // included, whose r^2 is raised to the smallest normal double) runs
+          // the series at its own tiny but finite arguments with 1/r and H0
+          // set to 0, which zeroes every term it adds.
           // With g = H1 / r and k H2 = 2 g - k H0, the charge and dipole
           // gradients are -a (dx, dy) + b v, a = g c + q k H2 (d.v) / r^2,
           // b = q g, and the dipole potential is b (d.v).
@@ -935,14 +937,11 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
             const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
             const double rr = fmax(dx * dx + dy * dy, 0x1p-1022);
             const double ri0 = rsqrt_pos(rr);
-            const double az0 = zk.x * (rr * ri0);
-            const bool skip = cp_mode ? az0 <= thresh : az0 < thresh;
-            // a skipped pair: the series at x = 1 (w = 1/4), then H0 and
-            // 1/r set to 0, which zeroes every term of the pair
-            const double az = skip ? 1.0 : az0, ri = skip ? 0.0 : ri0;
-            const double w = skip ? 0.25 : kk4 * rr;
+            const double az = zk.x * (rr * ri0);
+            const bool skip = cp_mode ? az <= thresh : az < thresh;
+            const double ri = skip ? 0.0 : ri0;
             cd h0, h1;
-            hank_series_real_w<(NR > 0 ? NR : 1), true>(az, ri * izk, w, h0, h1);
+            hank_series_real_w<(NR > 0 ? NR : 1), true>(az, ri * izk, kk4 * rr, h0, h1);
             if (skip) h0 = cd{0.0, 0.0};
             const cd g = h1 * ri;
             cd a = mk(0, 0);
@@ -1157,6 +1156,7 @@ void launch_hp2p(int pg, int nbx, hipStream_t st, const int* work, const DTree& 
     if (nr == JY_NTERMS0) FMM_HP2P(PGV, JY_NTERMS0);     \
     else if (nr == JY_NTERMS1) FMM_HP2P(PGV, JY_NTERMS1); \
     else if (nr == JY_NTERMS2) FMM_HP2P(PGV, JY_NTERMS2); \
+    else if (nr == JY_NTERMS3) FMM_HP2P(PGV, JY_NTERMS3); \
     else FMM_HP2P(PGV, 0);                               \
   } while (0)
   if (pg == 1) FMM_HP2P_NR(1);
@@ -1203,6 +1203,7 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
       if (nr == JY_NTERMS0) FMM_HEVAL(PGV, JY_NTERMS0);                  \
       else if (nr == JY_NTERMS1) FMM_HEVAL(PGV, JY_NTERMS1);             \
       else if (nr == JY_NTERMS2) FMM_HEVAL(PGV, JY_NTERMS2);             \
+      else if (nr == JY_NTERMS3) FMM_HEVAL(PGV, JY_NTERMS3);             \
       else FMM_HEVAL(PGV, 0);                                            \
     }                                                                    \
   } while (0)
@@ -1495,10 +1496,14 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     else if (fg && nr4 == JY_NTERMS2)
       k_hform_g<false, JY_NTERMS2><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
           wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+    else if (fg && nr4 == JY_NTERMS3)
+      k_hform_g<false, JY_NTERMS3><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
+          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else
       (nr4 == JY_NTERMS0   ? k_hform<false, JY_NTERMS0>
        : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1>
        : nr4 == JY_NTERMS2 ? k_hform<false, JY_NTERMS2>
+       : nr4 == JY_NTERMS3 ? k_hform<false, JY_NTERMS3>
                            : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(
           wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
   }
@@ -1574,15 +1579,11 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
                   a.gradt, a.hesst, Pt.perm_t, a.upott, a.ugradt, a.uhesst, &a.fused_t}};
   a.fused_s = a.fused_t = false;
   const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
-  // k r over list 1: a leaf of size s meets boxes of size 2s at most, so
-  // |dx|, |dy| <= 3 s and r <= 4.25 s, s the largest such leaf
+  // k r over list 1, from the tree's largest list-1 box-pair distance (for
+  // leaves of one size s, 2 sqrt(2) s; with a neighbour of size 2s, up to
+  // 4.25 s)
   double xmax = 0.0;
-  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) {
-    int lmin = L + 1;
-    for (int l = 0; l <= L && lmin > L; ++l)
-      if (plan.p2p_s.at(l).n || plan.p2p_t.at(l).n) lmin = l;
-    if (lmin <= L) xmax = a.zk.real() * 4.25 * bs[lmin];
-  }
+  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) xmax = a.zk.real() * plan.p2p_rmax;
   // k r over list 3: a point of a leaf of size s to the centre of a child
   // of a colleague is at most 1.75 s away per coordinate, r <= 2.48 s
   double xmax3 = 0.0;
