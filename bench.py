@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE of k_hp2p, each scaled
 # by what the same counter reads for a 1 GiB device copy (x2.0 and x1.0):
 # profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md
-P2P_TRAFFIC_BYTES_1M = 780.8e6
+P2P_TRAFFIC_BYTES_1M = 784.8e6
 TRAFFIC_SOURCE = "profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md"
 STEP_NAMES = ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"]
 
@@ -190,13 +190,18 @@ class DeviceProblem:
         return self.pot.cpu().numpy(), g, self.pott.cpu().numpy()[:self.nt], gt
 
 
-def timed(prob, kernel, eps, zk, ndiv, steps, warmup=1):
-    """Seconds per solve over `steps` solves (after `warmup`) and the last
-    solve's step times."""
+def timed(prob, kernel, eps, zk, ndiv, steps, warmup=3, warm_s=0.3):
+    """Seconds per solve over `steps` solves (after at least `warmup` solves
+    and `warm_s` seconds of them, so that the GPU clock has ramped up after
+    the host-only phases between legs) and the last solve's step times."""
     import torch
     ti = np.zeros(8)
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    k = 0
+    while k < warmup or time.perf_counter() - t0 < warm_s:
         prob.solve(kernel, eps, zk, ndiv, ti)
+        torch.cuda.synchronize()
+        k += 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -275,7 +280,7 @@ def reference_leg(lib, zk, eps, n_ref, threads):
     return out
 
 
-def c1_leg(lib, steps=10):
+def c1_leg(lib, steps=50):
     """cfmm2d_ndiv on 1e5 uniform points, complex charges, pot + grad at the
     sources, eps 1e-6, ndiv 40: FP64 and mixed Mpts/s, errors against direct
     sums (gradient: the pot has a box-dependent gauge) and against the
@@ -533,8 +538,8 @@ def served_c5(lib, seconds, precision, nworkers=8, budget=200e9):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
                     help="sources (= targets) of the OpenMP CPU baseline's sample")
