@@ -535,6 +535,59 @@ def served_c5(lib, seconds, precision, nworkers=8, budget=200e9):
             "data": "host arrays, arrival to result written (PCIe transfers included)"}
 
 
+def _stfmm_call(lib, eps, src, f, m, nv, pp):
+    """stfmm2d_ on host arrays (its only interface); returns the outputs."""
+    nd, ns = f.shape[0], src.shape[1]
+    z = lambda *sh: np.zeros(sh, order="F")
+    pot, pre, grad = z(nd, 2, ns), z(nd, ns), z(nd, 2, 2, ns)
+    tg = z(2, 1)
+    dum = z(nd, 2, 1)
+    ier = C.c_int32(-1)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    I = lambda v: C.byref(C.c_int32(v))
+    lib.stfmm2d_(I(nd), C.byref(C.c_double(eps)), I(ns), P(src), I(1), P(f), I(1), P(m), P(nv),
+                 I(pp), P(pot), P(pre), P(grad), I(0), P(tg), I(0), P(dum), P(dum[0]), P(dum),
+                 C.byref(ier))
+    if ier.value != 0:
+        raise RuntimeError(f"stfmm2d_ ier={ier.value}")
+    return pot, pre, grad
+
+
+def stokes_leg(lib, n=1_000_000, n_ref=200_000, steps=5, with_cpu=True):
+    """stfmm2d_ (Stokes FMM on the biharmonic engine): n uniform sources with
+    Stokeslets and stresslets, velocity + pressure + gradient at the sources,
+    eps 1e-9, host arrays (PCIe-inclusive; the entry point takes host
+    arrays); and the compiled reference (single thread) on an n_ref sample
+    of the same distribution, with this engine's result there compared."""
+    import oracle
+    rng = np.random.default_rng(21)
+    mk = lambda k: (np.asfortranarray(rng.random((2, k))),
+                    *(np.asfortranarray(rng.standard_normal((1, 2, k))) for _ in range(3)))
+    src, f, m, nv = mk(n)
+    eps = 1e-9
+    _stfmm_call(lib, eps, src, f, m, nv, 3)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _stfmm_call(lib, eps, src, f, m, nv, 3)
+    dt = (time.perf_counter() - t0) / steps
+    out = {"workload": f"stfmm2d: ns={n} uniform, Stokeslets+stresslets, ifppreg=3, eps={eps}, "
+                       "host arrays", "mpts_s": round(n / dt / 1e6, 3),
+           "ms_per_solve": round(dt * 1e3, 2)}
+    ref = oracle.ref_lib()
+    if with_cpu and ref is not None:
+        rs, rf, rm, rn = mk(n_ref)
+        a = _stfmm_call(lib, eps, rs, rf, rm, rn, 3)
+        t0 = time.perf_counter()
+        b = _stfmm_call(ref, eps, rs, rf, rm, rn, 3)
+        tr = time.perf_counter() - t0
+        out["reference_1t"] = {"mpts_s": round(n_ref / tr / 1e6, 4), "seconds": round(tr, 2),
+                               "sample": f"ns={n_ref}, same distribution"}
+        out["gpu_vs_reference_rel_l2"] = {k: e3(rel(x, y)) for k, x, y in
+                                          zip(("pot", "pre", "grad"), a, b)}
+        out["gpu_over_reference_1t"] = round(out["mpts_s"] / out["reference_1t"]["mpts_s"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -659,6 +712,8 @@ def main():
         print("C1 done", file=sys.stderr, flush=True)
         out["c3"] = c3_leg(lib, with_cpu=not args.no_cpu)
         print("C3 done", file=sys.stderr, flush=True)
+        out["stokes"] = stokes_leg(lib, with_cpu=not args.no_cpu)
+        print("Stokes done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_batch:
         out["batched_c4"] = batched_c4(lib)

@@ -11,8 +11,7 @@ the mwrap identifier first, read the outputs back.  The functions below
 restate the MATLAB wrappers that a numbl or MATLAB user calls, argument for
 argument: cfmm2d/lfmm2d/rfmm2d/hfmm2d (matlab/{c,l,r,h}fmm2d.m: hndiv2d for
 ndiv/idivflag, then the *_ndiv routine), the direct sums c2ddir/l2ddir/
-r2ddir/h2ddir/st2ddir, and stfmm2d, which the gateway rejects (the library
-has no Stokes FMM).
+r2ddir/h2ddir/st2ddir, and stfmm2d (matlab/stfmm2d.m).
 
 Arrays follow MATLAB's layout: column-major, sources (2, ns), charges
 (nd, ns), dipvec (nd, 2, ns) or (2, ns), and the results are squeezed the
@@ -453,14 +452,25 @@ def st2ddir(srcinfo, targ, ifppregtarg=3):
 
 
 def stfmm2d(eps, srcinfo, ifppreg, targ=None, ifppregtarg=0):
-    """matlab/stfmm2d.m: reaches the gateway, whose stfmm2d_ raises
-    MexError -- the library has no Stokes FMM."""
+    """matlab/stfmm2d.m: Stokeslets (stoklet) and stresslets (strslet with
+    strsvec) to velocity, pressure and velocity gradient at the sources
+    (ifppreg 1/2/3) and at targ (ifppregtarg), through the gateway's
+    stfmm2d_ (the biharmonic FMM of the library).  Output arrays are sized
+    by the flags as the wrapper sizes them; no targets means nt = 0."""
     src = np.asfortranarray(np.asarray(srcinfo["sources"], dtype=float))
     ns, nd = src.shape[1], _nd(srcinfo)
     if targ is None:
-        targ = np.zeros((2, 1), order="F")
-    nt = targ.shape[1] if ifppregtarg else 0
+        nt, ifppregtarg = 0, 0
+        targ = np.zeros((2, 0), order="F")
+    else:
+        targ = np.asfortranarray(np.asarray(targ, dtype=float))
+        nt = targ.shape[1]
     z = lambda m, n: np.zeros((m, n), order="F")
+    ns_pot, ns_pre, ns_grad = (ns if ifppreg >= k else 1 for k in (1, 2, 3))
+    nt_pot, nt_pre, nt_grad = (nt if ifppregtarg >= k else 1 for k in (1, 2, 3))
+    if ifppreg == 0 and ifppregtarg == 0:
+        print("Nothing to compute, set eigher ifppreg or ifppregtarg to 1 or 2 or 3")
+        return Result()
     stoklet = _field(srcinfo, "stoklet", (2 * nd, ns), float)
     strslet = _field(srcinfo, "strslet", (2 * nd, ns), float)
     strsvec = _field(srcinfo, "strsvec", (2 * nd, ns), float)
@@ -470,7 +480,22 @@ def stfmm2d(eps, srcinfo, ifppreg, targ=None, ifppregtarg=0):
         strslet = strsvec = z(2 * nd, 1)
     out = gateway().call(
         "stfmm2d", SPECS["stfmm2d"], nd, eps, ns, src,
-        ifst, stoklet, ifsr, strslet, strsvec, ifppreg, z(2 * nd, ns), z(nd, ns), z(4 * nd, ns),
-        nt, targ, ifppregtarg, z(2 * nd, max(nt, 1)), z(nd, max(nt, 1)), z(4 * nd, max(nt, 1)),
-        0)
-    return out
+        ifst, stoklet, ifsr, strslet, strsvec, ifppreg, z(2 * nd, ns_pot), z(nd, ns_pre),
+        z(4 * nd, ns_grad), nt, targ if nt else z(2, 1), ifppregtarg, z(2 * nd, nt_pot),
+        z(nd, nt_pre), z(4 * nd, nt_grad), 0)
+    pot, pre, grad, pottarg, pretarg, gradtarg, ier = out
+    U = Result(pot=None, pre=None, grad=None, pottarg=None, pretarg=None, gradtarg=None)
+    U.ier = int(np.asarray(ier).ravel()[0])
+    if ifppreg >= 1:
+        U["pot"] = squeeze(pot.reshape((nd, 2, ns), order="F"))
+    if ifppreg >= 2:
+        U["pre"] = pre
+    if ifppreg >= 3:
+        U["grad"] = squeeze(grad.reshape((nd, 2, 2, ns), order="F"))
+    if ifppregtarg >= 1:
+        U["pottarg"] = squeeze(pottarg.reshape((nd, 2, nt), order="F"))
+    if ifppregtarg >= 2:
+        U["pretarg"] = pretarg
+    if ifppregtarg >= 3:
+        U["gradtarg"] = squeeze(gradtarg.reshape((nd, 2, 2, nt), order="F"))
+    return U

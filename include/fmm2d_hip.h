@@ -214,13 +214,60 @@ void h2d_directcdp_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sou
 void h2d_directcdg_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, const double *thresh);
 void h2d_directcdh_(const fmm2d_int *nd, const fmm2d_cplx *zk, const double *sources, const fmm2d_int *ns, const fmm2d_cplx *charge, const fmm2d_cplx *dipstr, const double *dipvec, const double *targ, const fmm2d_int *nt, fmm2d_cplx *pot, fmm2d_cplx *grad, fmm2d_cplx *hess, const double *thresh);
 
+/* ---- biharmonic FMM: replaces bhfmm2d_ (c_translation/src/bhfmm2d.c:924,
+ * include/bhfmm2d.h:22-31).  charge (nd,2,ns) and dip (nd,3,ns) complex;
+ * pot (nd,n) and grad (nd,3,n) complex in the reference's layouts; leaf
+ * size from bhndiv2d, order from bh2dterms.  As in the reference, pg 3
+ * evaluates nothing (pot, grad and hess are written as zeros). */
+void bhfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_int *ns, const double *sources,
+              const fmm2d_int *ifcharge, const fmm2d_cplx *charge, const fmm2d_int *ifdipole,
+              const fmm2d_cplx *dip, fmm2d_int *iper, const fmm2d_int *ifpgh, fmm2d_cplx *pot,
+              fmm2d_cplx *grad, fmm2d_cplx *hess, const fmm2d_int *nt, const double *targ,
+              const fmm2d_int *ifpghtarg, fmm2d_cplx *pottarg, fmm2d_cplx *gradtarg,
+              fmm2d_cplx *hesstarg, fmm2d_int *ier);
+/* replaces stfmm2d_ (stfmm2d.c:40, include/stfmm2d.h:23-32): Stokeslets and
+ * stresslets (nd,2,ns) to velocity (nd,2,n), pressure (nd,n) and velocity
+ * gradient (nd,2,2,n) through the biharmonic FMM; ifppreg 1/2/3. */
+void stfmm2d_(const fmm2d_int *nd, const double *eps, const fmm2d_int *nsource,
+              const double *source, const fmm2d_int *ifstoklet, const double *stoklet,
+              const fmm2d_int *ifstrslet, const double *strslet, const double *strsvec,
+              const fmm2d_int *ifppreg, double *pot, double *pre, double *grad,
+              const fmm2d_int *ntarg, const double *targ, const fmm2d_int *ifppregtarg,
+              double *pottarg, double *pretarg, double *gradtarg, fmm2d_int *ier);
+/* replace bh2dterms_ (bh2dterms.c:32) and bhndiv2d_ (bhndiv2d.c:17) */
+void bh2dterms_(const double *eps, fmm2d_int *nterms, fmm2d_int *ier);
+void bhndiv2d_(const double *eps, const fmm2d_int *ns, const fmm2d_int *nt,
+               const fmm2d_int *ifcharge, const fmm2d_int *ifdipole, const fmm2d_int *ifpgh,
+               const fmm2d_int *ifpghtarg, fmm2d_int *ndiv, fmm2d_int *idivflag);
+/* replace bh2d_direct{c,d,cd}{p,g}_ (bhkernels2d.c:30-398, include/
+ * bhkernels2d.h:34-67): vel (nd,nt) and grad (nd,3,nt) incremented, pairs
+ * with |t - s| <= thresh skipped. */
+void bh2d_directcp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *charges, const double *targ, const fmm2d_int *nt,
+                    fmm2d_cplx *vel, const double *thresh);
+void bh2d_directcg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *charges, const double *targ, const fmm2d_int *nt,
+                    fmm2d_cplx *vel, fmm2d_cplx *grad, const double *thresh);
+void bh2d_directdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *dippar, const double *targ, const fmm2d_int *nt,
+                    fmm2d_cplx *vel, const double *thresh);
+void bh2d_directdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                    const fmm2d_cplx *dippar, const double *targ, const fmm2d_int *nt,
+                    fmm2d_cplx *vel, fmm2d_cplx *grad, const double *thresh);
+void bh2d_directcdp_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                     const fmm2d_cplx *charges, const fmm2d_cplx *dippar, const double *targ,
+                     const fmm2d_int *nt, fmm2d_cplx *vel, const double *thresh);
+void bh2d_directcdg_(const fmm2d_int *nd, const double *sources, const fmm2d_int *ns,
+                     const fmm2d_cplx *charges, const fmm2d_cplx *dippar, const double *targ,
+                     const fmm2d_int *nt, fmm2d_cplx *vel, fmm2d_cplx *grad,
+                     const double *thresh);
+
 /* ---- direct sums, Stokes: replace st2ddirectstokg_ and st2ddirectstokstrsg_
  * (c_translation/src/stokkernels2d.c:11-173, MEX-called through st2ddir,
  * matlab/fmm2d.mw): Stokeslets stoklet (nd,2,ns) and stresslets
  * strslet/strsvec (nd,2,ns) to velocity pot (nd,2,nt), pressure pre (nd,nt)
  * and velocity gradient grad (nd,2,2,nt), all float64 and incremented;
- * pairs with r^2 < thresh^2 skipped.  The Stokes FMM itself (stfmm2d) is
- * not built. */
+ * pairs with r^2 < thresh^2 skipped. */
 void st2ddirectstokg_(const fmm2d_int *nd, const double *sources, const double *stoklet,
                       const fmm2d_int *ns, const double *targ, const fmm2d_int *nt, double *pot,
                       double *pre, double *grad, const double *thresh);

@@ -57,6 +57,11 @@ void o_h2dall(int nterms, cplx z, double rscale, cplx *hvec);
 void o_jbessel(int nterms, cplx z, double rscale, cplx *fjs);
 int  o_h2dterms(double bsize, cplx zk, double eps, int *ier);
 
+/* biharmonic direct sums, bhkernels2d.c:30-398 (obh.c); pg 1 or 2 */
+void oracle_bh2d_direct(int nd, const double *src, int ns, int ifc, const cplx *chg, int ifd,
+                        const cplx *dip, const double *targ, int nt, int pg, cplx *vel,
+                        cplx *grad, double thresh);
+
 /* Stokes direct sums, stokkernels2d.c:11-173 (ostokes.c) */
 void oracle_st2d_direct(int nd, const double *src, int ns, int ifstok, const double *stok,
                         int istress, const double *mu, const double *nu, const double *targ,

@@ -142,3 +142,47 @@ def test_st2ddir_through_gateway(orc):
     p1, q1, g1 = orc_st(orc, 1, np.asfortranarray(s["sources"]), targ, st1, st1, st1,
                         ifst=1, istr=0, th=1e-15)
     assert rel_l2(V.pottarg, p1[0]) < 1e-12 and rel_l2(V.gradtarg, g1[0]) < 1e-12
+
+
+@pytest.mark.parametrize("nd", [1, 2])
+def test_stfmm2d_reference_suite(nd):
+    """matlab/stfmm2dTest.m through the gateway: 4000 sources with
+    Stokeslets and stresslets, 3999 targets, eps 1e-5, ifppreg =
+    ifppregtarg = 3; pot, pre and grad at the first ten sources and targets
+    against st2ddir there, relative l2 over all six below eps."""
+    print(f"mex stfmm2dTest nd={nd}", flush=True)
+    rng = np.random.default_rng(40 + nd)
+    ns, nt, eps, ntest = 4000, 3999, 1e-5, 10
+    sh = (2, ns) if nd == 1 else (nd, 2, ns)
+    s = {"sources": rng.random((2, ns)), "stoklet": rng.random(sh), "strslet": rng.random(sh),
+         "strsvec": rng.random(sh)}
+    if nd > 1:
+        s["nd"] = nd
+    targ = rng.random((2, nt))
+    U1 = mex.stfmm2d(eps, s, 3, targ, 3)
+    assert U1.ier == 0
+    U2 = mex.st2ddir(s, s["sources"][:, :ntest], 3)
+    U3 = mex.st2ddir(s, targ[:, :ntest], 3)
+    first = lambda a: np.asarray(a)[..., :ntest].ravel(order="F")
+    pairs = [(U1.pot, U2.pottarg), (U1.pre, U2.pretarg), (U1.grad, U2.gradtarg),
+             (U1.pottarg, U3.pottarg), (U1.pretarg, U3.pretarg), (U1.gradtarg, U3.gradtarg)]
+    err = sum(np.linalg.norm(first(a) - np.asarray(b).ravel(order="F")) ** 2 for a, b in pairs)
+    ra = sum(np.linalg.norm(np.asarray(b)) ** 2 for _, b in pairs)
+    assert np.sqrt(err / ra) < eps
+
+
+def test_stfmm2d_gateway_matches_library(eng):
+    """The gateway's stfmm2d output equals a direct stfmm2d_ call of the
+    library on the same inputs (sources only, Stokeslets only)."""
+    from bhcalls import stfmm
+    print("mex stfmm2d vs library", flush=True)
+    rng = np.random.default_rng(44)
+    ns = 3000
+    s = {"sources": rng.random((2, ns)), "stoklet": rng.standard_normal((2, ns))}
+    U = mex.stfmm2d(1e-9, s, 2)
+    a = stfmm(eng, 1e-9, 1, np.asfortranarray(s["sources"]),
+              np.asfortranarray(s["stoklet"].reshape(1, 2, ns)), None, None, 2)
+    assert U.ier == 0 and a["ier"] == 0
+    assert np.array_equal(np.asarray(U.pot), a["pot"][0])
+    assert np.array_equal(np.asarray(U.pre).ravel(), a["pre"].ravel())
+    assert U.grad is None and U.pottarg is None

@@ -3,7 +3,7 @@ mex_shim.cpp) linked against libfmm2d_hip.so, without a GPU: every
 identifier the Python driver sends reaches its mwrap stub (a wrong dimension
 gives that stub's "Bad argument size" error before any library call, an
 unknown identifier gives "Unknown identifier"), hndiv2d through the gateway
-equals the library's hndiv2d_ (host-only), the stfmm2d stub raises, and
+equals the library's hndiv2d_ (host-only), stfmm2d's early return, and
 MATLAB's squeeze.  The gateway is built only where the reference's matlab/
 sources are present."""
 import ctypes as C
@@ -56,11 +56,13 @@ def test_hndiv2d_through_gateway(eng, eps, ns, nt):
     assert (int(nd[0, 0]), int(idf[0, 0])) == (a.value, b.value)
 
 
-def test_stfmm2d_is_rejected():
+def test_stfmm2d_nothing_to_compute(capsys):
+    """matlab/stfmm2d.m returns at once when neither flag asks for output."""
     src = {"sources": np.random.default_rng(0).random((2, 10)),
            "stoklet": np.ones((2, 10))}
-    with pytest.raises(mex.MexError, match="Stokes FMM is not provided"):
-        mex.stfmm2d(1e-6, src, 1)
+    U = mex.stfmm2d(1e-6, src, 0)
+    assert dict(U) == {}
+    assert "Nothing to compute" in capsys.readouterr().out
 
 
 def test_squeeze_matches_matlab():
