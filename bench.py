@@ -458,15 +458,52 @@ def batched_c4(lib, nworkers=8):
             raise RuntimeError(f"batched device solve: {nbad} problems failed")
         out[label + "_device"] = {"mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4),
                                   "data": "device arrays (resident in HBM)"}
-        del keep
         print(f"batched {label}: host arrays {out[label]['seconds']} s, device arrays "
               f"{dt:.3f} s", file=sys.stderr, flush=True)
+        # fused: each group of compatible problems as one forest (one launch
+        # sequence per group; here one Cauchy and one Helmholtz group)
+        lib.fmm2dhip_batch_fused.restype = C.c_int
+        if lib.fmm2dhip_batch_fused(len(specs), arr, nworkers):  # workspace sized once
+            raise RuntimeError("fused device solve failed")
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            nbad = lib.fmm2dhip_batch_fused(len(specs), arr, nworkers)
+            ts.append(time.perf_counter() - t0)
+            if nbad:
+                raise RuntimeError(f"fused device solve: {nbad} problems failed")
+        dt = min(ts)
+        out[label + "_fused_device"] = {
+            "mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4),
+            "seconds_all": [round(t, 4) for t in ts],
+            "data": "device arrays (resident in HBM), fmm2dhip_batch_fused"}
+        del keep
+        probs = [serve.Problem(precision=mode, **s) for s in specs]
+        t0 = time.perf_counter()
+        nbad = serve.batch_fused(lib, probs, nworkers)
+        dt = time.perf_counter() - t0
+        if nbad:
+            raise RuntimeError(f"fused solve: {nbad} problems failed")
+        results[label + "_fused"] = probs
+        out[label + "_fused"] = {"mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4),
+                                 "data": "host arrays (PCIe inside), fmm2dhip_batch_fused"}
+        print(f"fused {label}: device arrays {out[label + '_fused_device']['seconds']} s, "
+              f"host arrays {dt:.3f} s", file=sys.stderr, flush=True)
     worst = 0.0
     for a, b in zip(results["fp64"], results["mixed"]):
         ra, rb = a.result(), b.result()
         for k in ("pot", "grad"):
             worst = max(worst, rel(rb[k], ra[k]))
     out["mixed_max_rel_l2_diff_vs_fp64"] = e3(worst)
+    # fused against the pool's solves (the same problems solved alone):
+    # Laplace to rounding, Helmholtz within 2 eps (rescaled, enlarged root)
+    worst = {}
+    for s, a, b in zip(specs, results["fp64"], results["fp64_fused"]):
+        ra, rb = a.result(), b.result()
+        key = "helmholtz" if s["kernel"] == serve.HELMHOLTZ else "cauchy"
+        for k in ("pot", "grad"):
+            worst[key] = max(worst.get(key, 0.0), rel(rb[k], ra[k]))
+    out["fused_max_rel_l2_diff_vs_alone_fp64"] = {k: e3(v) for k, v in worst.items()}
     t0 = time.perf_counter()
     for s in specs:
         if s["kernel"] == serve.HELMHOLTZ:

@@ -117,6 +117,7 @@ struct LapReq {
   bool device = false;
   hipStream_t stream = nullptr;
   double* timeinfo = nullptr;
+  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
 };
 
 template <class T>
@@ -157,7 +158,8 @@ int run_lap(const LapReq& r) {
   }
   SortedPoints P;
   Plan plan;
-  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt);
+  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt,
+                 r.forest);
   if (e) return e;
   const int ndc = r.logk ? 2 * nd : nd;
   LapArgs a;
@@ -283,6 +285,7 @@ struct HelmReq {
   bool device = false;
   hipStream_t stream = nullptr;
   double* timeinfo = nullptr;
+  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
 };
 
 int run_helm(const HelmReq& r) {
@@ -305,7 +308,8 @@ int run_helm(const HelmReq& r) {
   }
   SortedPoints P;
   Plan plan;
-  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt);
+  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt,
+                 r.forest);
   if (e) return e;
   HelmArgs a;
   a.nd = nd; a.eps = r.eps; a.zk = r.zk;
@@ -394,8 +398,9 @@ void zero8(double* t) {
 }  // namespace
 
 // One problem of the batched/served interface on the calling thread, with
-// the problem's precision mode for the duration of the solve.
-int solve_problem(fmm2dhip_problem& p) {
+// the problem's precision mode for the duration of the solve; with a
+// forest, the problem is the concatenation of the forest's problems.
+int solve_problem(fmm2dhip_problem& p, const ForestIn* forest) {
   const double t0 = wall();
   int& mode = thread_precision();
   const int saved = mode;
@@ -410,7 +415,7 @@ int solve_problem(fmm2dhip_problem& p) {
     r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
     r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
     r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
-    r.ndiv = ndiv; r.device = p.device_ptrs == 1;
+    r.ndiv = ndiv; r.device = p.device_ptrs == 1; r.forest = forest;
     e = guarded_h(r);
   } else if (p.kernel == FMM2DHIP_CAUCHY || p.kernel == FMM2DHIP_LOG ||
              p.kernel == FMM2DHIP_REAL) {
@@ -422,7 +427,7 @@ int solve_problem(fmm2dhip_problem& p) {
     r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
     r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
     r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
-    r.ndiv = ndiv; r.device = p.device_ptrs == 1;
+    r.ndiv = ndiv; r.device = p.device_ptrs == 1; r.forest = forest;
     e = guarded(r);
   } else {
     e = IER_ARGS;

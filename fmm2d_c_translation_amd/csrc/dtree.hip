@@ -166,34 +166,57 @@ __global__ void k_pad_tail(uint64_t* a, const int* tot_s, const int* tot_t, long
 // neighbours' parents must then be split at k-2, and so on: at each coarser
 // level the forced boxes form a rectangle of at most 3 x 3 cells.  Written
 // as (level << 2L | key) for a tree of depth L, 9 slots per level below k
-// (short rectangles repeat their first cell); the adaptive split boxes
-// themselves come first, already level-tagged.
+// and above mlo (short rectangles repeat their first cell); the adaptive
+// split boxes themselves come first, already level-tagged.  The domain is
+// the unit square (mlo = 0), or in a forest the split box's problem root
+// (cell (rx, ry) of level m >= mlo): there only boxes at least two levels
+// below the root force anything, and only down to the level below the
+// root, as the problem's own tree would; unused slots repeat the split key.
 __global__ void k_closure_cand(const uint64_t* R, const int* roff, const long* cbase, int L,
-                               uint64_t* out) {
+                               ForestDev F, int mlo, uint64_t* out) {
   const int kLevShift = 2 * L;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= roff[L + 1]) return;
   const int k = (int)(R[r] >> kLevShift);  // R: (level << 2L) | key, sorted
   const uint64_t key = R[r] & ((uint64_t(1) << kLevShift) - 1);
   out[r] = R[r];
-  if (k < 2) return;
-  uint64_t* o = out + cbase[k] + (long)(r - roff[k]) * 9 * (k - 1);
+  if (k < mlo + 2) return;
+  uint64_t* o = out + cbase[k] + (long)(r - roff[k]) * 9 * (k - 1 - mlo);
+  int m = 0, rx = 0, ry = 0;
+  if (F.M >= 0) {
+    ForestSlot s;
+    if (!forest_slot(F, key, k, s) || k - s.m < 2) {
+      for (long q = 0; q < 9L * (k - 1 - mlo); ++q) o[q] = R[r];
+      return;
+    }
+    m = s.m;
+    rx = s.rx;
+    ry = s.ry;
+  }
   int ix, iy;
   key_cell(key, ix, iy);
-  int side = 1 << k;
-  int ax = max(ix - 1, 0), bx = min(ix + 1, side - 1);
-  int ay = max(iy - 1, 0), by = min(iy + 1, side - 1);
-  for (int kk = k - 1; kk >= 1; --kk) {
+  // the domain's cells at level k: [x0, x1] x [y0, y1]
+  int x0 = rx << (k - m), x1 = ((rx + 1) << (k - m)) - 1;
+  int y0 = ry << (k - m), y1 = ((ry + 1) << (k - m)) - 1;
+  int ax = max(ix - 1, x0), bx = min(ix + 1, x1);
+  int ay = max(iy - 1, y0), by = min(iy + 1, y1);
+  for (int kk = k - 1; kk >= mlo + 1; --kk) {
+    if (kk <= m) {  // at or above the root: nothing forced
+      for (int q = 0; q < 9; ++q) o[q] = R[r];
+      o += 9;
+      continue;
+    }
     ax >>= 1; bx >>= 1; ay >>= 1; by >>= 1;
     const uint64_t lev = (uint64_t)kk << kLevShift;
-    int m = 0;
+    int n = 0;
     for (int y = ay; y <= by && y <= ay + 2; ++y)
-      for (int x = ax; x <= bx && x <= ax + 2; ++x) o[m++] = lev | cell_key(x, y);
-    for (; m < 9; ++m) o[m] = lev | cell_key(ax, ay);
+      for (int x = ax; x <= bx && x <= ax + 2; ++x) o[n++] = lev | cell_key(x, y);
+    for (; n < 9; ++n) o[n] = lev | cell_key(ax, ay);
     o += 9;
-    side = 1 << kk;
-    ax = max(ax - 1, 0); bx = min(bx + 1, side - 1);
-    ay = max(ay - 1, 0); by = min(by + 1, side - 1);
+    x0 = rx << (kk - m); x1 = ((rx + 1) << (kk - m)) - 1;
+    y0 = ry << (kk - m); y1 = ((ry + 1) << (kk - m)) - 1;
+    ax = max(ax - 1, x0); bx = min(bx + 1, x1);
+    ay = max(ay - 1, y0); by = min(by + 1, y1);
   }
 }
 
@@ -240,7 +263,8 @@ __global__ void k_box_keys(Levels lv, const uint64_t* R, int nb, uint64_t* key) 
 
 // per box: level, centre, children (and their parent link)
 __global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int nb, int* level,
-                        double2* ctr, int* nchild, int4* child, int* parent, double* bs_out) {
+                        double2* ctr, int* nchild, int4* child, int* parent, double* bs_out,
+                        double* bsb) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0)
     for (int l = 0; l <= lv.L; ++l) bs_out[l] = lv.bs[l];
@@ -248,6 +272,7 @@ __global__ void k_boxes(Levels lv, const uint64_t* R, const uint64_t* key, int n
   const int l = level_of(lv, b);
   const uint64_t k = key[b];
   level[b] = l;
+  bsb[b] = lv.bs[l];
   double cx = lv.c0x, cy = lv.c0y;
   for (int j = 1; j <= l; ++j) {
     int q = (int)(k >> (2 * (l - j))) & 3;
@@ -410,11 +435,21 @@ struct WlArgs {
   int nb, pg, pgt;
   const double2* ctr;  // box centres
   const double* bs;    // box size per level
+  const uint64_t* key;
+  ForestDev F;
 };
 
+// In a forest, a box's level in its own problem's tree takes the place of
+// its level: expansions (formation, M2M, M2L, L2L, list 4) start two
+// levels below the root as they start at level 2 alone, and the near field
+// runs over leaves only (above the roots, where no level restriction
+// applies, a split box can have a coarse empty leaf in its list 1; below
+// them, as alone, level restriction leaves a split box no list-1 entry).
 __device__ inline unsigned wl_mask(const WlArgs& a, int b) {
   if (b >= a.nb) return 0;
   int l = a.level[b];
+  const bool fo = a.F.M >= 0;
+  if (fo) l = forest_plev(a.F, a.key[b], l);
   bool leaf = a.nchild[b] == 0;
   int2 s = a.srng[b], t = a.trng[b];
   int ns = s.y - s.x, nt = t.y - t.x;
@@ -422,16 +457,16 @@ __device__ inline unsigned wl_mask(const WlArgs& a, int b) {
   int4 n = a.cnt[b];
   unsigned m = 0;
   m |= (unsigned)(l >= 2 && leaf && ns > 0) << WL_FORMMP;
-  m |= (unsigned)(!leaf && ns > 0) << WL_M2M;
+  m |= (unsigned)(!leaf && ns > 0 && (!fo || l >= 2)) << WL_M2M;
   m |= (unsigned)(l >= 2 && np > 0 && n.y > 0) << WL_M2L;
-  m |= (unsigned)(np > 0 && !leaf) << WL_L2L;
+  m |= (unsigned)(np > 0 && !leaf && (!fo || l >= 2)) << WL_L2L;
   m |= (unsigned)(l >= 2 && np > 0 && n.w > 0) << WL_LIST4;
   m |= (unsigned)(n.z > 0 && ns > 0) << WL_LIST3_S;
   m |= (unsigned)(n.z > 0 && nt > 0) << WL_LIST3_T;
   m |= (unsigned)(leaf && ns > 0) << WL_LEAVES_S;
   m |= (unsigned)(leaf && nt > 0) << WL_LEAVES_T;
-  m |= (unsigned)(n.x > 0 && ns > 0) << WL_P2P_S;
-  m |= (unsigned)(n.x > 0 && nt > 0) << WL_P2P_T;
+  m |= (unsigned)(n.x > 0 && ns > 0 && (!fo || leaf)) << WL_P2P_S;
+  m |= (unsigned)(n.x > 0 && nt > 0 && (!fo || leaf)) << WL_P2P_T;
   return m;
 }
 
@@ -496,6 +531,10 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
     for (int w = 0; w < WLB / 64; ++w) t += sp[threadIdx.x][w];
     if (t != 0) atomicAdd(&pairs[threadIdx.x], t);
   }
+  // per level of the box (pairs[5 + l]), then over all levels (pairs[4])
+  if (r2max > 0.0)
+    atomicMax(reinterpret_cast<unsigned long long*>(pairs + 5 + a.level[b]),
+              (unsigned long long)__double_as_longlong(r2max));
   for (int w = 32; w >= 1; w >>= 1) r2max = fmax(r2max, __shfl_xor(r2max, w));
   if (lane == 0 && r2max > 0.0)
     atomicMax(reinterpret_cast<unsigned long long*>(pairs + 4),
@@ -660,7 +699,8 @@ int sync_read(Ctx& c, const int* d, int n, int* h, const uint32_t* sort_err) {
 
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
                 const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
-                int sorted_levels, Plan& plan, double* host_s) {
+                int sorted_levels, Plan& plan, double* host_s, const ForestDev* forest,
+                int forest_mlo) {
   double t0 = wall();
   // ---- 0. depth of the adaptive tree (one read-back of per-block maxima)
   const bool depth_s = idivflag != 1 && ns > ndiv;
@@ -773,6 +813,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   double t1 = wall();
 
   // ---- 2. level restriction: closure of the split sets in one sort
+  // (in a forest nothing is forced at or above the highest root, level mlo)
+  const int mlo = forest && forest->M >= 0 ? forest_mlo : 0;
   uint64_t* Rall = nullptr;
   {
     std::vector<int> roff(L + 2, 0);
@@ -782,7 +824,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     nc = roff[L + 1];
     for (int l = 0; l <= L; ++l) {
       cbase[l] = nc;
-      if (l >= 2) nc += 9L * (l - 1) * nR[l];
+      if (l >= mlo + 2) nc += 9L * (l - 1 - mlo) * nR[l];
     }
     long* d_cbase = S.get<long>(L + 1);
     uint64_t* cand = S.get<uint64_t>(std::max<long>(nc, 1));
@@ -797,7 +839,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     HIP_TRY(hipMemcpyAsync(d_cbase, hcb, sizeof(long) * (L + 1), hipMemcpyHostToDevice, c.st));
     // cand[0 .. roff[L]): the split keys (level-tagged), then the forced boxes
     if (roff[L] > 0)
-      k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(Rs, d_roff, d_cbase, L, cand);
+      k_closure_cand<<<nblk(roff[L], 256), 256, 0, c.st>>>(
+          Rs, d_roff, d_cbase, L, forest ? *forest : ForestDev{}, mlo, cand);
     size_t tb2 = 0;
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, Rall, d_nu, (int)nc, c.st));
     void* tmp = S.get<char>(std::max(tb2, size_t(1)));
@@ -848,13 +891,14 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   int* parent = S.get<int>(nb);
   int* nbr = S.get<int>((size_t)9 * nb);
   double* dbs = S.get<double>(L + 1);
+  double* bsb = S.get<double>(nb);
   int4* cnt = S.get<int4>(nb + 1);
   int4* off = S.get<int4>(nb + 1);
   int* o = S.get<int>(4 * ((size_t)nb + 1));
   if (S.err) return S.err;
   k_box_keys<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, nb, key);
   k_boxes<<<nblk(nb, 256), 256, 0, c.st>>>(lv, Rall, key, nb, level, ctr, nchild, child, parent,
-                                          dbs);
+                                          dbs, bsb);
   k_box_search<<<dim3(nblk(nb, 256), kBoxSearches), 256, 0, c.st>>>(
       lv, key, nb, ks, ns, kt, nt, reinterpret_cast<int*>(srng), reinterpret_cast<int*>(trng), nbr);
   // ---- lists 1-4 (count, scan, fill)
@@ -891,8 +935,9 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   const int ninfo = 2 * WL_N + 1;
   // small results read back in one copy: info, level offsets, m2l pairs
   // (as 2 ints each), list-1 pairs, source reads and the squared list-1
-  // distance bound (5 doubles as 10 ints)
-  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 10;
+  // distance bound, over all levels and per level (5 + kMaxDepth + 1
+  // doubles as ints)
+  const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 10 + 2 * (kMaxDepth + 1);
   int* small = S.get<int>(nsmall + 2);
   int* bcnt = S.get<int>((size_t)WL_N * nbk);
   if (S.err) return S.err;
@@ -901,7 +946,8 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   unsigned long long* m2l_lev = reinterpret_cast<unsigned long long*>(small + ((ninfo + WL_N * W + 1) & ~1));
   double* pairs = reinterpret_cast<double*>(m2l_lev + kMaxDepth + 1);
   HIP_TRY(hipMemsetAsync(small, 0, sizeof(int) * (nsmall + 2), c.st));
-  WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg, ctr, dbs};
+  WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg, ctr, dbs, key,
+            forest ? *forest : ForestDev{}};
   k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, bcnt, nbk, pairs, m2l_lev);
   k_wl_scan<<<1, 64 * WL_N, 0, c.st>>>(bcnt, nbk, info);
   // every list holds at most nb entries; the L2L children at most 4 nb
@@ -989,6 +1035,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   D.ctr = ctr;
   D.level = level;
   D.bs = dbs;
+  D.bsb = bsb;
   D.nchild = nchild;
   D.parent = parent;
   D.child = child;
@@ -1012,6 +1059,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   st.p2p_reads_src = hp[2];
   st.p2p_reads_trg = hp[3];
   plan.p2p_rmax = hp[4] >= 0.0 && hp[4] < 1e300 ? std::sqrt(hp[4]) : 0.0;
+  plan.p2p_rmax_lev.assign(L + 1, 0.0);
+  for (int l = 0; l <= L; ++l)
+    plan.p2p_rmax_lev[l] = hp[5 + l] >= 0.0 && hp[5 + l] < 1e300 ? std::sqrt(hp[5 + l]) : 0.0;
+  plan.forest = forest ? *forest : ForestDev{};
   st.nboxes = nb;
   st.nlevels = L;
   double t3 = wall();

@@ -74,6 +74,60 @@ int h2d_direct_cdg(int nd, const fmm2d_cplx* zk, const double* s, int ns, const 
                    const fmm2d_cplx* q, const double* v, const double* t, int nt,
                    fmm2d_cplx* pot, fmm2d_cplx* grad, double th);
 
+// A forest: independent problems solved as one tree.  Problem i's root box
+// is the cell (rx, ry) of level m of the unit square, inside a slot (a cell
+// of level M) that holds nothing else; the root sits one cell in from the
+// slot's corner, so the eight cells around it at level m hold no points.
+// Below its root a problem's tree is the one it has alone (its keys are
+// its own, prefixed by the root cell's), and the cells around the roots
+// keep every list of a box below a root inside that problem; the boxes at
+// or above the roots do no FMM work.
+struct ForestSlot {
+  int m, rx, ry, prob;  // m < 0: an empty slot
+};
+struct ForestDev {
+  int M = -1;                        // slot level; -1: not a forest
+  const ForestSlot* slot = nullptr;  // 4^M entries, by the Morton index of the level-M cell
+};
+// The slot of box (key, level l), false when the box lies above the slots
+// or in an empty one.
+FMM_HD bool forest_slot(const ForestDev& F, uint64_t key, int l, ForestSlot& s) {
+  if (l < F.M) return false;
+  s = F.slot[key >> (2 * (l - F.M))];
+  return s.m >= 0;
+}
+// Level of box (key, l) in its own problem's tree: l outside a forest, -1
+// for a box above every root or in an empty slot.
+FMM_HD int forest_plev(const ForestDev& F, uint64_t key, int l) {
+  if (F.M < 0) return l;
+  ForestSlot s;
+  if (!forest_slot(F, key, l, s) || l < s.m) return -1;
+  return l - s.m;
+}
+// One problem of a forest (device copy read by the key and frame kernels).
+struct ForestProb {
+  double cx, cy, R;     // root box in the problem's own coordinates (keys)
+  double a;             // normalised coordinates (x - cx) a; a = 0: none
+  double fcx, fcy, fR;  // root box in the coordinates the kernels see
+  int m, rx, ry;        // root cell (rx, ry) of level m
+  int soff, toff;       // first source / target in the concatenated arrays
+};
+// Forest input of the point front end (build_plan).
+struct ForestIn {
+  int np = 0;
+  const ForestProb* prob = nullptr;  // device, np entries
+  const int* soff = nullptr;         // device, np + 1 prefix offsets of the sources
+  const int* toff = nullptr;         // and of the targets
+  bool normalise = false;  // the kernels see normalised coordinates (Helmholtz)
+  // box centres and sizes in each problem's own kernel coordinates, so
+  // that point-to-centre and point-to-point differences keep their relative
+  // precision whatever the problem's place in the unit square
+  bool frames = false;
+  double min_bs0 = 0;      // smallest root box, kernel coordinates (the near field's skip threshold)
+  int lfirst = 2;          // 2 below the highest root (the smallest m)
+  ForestDev dev;
+};
+
 // Tree arrays on the device (0-based box ids, lists in CSR form).
 struct DTree {
   int nb = 0;
@@ -81,6 +135,7 @@ struct DTree {
   const double2* ctr = nullptr;
   const int* level = nullptr;
   const double* bs = nullptr;    // box size per level
+  const double* bsb = nullptr;   // box size per box (in a forest, in its problem's coordinates)
   const int* nchild = nullptr;
   const int* parent = nullptr;
   const int4* child = nullptr;
@@ -135,6 +190,12 @@ struct Plan {
   // a box with sources in its list 1 (from box centres and sizes): bounds
   // r over every near-field pair
   double p2p_rmax = 0;
+  // the same bound per level of the target leaves (0: no list-1 pair there)
+  std::vector<double> p2p_rmax_lev;
+  // first level whose boxes can take part in M2L, list 3/4 or carry
+  // expansions: 2, or in a forest 2 below the highest problem root
+  int lfirst = 2;
+  ForestDev forest;
   // pinned host word, cleared before the front end's sorts and set by a sort
   // whose look-back wait gave up; read after the stream synchronises
   uint32_t* sort_err = nullptr;
@@ -153,7 +214,8 @@ constexpr int kSortLevels = 15;  // with the target flag, 31 bits: four 8-bit pa
 constexpr int IER_RESORT = -1;  // internal: sort on more key bits and retry
 int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* kt, int nt,
                 const double* geo, int idivflag, int ndiv, int ifpgh, int ifpghtarg,
-                int sorted_levels, Plan& plan, double* host_s);
+                int sorted_levels, Plan& plan, double* host_s,
+                const ForestDev* forest = nullptr, int forest_mlo = 0);
 
 // Pinned host staging reused across the calls of one thread (slot < 8).
 void* pinned(size_t bytes, int slot);
@@ -205,9 +267,11 @@ SolveStats& last_stats();
 
 
 // Point front end: bounding box, keys, sort, tree, lists, sorted coordinates.
+// With a forest (fin), the points are the concatenated points of its
+// problems, keyed below their root cells; the root box is the unit square.
 int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_trg, int nt,
                int idivflag, int ndiv, SortedPoints& P, Plan& plan, bool want_lap_work,
-               int ifpgh, int ifpghtarg);
+               int ifpgh, int ifpghtarg, const ForestIn* fin = nullptr);
 
 // Gathers (nd-fastest, original order) strengths into channel-major sorted
 // order: out[(c*nd + d)*n + i] = in[(perm[i]*ncomp + c)*nd + d].
@@ -307,7 +371,19 @@ int l2dterms(double eps);
 int ndiv_rule(double eps, int ns, int nt);
 
 // One problem of the batched/served interface, solved on the calling
-// thread (abi.hip); writes p.ier and p.seconds, returns p.ier.
-int solve_problem(fmm2dhip_problem& p);
+// thread (abi.hip); writes p.ier and p.seconds, returns p.ier.  With a
+// forest, p holds the concatenated device arrays of the forest's problems.
+int solve_problem(fmm2dhip_problem& p, const ForestIn* forest = nullptr);
+
+// Fused batches (forest.hip).  fused_groups: the groups of at least two
+// problems of probs[0..n) that can be solved together as one forest (same
+// kernel, flags, channels, eps, leaf size, precision and memory kind, more
+// sources than the leaf size, a real positive Helmholtz wavenumber), as
+// indices; every other problem is in no group.  solve_forest: solves one
+// group on the calling thread (a member whose root box turns out unfit for
+// the forest is solved alone); writes each member's ier and seconds and
+// returns the number of members whose ier is not 0.
+std::vector<std::vector<int>> fused_groups(int n, const fmm2dhip_problem* probs);
+int solve_forest(fmm2dhip_problem* probs, const std::vector<int>& group);
 
 }  // namespace fmm

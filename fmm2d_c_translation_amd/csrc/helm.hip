@@ -402,7 +402,7 @@ k_hform_g(const int* work, int nwork, DTree D, const HLev* lev, double k, const 
           emit(tm * zc);
         };
         const double x = k * r;
-        if (!MP) {
+        if constexpr (!MP) {
           // H_n rs^n upward (hvals_real); every list-4 source is off centre
           const H01 hh = hank_real_point<(NR > 0 ? NR : 1)>(x);
           const double zr = rs / x, s2 = rs * rs;
@@ -1284,36 +1284,40 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   // per-level order and scale (hfmm2d.c:1768-1779)
   std::vector<HLev> lev(L + 1);
   std::vector<char> gate(L + 2, 0);
+  // the first level with expansions: 2, or in a forest 2 below the highest
+  // problem root (coarser levels are shared cells above the roots)
+  const int lf = plan.lfirst;
   int pmax = 0;
   for (int l = 0; l <= L; ++l) {
     std::complex<double> zt = a.zk * bs[l] / (2.0 * kPi);
     lev[l].rs = std::min(std::abs(zt), 1.0);
     int e = 0;
-    lev[l].p = h2dterms_cached(bs[l], a.zk, a.eps, &e);
+    // (in a forest, the shared cells above the roots need no order)
+    lev[l].p = l >= lf || plan.forest.M < 0 ? h2dterms_cached(bs[l], a.zk, a.eps, &e) : 0;
     gate[l] = zi * bs[l] < 40.0;
     // levels 0 and 1 take part in no M2L, list 3 or list 4 interaction, so
     // their expansions stay zero and are neither formed nor stored
-    if (gate[l] && l >= 2) pmax = std::max(pmax, lev[l].p);
+    if (gate[l] && l >= lf) pmax = std::max(pmax, lev[l].p);
   }
   if (pmax > 2000) return IER_P_TOO_LARGE;
   // Levels whose boxes exceed 16 wavelengths translate in the diagonal form
   // (hfmm2d.c:946-960: M2M into them, M2L at them and L2L out of them;
   // levels 0 and 1 form no expansion that any step reads).
   std::vector<char> hf(L + 2, 0);
-  for (int l = 2; l <= L; ++l)
+  for (int l = lf; l <= L; ++l)
     hf[l] = gate[l] && bs[l] * a.zk.real() / (2 * kPi) > 16.0;
   // the largest order the per-point steps meet: formation, list 4 and local
   // evaluation at their own level, list 3 at the level below (the orders of
   // the diagonal-form levels are far above those of the leaves)
   int pmax_pt = 0;
   for (int l = 0; l <= L; ++l) {
-    if (!gate[l] || l < 2) continue;
+    if (!gate[l] || l < lf) continue;
     if (plan.formmp.at(l).n || plan.list4.at(l).n || plan.leaves_s.at(l).n ||
         plan.leaves_t.at(l).n)
       pmax_pt = std::max(pmax_pt, lev[l].p);
   }
   for (int l = 0; l < L; ++l)
-    if (gate[l + 1] && l + 1 >= 2 && (plan.list3_s.at(l).n || plan.list3_t.at(l).n))
+    if (gate[l + 1] && l + 1 >= lf && (plan.list3_s.at(l).n || plan.list3_t.at(l).n))
       pmax_pt = std::max(pmax_pt, lev[l + 1].p);
   const double2* chg = a.ifcharge ? a.chg : nullptr;
   const double2* dip = a.ifdipole ? a.dip : nullptr;
@@ -1335,7 +1339,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   int vmax = 0;
   for (int l = 0; l <= L; ++l) {
     if (!gate[l]) continue;
-    if (l >= 2 && l <= L - 1 && !hf[l]) {
+    if (l >= lf && l <= L - 1 && !hf[l]) {
       double h = bs[l + 1] / 2;
       size_t sz1 = (size_t)(2 * lev[l + 1].p + 1) * (2 * lev[l].p + 1);
       size_t base_m = toff, base_l = toff + 4 * sz1;
@@ -1351,7 +1355,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       tab[l].l2l[0] = base_l;
       vmax = std::max(vmax, lev[l].p + lev[l + 1].p + 4);
     }
-    if (l >= 2 && !hf[l]) {
+    if (l >= lf && !hf[l]) {
       size_t sz = (size_t)(2 * lev[l].p + 1) * (2 * lev[l].p + 1);
       size_t base = toff;
       toff += 49 * sz;  // near slots stay unused (no list-2 pair maps to them)
@@ -1424,7 +1428,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   {
     SolveStats& st = last_stats();
     st.host_s[6] = wall() - t_setup;
-    for (int l = 2; l <= L; ++l) {
+    for (int l = lf; l <= L; ++l) {
       if (!gate[l]) continue;
       double np = plan.m2l_pairs[l];
       double nc = 2 * lev[l].p + 1;
@@ -1460,7 +1464,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     k_map_l2l<<<nblk(plan.l2l_kids.n, 128), 128, 0, c.st>>>(W + plan.l2l_kids.off,
                                                              plan.l2l_kids.n, D.parent, D.child,
                                                              map_l2l);
-  for (int l = L - 1; l >= 2; --l) {
+  for (int l = L - 1; l >= lf; --l) {
     const Span sp = plan.m2m.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* map = map_m2m + (size_t)(sp.off - plan.m2m.off) * 4;
@@ -1518,7 +1522,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
                                                                 map2);
   }
   std::vector<M2LLevel> m2l_levels;
-  for (int l = 2; l <= L; ++l) {
+  for (int l = lf; l <= L; ++l) {
     const Span sp = plan.m2l.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
@@ -1544,7 +1548,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);
   // step 5: L2L
-  for (int l = 2; l <= L - 1; ++l) {
+  for (int l = lf; l <= L - 1; ++l) {
     const Span sp = plan.l2l_kids.at(l);
     if (!sp.n || !gate[l]) continue;
     const int* map = map_l2l + (size_t)(sp.off - plan.l2l_kids.off) * 4;

@@ -296,6 +296,63 @@ __global__ void k_keys(const double2* s, int ns, const double2* t, int nt, const
   key[i] = point_key(q.x, q.y, geo[0], geo[1], geo[2]) | (tg ? kTargetFlag : 0);
   idx[i] = tg ? i - ns : i;
 }
+// Keys of a forest's points: a point of problem p takes the path key of its
+// own root box (cx, cy, R) -- the key it has alone -- below the Morton key
+// of p's root cell (rx, ry) at level m; with a != 0 the point's normalised
+// coordinates (x - cx) a go to norm (indexed as the input).
+__device__ inline int forest_prob_of(const int* off, int np, int i) {
+  int lo = 0, hi = np;  // largest p with off[p] <= i
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__global__ void k_keys_forest(const double2* s, int ns, const double2* t, int nt,
+                              const ForestProb* prob, const int* soff, const int* toff, int np,
+                              uint64_t* key, int* idx, double2* s_norm, double2* t_norm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns + nt) return;
+  const bool tg = i >= ns;
+  const int j = tg ? i - ns : i;
+  const ForestProb P = prob[forest_prob_of(tg ? toff : soff, np, j)];
+  const double2 q = tg ? t[j] : s[j];
+  const uint64_t local = point_key(q.x, q.y, P.cx, P.cy, P.R);
+  uint64_t root = 0;
+  for (int b = 0; b < P.m; ++b)
+    root |= ((uint64_t)((P.rx >> b) & 1) << (2 * b)) | ((uint64_t)((P.ry >> b) & 1) << (2 * b + 1));
+  key[i] = (root << (2 * (kMaxDepth - P.m))) | (local >> (2 * P.m)) | (tg ? kTargetFlag : 0);
+  idx[i] = j;
+  double2* norm = tg ? t_norm : s_norm;
+  if (norm && P.a != 0.0) norm[j] = make_double2((q.x - P.cx) * P.a, (q.y - P.cy) * P.a);
+}
+
+// Box centres and sizes of a forest in each problem's own kernel
+// coordinates: for a box at or below its problem's root, the reference's
+// centre recurrence from the problem's root box there (fcx, fcy, fR: the
+// centre and size the box has when the problem is solved alone, or its
+// normalised image); other boxes keep the unit square's.
+__global__ void k_forest_frames(int nb, const uint64_t* key, const int* level, ForestDev F,
+                                const ForestProb* prob, double2* ctr, double* bsb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int l = level[b];
+  const uint64_t k = key[b];
+  ForestSlot sl;
+  if (!forest_slot(F, k, l, sl) || l < sl.m) return;
+  const ForestProb P = prob[sl.prob];
+  double cx = P.fcx, cy = P.fcy, bs = P.fR;
+  for (int j = sl.m + 1; j <= l; ++j) {
+    const int q = (int)(k >> (2 * (l - j))) & 3;
+    bs = bs / 2;
+    cx = child_coord(cx, q & 1, bs);
+    cy = child_coord(cy, q & 2, bs);
+  }
+  ctr[b] = make_double2(cx, cy);
+  bsb[b] = bs;
+}
+
 __global__ void k_unflag(uint64_t* key, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) key[i] &= kTargetFlag - 1;
@@ -403,7 +460,7 @@ inline int nblk(int n, int b) { return (n + b - 1) / b; }
 // *kt are the sorted keys of each set (on the device).
 int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* trg, int nt,
                 const double* geo, int begin_bit, uint32_t* err, int* perm, uint64_t** ks,
-                uint64_t** kt) {
+                uint64_t** kt, const ForestIn* fin, double2* s_norm, double2* t_norm) {
   const int n = ns + nt;
   uint64_t* k0 = S.get<uint64_t>(n);
   uint64_t* k1 = S.get<uint64_t>(n);
@@ -412,7 +469,11 @@ int sort_points(Ctx& c, Scratch& S, const double2* src, int ns, const double2* t
   *ks = k1;
   *kt = k1 + ns;
   if (n == 0) return IER_OK;
-  k_keys<<<nblk(n, 256), 256, 0, c.st>>>(src, ns, trg, nt, geo, k0, i0);
+  if (fin)
+    k_keys_forest<<<nblk(n, 256), 256, 0, c.st>>>(src, ns, trg, nt, fin->prob, fin->soff,
+                                                  fin->toff, fin->np, k0, i0, s_norm, t_norm);
+  else
+    k_keys<<<nblk(n, 256), 256, 0, c.st>>>(src, ns, trg, nt, geo, k0, i0);
   int e = rsort::radix_sort(c, S, k0, k1, i0, perm, n, begin_bit, 2 * kMaxDepth + 1, err);
   if (e) return e;
   if (nt) k_unflag<<<nblk(nt, 256), 256, 0, c.st>>>(k1 + ns, nt);
@@ -441,7 +502,7 @@ void scatter_channels(hipStream_t st, const double2* in, double2* out, const int
 
 int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_trg, int nt,
                int idivflag, int ndiv, SortedPoints& P, Plan& plan, bool want_lap_work,
-               int ifpgh, int ifpghtarg) {
+               int ifpgh, int ifpghtarg, const ForestIn* fin) {
   (void)want_lap_work;
   const double2* src = reinterpret_cast<const double2*>(d_src);
   const double2* trg = reinterpret_cast<const double2*>(d_trg);
@@ -450,19 +511,34 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   last_stats() = SolveStats();
   double th[7];
   th[0] = wall();
-  // ---- bounding box of sources and targets (pts_tree2d.c:493-516)
-  int nbk = std::min(1024, std::max(1, nblk(ns + nt, kBB)));
-  double4* part = S.get<double4>(nbk);
-  if (S.err) return S.err;
   // The root box stays on the device for the keys; its host copy is valid
   // after the tree's first read-back synchronises the stream.
   double* d_geo = S.get<double>(3);
-  if (S.err) return S.err;
-  k_bbox_partial<<<nbk, kBB, 0, c.st>>>(src, ns, trg, nt, part);
-  k_bbox_final<<<1, 1024, 0, c.st>>>(part, nbk, d_geo);
   double* h_geo = static_cast<double*>(pinned(sizeof(double) * 3, 2));
+  if (S.err) return S.err;
   if (!h_geo) return IER_ALLOC;
-  HIP_TRY(hipMemcpyAsync(h_geo, d_geo, sizeof(double) * 3, hipMemcpyDeviceToHost, c.st));
+  // a forest's normalised coordinates (input order), when its kernels use them
+  double2* s_norm = nullptr;
+  double2* t_norm = nullptr;
+  if (fin) {
+    // ---- a forest: the unit square holds every problem's root cell
+    h_geo[0] = 0.5;
+    h_geo[1] = 0.5;
+    h_geo[2] = 1.0;
+    if (fin->normalise) {
+      s_norm = S.get<double2>(ns);
+      t_norm = S.get<double2>(nt);
+      if (S.err) return S.err;
+    }
+  } else {
+    // ---- bounding box of sources and targets (pts_tree2d.c:493-516)
+    int nbk = std::min(1024, std::max(1, nblk(ns + nt, kBB)));
+    double4* part = S.get<double4>(nbk);
+    if (S.err) return S.err;
+    k_bbox_partial<<<nbk, kBB, 0, c.st>>>(src, ns, trg, nt, part);
+    k_bbox_final<<<1, 1024, 0, c.st>>>(part, nbk, d_geo);
+    HIP_TRY(hipMemcpyAsync(h_geo, d_geo, sizeof(double) * 3, hipMemcpyDeviceToHost, c.st));
+  }
   th[1] = wall();
 
   // ---- keys + sort.  Boxes down to level D only need the keys ordered by
@@ -478,22 +554,34 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   P.perm_t = P.perm_s + ns;
   double tt[3] = {0, 0, 0};
   int e = IER_OK;
-  for (int levels : {kSortLevels, kMaxDepth}) {
+  // (a forest's trees sit below its root cells: every key bit, at once)
+  for (int levels : {fin ? kMaxDepth : kSortLevels, kMaxDepth}) {
     const int begin_bit = 2 * (kMaxDepth - levels);
     uint64_t *ks = nullptr, *kt = nullptr;
     e = sort_points(c, S, src, ns, trg, nt, d_geo, begin_bit, plan.sort_err, P.perm_s, &ks,
-                    &kt);
+                    &kt, fin, s_norm, t_norm);
     if (e) return e;
     th[2] = wall();
     // ---- tree, lists and work lists on the device
     e = build_dtree(c, S, ks, ns, kt, nt, h_geo, idivflag, ndiv, ifpgh, ifpghtarg,
-                    levels, plan, tt);
+                    levels, plan, tt, fin ? &fin->dev : nullptr, fin ? fin->lfirst - 2 : 0);
     if (e != IER_RESORT) break;
   }
   if (e) return e;
   P.c0x = h_geo[0];  // the tree's read-backs have synchronised the stream
   P.c0y = h_geo[1];
   P.bs0 = h_geo[2];
+  if (fin) {
+    plan.lfirst = fin->lfirst;
+    if (fin->frames) {
+      P.bs0 = fin->min_bs0;
+      k_forest_frames<<<nblk(plan.nb, 256), 256, 0, c.st>>>(
+          plan.nb, plan.D.key, plan.D.level, fin->dev, fin->prob,
+          const_cast<double2*>(plan.D.ctr), const_cast<double*>(plan.D.bsb));
+    }
+    if (s_norm) src = s_norm;
+    if (t_norm) trg = t_norm;
+  }
   th[3] = th[2] + tt[0];
   th[4] = th[3] + tt[1];
   const DTree& D = plan.D;

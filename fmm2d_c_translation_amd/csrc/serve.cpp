@@ -357,4 +357,44 @@ void fmm2dhip_server_destroy(fmm2dhip_server* s) {
   delete s;
 }
 
+int fmm2dhip_batch_fused(int n, fmm2dhip_problem* probs, int nworkers) {
+  if (n <= 0) return 0;
+  if (!probs) return n;
+  try {
+    const std::vector<std::vector<int>> groups = fmm::fused_groups(n, probs);
+    std::vector<char> grouped(n, 0);
+    for (const auto& g : groups)
+      for (int i : g) grouped[i] = 1;
+    // the others go to the worker pool first and run beside the forests
+    std::vector<int> rest;
+    for (int i = 0; i < n; ++i)
+      if (!grouped[i]) rest.push_back(i);
+    std::stable_sort(rest.begin(), rest.end(), [&](int a, int b) {
+      return (double)probs[a].ns + probs[a].nt > (double)probs[b].ns + probs[b].nt;
+    });
+    std::shared_ptr<fmm::Server> pool;
+    std::vector<long long> tickets(rest.size(), -1);
+    int failed = 0;
+    if (!rest.empty()) {
+      pool = fmm::batch_pool(std::min(std::max(nworkers, 1), 32));
+      for (size_t k = 0; k < rest.size(); ++k) {
+        tickets[k] = pool->submit(&probs[rest[k]]);
+        if (tickets[k] < 0) {
+          probs[rest[k]].ier = fmm::IER_ALLOC;
+          ++failed;
+        }
+      }
+    }
+    for (const auto& g : groups) failed += fmm::solve_forest(probs, g);
+    for (size_t k = 0; k < rest.size(); ++k) {
+      if (tickets[k] < 0) continue;
+      pool->wait(tickets[k], -1.0);
+      if (probs[rest[k]].ier != 0) ++failed;
+    }
+    return failed;
+  } catch (...) {
+    return n;
+  }
+}
+
 }  // extern "C"

@@ -118,7 +118,7 @@ __global__ void k_map_m2l_all(const int* work, int nwork, DTree D, int* map) {
   if (k >= D.o2[b + 1] - k0) return;
   const int j = D.l2[k0 + k];
   if (D.srng[j].y <= D.srng[j].x) return;
-  const int s = m2l_slot(D.ctr[j], D.ctr[b], D.bs[D.level[b]]);
+  const int s = m2l_slot(D.ctr[j], D.ctr[b], D.bsb[b]);
   if (s >= 0) map[(size_t)e * 49 + s] = j;
 }
 

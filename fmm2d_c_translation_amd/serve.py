@@ -5,7 +5,8 @@
 (``abi.cfmm2d_call`` and friends) together with the ``fmm2dhip_problem``
 struct that points at them; ``Problem.result()`` reads the outputs back in
 the same form as the single-problem calls.  ``batch`` solves a list of
-problems on the library's worker pool; ``Server`` is the request loop with
+problems on the library's worker pool; ``batch_fused`` solves compatible
+problems together as one tree (``fmm2dhip_batch_fused``); ``Server`` is the request loop with
 HBM-budget admission, continuous batching and latency statistics.
 """
 from __future__ import annotations
@@ -100,6 +101,20 @@ def batch(lib, problems, nworkers=8):
         arr[i] = p.c
     lib.fmm2dhip_batch.restype = C.c_int
     nbad = lib.fmm2dhip_batch(len(problems), arr, int(nworkers))
+    for i, p in enumerate(problems):
+        p.c = arr[i]
+    return nbad
+
+
+def batch_fused(lib, problems, nworkers=8):
+    """Solve the problems with fmm2dhip_batch_fused: groups of compatible
+    problems as one forest each (one launch sequence per group), the rest on
+    the worker pool; returns the number of problems whose ier is not 0."""
+    arr = (fmm2dhip_problem * len(problems))()
+    for i, p in enumerate(problems):
+        arr[i] = p.c
+    lib.fmm2dhip_batch_fused.restype = C.c_int
+    nbad = lib.fmm2dhip_batch_fused(len(problems), arr, int(nworkers))
     for i, p in enumerate(problems):
         p.c = arr[i]
     return nbad

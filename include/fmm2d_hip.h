@@ -448,6 +448,22 @@ typedef struct fmm2dhip_problem {
  * problems whose ier is not 0. */
 int fmm2dhip_batch(int n, fmm2dhip_problem *probs, int nworkers);
 
+/* Solves probs[0..n) fused: problems that share kernel, nd, eps, leaf size
+ * (ndiv), charge/dipole flags, ifpgh/ifpghtarg, whether they have targets,
+ * precision and device_ptrs, and have more sources than the leaf size (and,
+ * for Helmholtz, a real positive zk), are solved together as one forest:
+ * each problem's tree sits below its own cell of one quadtree, so the whole
+ * group runs as one level-synchronous launch sequence on the calling
+ * thread's stream (one point sort, one tree build, one launch per FMM step
+ * and level).  Every other problem runs on the fmm2dhip_batch worker pool
+ * meanwhile.  Laplace results (cfmm2d, lfmm2d, rfmm2d) equal the problem's
+ * solve alone up to rounding (same tree, same expansions); a Helmholtz
+ * problem is rescaled to the group's common wavenumber, its root box
+ * enlarged at most twofold, and agrees with its solve alone within eps.
+ * Outputs, ier and seconds (the group's wall time) are written as by
+ * fmm2dhip_batch; returns the number of problems whose ier is not 0. */
+int fmm2dhip_batch_fused(int n, fmm2dhip_problem *probs, int nworkers);
+
 /* A persistent request loop.  Requests are admitted in arrival order as a
  * worker and device memory free up: a worker keeps the workspace of its
  * largest solve resident, and a request starts only while the workspaces of

@@ -104,3 +104,46 @@ def same(a, b, tag):
         if k in b:
             assert k in a, (tag, k)
             assert np.array_equal(a[k], b[k]), (tag, k)
+
+
+class Hip:
+    """Device buffers through the HIP runtime the library itself links
+    (/opt/rocm/lib/libamdhip64.so.7), for tests that hand device pointers to
+    the library without a second runtime in the process."""
+
+    def __init__(self):
+        import os
+        path = "/opt/rocm/lib/libamdhip64.so.7"
+        if not os.path.exists(path):
+            path = "libamdhip64.so"
+        self.h = C.CDLL(path)
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipDeviceSynchronize.argtypes = []
+        self.bufs = []
+
+    def put(self, a):
+        """A device copy of numpy array a (its bytes as laid out); returns the pointer."""
+        a = np.ascontiguousarray(a) if not a.flags.f_contiguous else a
+        p = C.c_void_p()
+        assert self.h.hipMalloc(C.byref(p), max(a.nbytes, 16)) == 0
+        if a.nbytes:
+            assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        self.bufs.append(p)
+        return p.value
+
+    def zeros(self, nbytes):
+        return self.put(np.zeros(max(nbytes, 16), np.uint8))
+
+    def get(self, ptr, like):
+        """Copy len(like.nbytes) bytes at ptr back into a new array shaped like like."""
+        out = np.empty_like(like)
+        assert self.h.hipDeviceSynchronize() == 0
+        assert self.h.hipMemcpy(out.ctypes.data, C.c_void_p(ptr), out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        for p in self.bufs:
+            self.h.hipFree(p)
+        self.bufs = []
