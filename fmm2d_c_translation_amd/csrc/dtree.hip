@@ -480,8 +480,9 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
            unsigned long long* m2l_lev) {
   __shared__ int sc[WL_N];
   __shared__ unsigned long long sm2l[kMaxDepth + 1];
+  __shared__ unsigned long long sr2[kMaxDepth + 1];  // per level: squared bound, as bits
   if (threadIdx.x < WL_N) sc[threadIdx.x] = 0;
-  if (threadIdx.x <= kMaxDepth) sm2l[threadIdx.x] = 0;
+  if (threadIdx.x <= kMaxDepth) sm2l[threadIdx.x] = sr2[threadIdx.x] = 0;
   __syncthreads();
   int b = blockIdx.x * WLB + threadIdx.x;
   unsigned m = wl_mask(a, b);
@@ -531,10 +532,10 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
     for (int w = 0; w < WLB / 64; ++w) t += sp[threadIdx.x][w];
     if (t != 0) atomicAdd(&pairs[threadIdx.x], t);
   }
-  // per level of the box (pairs[5 + l]), then over all levels (pairs[4])
+  // per level of the box (in LDS, then one atomic per level and block into
+  // pairs[5 + l]), and over all levels (pairs[4])
   if (r2max > 0.0)
-    atomicMax(reinterpret_cast<unsigned long long*>(pairs + 5 + a.level[b]),
-              (unsigned long long)__double_as_longlong(r2max));
+    atomicMax(&sr2[a.level[b]], (unsigned long long)__double_as_longlong(r2max));
   for (int w = 32; w >= 1; w >>= 1) r2max = fmax(r2max, __shfl_xor(r2max, w));
   if (lane == 0 && r2max > 0.0)
     atomicMax(reinterpret_cast<unsigned long long*>(pairs + 4),
@@ -542,6 +543,9 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
   if (threadIdx.x < WL_N) bcnt[threadIdx.x * nblocks + blockIdx.x] = sc[threadIdx.x];
   if (threadIdx.x <= kMaxDepth && sm2l[threadIdx.x])
     atomicAdd(&m2l_lev[threadIdx.x], sm2l[threadIdx.x]);
+  __syncthreads();  // every lane's per-level bound is in sr2
+  if (threadIdx.x <= kMaxDepth && sr2[threadIdx.x])
+    atomicMax(reinterpret_cast<unsigned long long*>(pairs + 5 + threadIdx.x), sr2[threadIdx.x]);
 }
 
 // exclusive scan of the per-block counts of every list (one block, wave j

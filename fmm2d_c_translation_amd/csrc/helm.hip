@@ -1274,6 +1274,33 @@ int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier) {
   return h2dterms_cached(bs, zk, eps, ier);
 }
 
+// Consecutive levels (from lfrom) of a level-sorted work list, grouped by
+// the H0/H1 series length jy_nterms(xmax(l)) that each level's bound on k r
+// allows (all one group, length 0, when rz is false): f(span, xmax) per
+// group, xmax the group's largest bound.  A tree whose leaves span many
+// sizes -- adaptive trees, and forests of problems of many sizes -- then
+// keeps the short series on its small leaves.
+template <class XF, class F>
+void by_tier(const WorkList& wl, int lfrom, int L, bool rz, XF xmax, F f) {
+  int l = std::max(lfrom, 0);
+  auto tier = [&](int q) { return rz ? jy_nterms(xmax(q)) : 0; };
+  while (l <= L) {
+    if (!wl.at(l).n) {
+      ++l;
+      continue;
+    }
+    const int nr = tier(l);
+    double x = rz ? xmax(l) : 0.0;
+    int l2 = l + 1;
+    while (l2 <= L && (!wl.at(l2).n || tier(l2) == nr)) {
+      if (wl.at(l2).n && rz) x = std::max(x, xmax(l2));
+      ++l2;
+    }
+    f(Span{wl.off + wl.lev[l], wl.lev[l2] - wl.lev[l]}, x);
+    l = l2;
+  }
+}
+
 int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const HelmArgs& a) {
   const DTree& D = plan.D;
   const int nd = a.nd, L = plan.L, nb = plan.nb;
@@ -1416,14 +1443,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
 
   // k r over list 4: a source of a coarse leaf to the centre of a target of
   // size s is at most 3.5 s away per coordinate, r <= 4.95 s
-  int nr4 = 0;
-  if (rz) {
-    int lmin = L + 1;
-    for (int l = 0; l <= L && lmin > L; ++l)
-      if (plan.list4.at(l).n) lmin = l;
-    const double xmax4 = lmin <= L ? zk.x * 4.95 * bs[lmin] : 0.0;
-    nr4 = jy_nterms(xmax4);
-  }
+  auto xmax4 = [&](int l) { return zk.x * 4.95 * bs[l]; };
   StepTimer* tm = step_timer();
   {
     SolveStats& st = last_stats();
@@ -1489,28 +1509,29 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
                              2 * lev[l].p + 1, mp, per, mp, per, nd, 0.0));
   }
   tm->mark(c.st);
-  // step 2: list 4
-  if (s_l4.n) {
+  // step 2: list 4, by groups of levels of one series length
+  if (s_l4.n) by_tier(plan.list4, lg, L, rz, xmax4, [&](const Span& sp, double xm) {
+    const int nr4 = rz ? jy_nterms(xm) : 0;
     if (fg && nr4 == JY_NTERMS0)
-      k_hform_g<false, JY_NTERMS0><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
-          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+      k_hform_g<false, JY_NTERMS0><<<nblk(sp.n, 4), WAVE, 0, c.st>>>(
+          wptr(sp), sp.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else if (fg && nr4 == JY_NTERMS1)
-      k_hform_g<false, JY_NTERMS1><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
-          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+      k_hform_g<false, JY_NTERMS1><<<nblk(sp.n, 4), WAVE, 0, c.st>>>(
+          wptr(sp), sp.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else if (fg && nr4 == JY_NTERMS2)
-      k_hform_g<false, JY_NTERMS2><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
-          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+      k_hform_g<false, JY_NTERMS2><<<nblk(sp.n, 4), WAVE, 0, c.st>>>(
+          wptr(sp), sp.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else if (fg && nr4 == JY_NTERMS3)
-      k_hform_g<false, JY_NTERMS3><<<nblk(s_l4.n, 4), WAVE, 0, c.st>>>(
-          wptr(s_l4), s_l4.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
+      k_hform_g<false, JY_NTERMS3><<<nblk(sp.n, 4), WAVE, 0, c.st>>>(
+          wptr(sp), sp.n, D, dlev, zk.x, Pt.src, chg, dip, dv, Pt.ns, nd, per, lo);
     else
       (nr4 == JY_NTERMS0   ? k_hform<false, JY_NTERMS0>
        : nr4 == JY_NTERMS1 ? k_hform<false, JY_NTERMS1>
        : nr4 == JY_NTERMS2 ? k_hform<false, JY_NTERMS2>
        : nr4 == JY_NTERMS3 ? k_hform<false, JY_NTERMS3>
-                           : k_hform<false, 0>)<<<s_l4.n, WAVE, shm, c.st>>>(
-          wptr(s_l4), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
-  }
+                           : k_hform<false, 0>)<<<sp.n, WAVE, shm, c.st>>>(
+          wptr(sp), D, dlev, zk, Pt.src, chg, dip, dv, Pt.ns, nd, CH, per, lo);
+  });
   tm->mark(c.st);
   // step 4: M2L (49 slot matrices per level, contiguous); the slot maps of
   // every level in one launch
@@ -1576,31 +1597,31 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   struct Set {
     int pg; Span l3, lv, nn; const int2* rng; const double2* pts; int N; double2 *po, *gr, *he;
     const int* perm; double2 *uo, *ug, *uh; bool* fused;
+    const WorkList *l3w, *nnw;
   };
   Set sets[2] = {{a.ifpgh, s_l3s, s_lvs, plan.p2p_s.all(), D.srng, Pt.src, Pt.ns, a.pot, a.grad,
-                  a.hess, Pt.perm_s, a.upot, a.ugrad, a.uhess, &a.fused_s},
+                  a.hess, Pt.perm_s, a.upot, a.ugrad, a.uhess, &a.fused_s, &plan.list3_s,
+                  &plan.p2p_s},
                  {a.ifpghtarg, s_l3t, s_lvt, plan.p2p_t.all(), D.trng, Pt.trg, Pt.nt, a.pott,
-                  a.gradt, a.hesst, Pt.perm_t, a.upott, a.ugradt, a.uhesst, &a.fused_t}};
+                  a.gradt, a.hesst, Pt.perm_t, a.upott, a.ugradt, a.uhesst, &a.fused_t,
+                  &plan.list3_t, &plan.p2p_t}};
   a.fused_s = a.fused_t = false;
   const double thresh = std::abs(a.zk) * Pt.bs0 * 0x1p-51;
-  // k r over list 1, from the tree's largest list-1 box-pair distance (for
-  // leaves of one size s, 2 sqrt(2) s; with a neighbour of size 2s, up to
-  // 4.25 s)
-  double xmax = 0.0;
-  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) xmax = a.zk.real() * plan.p2p_rmax;
+  // k r over list 1, per level of the target leaves, from the tree's largest
+  // list-1 box-pair distance there (for leaves of one size s, 2 sqrt(2) s;
+  // with a neighbour of size 2s, up to 4.25 s)
+  // (a level with no list-1 pair gets the shortest series, which it never uses)
+  auto xmax1 = [&](int l) { return zk.x * std::max(plan.p2p_rmax_lev[l], 1e-300); };
   // k r over list 3: a point of a leaf of size s to the centre of a child
   // of a colleague is at most 1.75 s away per coordinate, r <= 2.48 s
-  double xmax3 = 0.0;
-  if (a.zk.imag() == 0.0 && a.zk.real() > 0.0) {
-    int lmin = L + 1;
-    for (int l = 0; l <= L && lmin > L; ++l)
-      if (plan.list3_s.at(l).n || plan.list3_t.at(l).n) lmin = l;
-    if (lmin <= L) xmax3 = a.zk.real() * 2.48 * bs[lmin];
-  }
+  auto xmax3 = [&](int l) { return zk.x * 2.48 * bs[l]; };
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.l3.n)
-      launch_heval<true>(s.pg, s.l3.n, c.st, wptr(s.l3), D, dlev, zk, s.rng, s.pts, s.N, nd, per, mp, xmax3, pmax_pt, s.po, s.gr, s.he);
+      by_tier(*s.l3w, lg - 1, L, rz, xmax3, [&](const Span& sp, double xm) {
+        launch_heval<true>(s.pg, sp.n, c.st, wptr(sp), D, dlev, zk, s.rng, s.pts, s.N, nd, per,
+                           mp, xm, pmax_pt, s.po, s.gr, s.he);
+      });
   tm->mark(c.st);
   for (auto& s : sets)
     if (s.pg > 0 && s.N > 0 && s.lv.n)
@@ -1608,12 +1629,14 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
   tm->mark(c.st);
   for (auto& s : sets) {
     if (s.pg <= 0 || s.N == 0 || !a.ifnear || !s.nn.n) continue;
-    if (chg && dip)
-      launch_hp2p<true, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
-    else if (chg)
-      launch_hp2p<true, false>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
-    else
-      launch_hp2p<false, true>(s.pg, s.nn.n, c.st, W + s.nn.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xmax, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
+    by_tier(*s.nnw, 0, L, rz, xmax1, [&](const Span& sp, double xm) {
+      if (chg && dip)
+        launch_hp2p<true, true>(s.pg, sp.n, c.st, W + sp.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xm, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
+      else if (chg)
+        launch_hp2p<true, false>(s.pg, sp.n, c.st, W + sp.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xm, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
+      else
+        launch_hp2p<false, true>(s.pg, sp.n, c.st, W + sp.off, D, zk, Pt.src, chg, dip, dv, Pt.ns, s.rng, s.pts, s.N, nd, thresh, xm, s.po, s.gr, s.he, s.perm, s.uo, s.ug, s.uh);
+    });
     *s.fused = s.uo != nullptr;
   }
   tm->mark(c.st);

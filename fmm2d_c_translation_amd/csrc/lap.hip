@@ -716,31 +716,6 @@ void launch_p2p_pg(int pg, int mode, int nbx, hipStream_t st, const int* work, c
   if (pg == 3) launch_p2p_mode<C, Q, 3>(mode, nbx, st, work, D, src, chg, dip, NS, trng, tpts, NT, nd, th2, pot, grad, hess, out);
 }
 
-// M2L's log term where box sizes differ between boxes of one level (a
-// forest in its problems' own coordinates): operator entry (0, 0) is
-// log|c1 - c2| = log|offset| + log(box size), the tables hold the first
-// part, and this adds log(size of b) times the list-2 sources' order-0
-// coefficients to each target b's order-0 local coefficient, per channel.
-__global__ void k_m2l_lap_log(const int* work, int n, DTree D, const double2* mp, size_t xs,
-                              double2* lo, size_t ys, int nd, int P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = t / nd, d = t - e * nd;
-  if (e >= n) return;
-  const int b = work[e];
-  double2 acc = make_double2(0.0, 0.0);
-  for (int k = D.o2[b]; k < D.o2[b + 1]; ++k) {
-    const int j = D.l2[k];
-    if (D.srng[j].y <= D.srng[j].x) continue;
-    const double2 x = mp[(size_t)j * xs + (size_t)d * P];
-    acc.x += x.x;
-    acc.y += x.y;
-  }
-  const double g = log(D.bsb[b]);
-  double2* y = lo + (size_t)b * ys + (size_t)d * P;
-  y->x += g * acc.x;
-  y->y += g * acc.y;
-}
-
 }  // namespace
 
 int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const LapArgs& a) {
@@ -818,14 +793,11 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     const Span sp = plan.m2l.at(l);
     if (!sp.n) continue;
     const int* ml = map2 + (size_t)(sp.off - plan.m2l.off) * 49;
-    // (a forest's boxes of one level differ in size: their log term follows)
-    m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, forest ? 0.0 : std::log(plan.bs[l]),
-                          a.mixed ? tab.m2l32 : nullptr});
+    // (a forest's boxes of one level differ in size: each target's own)
+    m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, std::log(plan.bs[l]),
+                          a.mixed ? tab.m2l32 : nullptr, forest ? D.bsb : nullptr});
   }
   if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
-  if (forest && plan.m2l.n)
-    k_m2l_lap_log<<<nblk(plan.m2l.n * nd, 256), 256, 0, c.st>>>(
-        W + plan.m2l.off, plan.m2l.n, D, mp, per, lo, per, nd, P);
   tm->mark(c.st);
   // step 5: L2L into the children of boxes that hold targets
   for (int l = 1; l <= L - 1; ++l) {

@@ -214,7 +214,7 @@ __device__ __forceinline__ void xlate_block(
     int bx, int by, int g, const int* __restrict__ work, int nwork, const int* __restrict__ map,
     int nslot, const double2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
     size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log, int G,
-    double2* __restrict__ part) {
+    double2* __restrict__ part, const double* __restrict__ lap_bsb = nullptr) {
   __shared__ int smap[4][16 * 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
@@ -223,6 +223,8 @@ __device__ __forceinline__ void xlate_block(
   const bool wok = w < nwork;
   const int rt0 = by * MW;
   int* sm = smap[wv];
+  // per-target log(box size) when box sizes differ within a level (forests)
+  if (lap_bsb && wok) lap_log = log(lap_bsb[work[w]]);
   {
     const size_t base = (size_t)tile * 16 * nslot, end = (size_t)nwork * nslot;
     for (int e = lane; e < 16 * nslot; e += 64) sm[e] = base + e < end ? map[base + e] : -1;
@@ -352,6 +354,7 @@ struct XLevel {
   const double2* T;
   int nin, nout;
   double lap_log;
+  const double* lap_bsb;  // per-box sizes for log(box size), or null (lap_log)
   int block0;
   int by;
 };
@@ -368,7 +371,7 @@ k_xlate_mfma_levels(XLevels L, int nslot, const double2* __restrict__ X, size_t 
   while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
   const XLevel& v = L.lv[j];
   xlate_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
-                  v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr);
+                  v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr, v.lap_bsb);
 }
 
 // Y[work[w]] += sum over slot groups of the partials, group 0 first
@@ -409,7 +412,7 @@ __device__ __forceinline__ void xlate32_block(
     int bx, int by, int g, const int* __restrict__ work, int nwork, const int* __restrict__ map,
     int nslot, const float2* __restrict__ T, int nin, int nout, const double2* __restrict__ X,
     size_t xstride, double2* __restrict__ Y, size_t ystride, int nd, double lap_log, int G,
-    double2* __restrict__ part) {
+    double2* __restrict__ part, const double* __restrict__ lap_bsb = nullptr) {
   __shared__ int smap[4][16 * 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
@@ -418,6 +421,8 @@ __device__ __forceinline__ void xlate32_block(
   const bool wok = w < nwork;
   const int rt0 = by * MW;
   int* sm = smap[wv];
+  // per-target log(box size) when box sizes differ within a level (forests)
+  if (lap_bsb && wok) lap_log = log(lap_bsb[work[w]]);
   {
     const size_t base = (size_t)tile * 16 * nslot, end = (size_t)nwork * nslot;
     for (int e = lane; e < 16 * nslot; e += 64) sm[e] = base + e < end ? map[base + e] : -1;
@@ -538,6 +543,7 @@ struct XLevel32 {
   const float2* T;
   int nin, nout;
   double lap_log;
+  const double* lap_bsb;
   int block0;
   int by;
 };
@@ -553,7 +559,7 @@ k_xlate_mfma32_levels(XLevels32 L, int nslot, const double2* __restrict__ X, siz
   while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
   const XLevel32& v = L.lv[j];
   xlate32_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
-                    v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr);
+                    v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr, v.lap_bsb);
 }
 
 // Translation through k_xlate_mfma when the output order fits eight row
@@ -618,6 +624,9 @@ struct M2LLevel {
   int P;
   double lap_log;
   const float2* T32;    // the same in FP32 (mixed precision), or null
+  // Laplace: per-box sizes whose log replaces lap_log (a forest's levels
+  // hold boxes of different sizes), or null
+  const double* lap_bsb = nullptr;
 };
 inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
                                     const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
@@ -651,10 +660,11 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
       for (int by = 0; by * mw < MT; ++by) {
         if (L.n == kMaxXLevels || L32.n == kMaxXLevels) flush();
         if (v.T32) {
-          L32.lv[L32.n++] = XLevel32{v.work, v.nwork, v.map, v.T32, v.P, v.P, v.lap_log, nb32, by};
+          L32.lv[L32.n++] =
+              XLevel32{v.work, v.nwork, v.map, v.T32, v.P, v.P, v.lap_log, v.lap_bsb, nb32, by};
           nb32 += nbl;
         } else {
-          L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, nb, by};
+          L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, v.lap_bsb, nb, by};
           nb += nbl;
         }
       }
@@ -663,6 +673,7 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
   }
   for (const M2LLevel& v : lv) {
     if (v.nwork <= 0 || (v.P + 15) / 16 <= 8) continue;
+    if (v.lap_bsb) return hipErrorInvalidValue;  // per-box sizes: MFMA kernels only
     const hipError_t e = launch_translate(st, v.work, v.nwork, v.map, 49, v.T, v.P, v.P, X, xs, Y,
                                           ys, nd, v.lap_log);
     if (e != hipSuccess) return e;
