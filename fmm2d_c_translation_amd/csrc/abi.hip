@@ -101,24 +101,6 @@ __global__ void k_rfmm_unpack(const double2* P, const double2* G, const double2*
   }
 }
 
-struct LapReq {
-  bool logk = false;  // lfmm (log kernel, complex densities) vs cfmm (Cauchy)
-  bool realk = false;  // rfmm (log kernel, real densities and outputs)
-  int nd = 1;
-  double eps = 1e-6;
-  int ns = 0, nt = 0;
-  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
-  int ifc = 0, ifd = 0;
-  const void *chg = nullptr, *dip = nullptr;
-  int pg = 0, pgt = 0;
-  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
-  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
-  int ndiv = 0, idivflag = 0, ifnear = 1;
-  bool device = false;
-  hipStream_t stream = nullptr;
-  double* timeinfo = nullptr;
-  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
-};
 
 template <class T>
 int upload(Scratch& S, hipStream_t st, const T* h, size_t n, const T** d) {
@@ -129,38 +111,52 @@ int upload(Scratch& S, hipStream_t st, const T* h, size_t n, const T** d) {
   return IER_OK;
 }
 
-int run_lap(const LapReq& r) {
-  Ctx c;
-  int e = get_ctx(c);
-  if (e) return e;
-  if (r.stream) c.st = r.stream;
+}  // namespace
+
+// Laplace front end: arguments, host inputs to the device, point sort, tree
+// and lists (the part of a solve that reads sizes back).
+int lap_setup(const LapReq& r, Scratch& S, Prepared& q) {
+  Ctx& c = q.c;
   if (r.nd < 1 || r.ns < 1 || r.nt < 0 || r.pg < 0 || r.pg > 3 || r.pgt < 0 || r.pgt > 3)
     return IER_ARGS;
-  Scratch S(c.st);
+  int e = IER_OK;
+  const int nd = r.nd, ns = r.ns, nt = r.nt;
+  const bool split = r.logk || r.realk;
+  q.src = r.src;
+  q.trg = r.trg;
+  q.dv = r.dipvec;
+  q.chg = r.chg;
+  q.dip = r.dip;
+  if (!r.device) {
+    const size_t esz = r.realk ? sizeof(double) : sizeof(double2);
+    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &q.src))) return e;
+    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &q.trg))) return e;
+    const char *cc = static_cast<const char*>(r.chg), *qq = static_cast<const char*>(r.dip);
+    if (r.ifc && (e = upload(S, c.st, cc, esz * nd * ns, &cc))) return e;
+    if (r.ifd && (e = upload(S, c.st, qq, esz * nd * ns, &qq))) return e;
+    q.chg = cc;
+    q.dip = qq;
+    if (split && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+  }
+  return build_plan(c, S, q.src, ns, q.trg, nt, r.idivflag, r.ndiv, q.P, q.plan, true, r.pg,
+                    r.pgt, r.forest);
+}
+
+// Laplace evaluation after lap_setup: strengths in sorted order, the FMM
+// steps, outputs in the caller's layout (and to the host for host
+// outputs); stream work only, no synchronisation.
+int lap_eval(const LapReq& r, Scratch& S, Prepared& q) {
+  Ctx& c = q.c;
+  SortedPoints& P = q.P;
+  Plan& plan = q.plan;
+  int e = IER_OK;
   const int nd = r.nd, ns = r.ns, nt = r.nt;
   const bool split = r.logk || r.realk;  // gradient (nd,2,n), Hessian (nd,3,n)
   const int ncomp_g = split ? 2 : 1, ncomp_h = split ? 3 : 1;
   const size_t esz = r.realk ? sizeof(double) : sizeof(double2);  // user element
-  const double *src = r.src, *trg = r.trg, *dv = r.dipvec;
-  const double2 *chg = static_cast<const double2*>(r.chg), *dip = static_cast<const double2*>(r.dip);
-  const double *rchg = static_cast<const double*>(r.chg), *rdip = static_cast<const double*>(r.dip);
-  if (!r.device) {
-    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &src))) return e;
-    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &trg))) return e;
-    if (r.realk) {
-      if (r.ifc && (e = upload(S, c.st, rchg, (size_t)nd * ns, &rchg))) return e;
-      if (r.ifd && (e = upload(S, c.st, rdip, (size_t)nd * ns, &rdip))) return e;
-    } else {
-      if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
-      if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
-    }
-    if (split && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
-  }
-  SortedPoints P;
-  Plan plan;
-  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt,
-                 r.forest);
-  if (e) return e;
+  const double* dv = q.dv;
+  const double2 *chg = static_cast<const double2*>(q.chg), *dip = static_cast<const double2*>(q.dip);
+  const double *rchg = static_cast<const double*>(q.chg), *rdip = static_cast<const double*>(q.dip);
   const int ndc = r.logk ? 2 * nd : nd;
   LapArgs a;
   a.nd = ndc;
@@ -188,7 +184,7 @@ int run_lap(const LapReq& r) {
     if (flag < need) return nullptr;
     size_t cnt = (size_t)ndc * std::max(n, 1);
     double2* p = S.get<double2>(cnt);
-    if (p) (void)hipMemsetAsync(p, 0, sizeof(double2) * cnt, c.st);
+    if (p) fill_async(p, 0u, 4 * cnt, c.st);
     return p;
   };
   a.pot = zeros(r.pg, 1, ns); a.grad = zeros(r.pg, 2, ns); a.hess = zeros(r.pg, 3, ns);
@@ -250,10 +246,24 @@ int run_lap(const LapReq& r) {
       if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, esz * nh, hipMemcpyDeviceToHost, c.st));
     }
   }
+  (void)e;
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
+}
+
+namespace {
+
+int run_lap(const LapReq& r) {
+  Prepared q;
+  int e = get_ctx(q.c);
+  if (e) return e;
+  if (r.stream) q.c.st = r.stream;
+  Scratch S(q.c.st);
+  if ((e = lap_setup(r, S, q))) return e;
+  if ((e = lap_eval(r, S, q))) return e;
+  HIP_TRY(hipStreamSynchronize(q.c.st));
   finish_timer(r.timeinfo);
-  return sort_status(plan);
+  return sort_status(q.plan);
 }
 
 int guarded(const LapReq& r) {
@@ -270,47 +280,44 @@ int guarded(const LapReq& r) {
 }
 
 
-struct HelmReq {
-  int nd = 1;
-  double eps = 1e-6;
-  std::complex<double> zk = 1.0;
-  int ns = 0, nt = 0;
-  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
-  int ifc = 0, ifd = 0;
-  const void *chg = nullptr, *dip = nullptr;
-  int pg = 0, pgt = 0;
-  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
-  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
-  int ndiv = 0, idivflag = 0, ifnear = 1;
-  bool device = false;
-  hipStream_t stream = nullptr;
-  double* timeinfo = nullptr;
-  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
-};
 
-int run_helm(const HelmReq& r) {
-  Ctx c;
-  int e = get_ctx(c);
-  if (e) return e;
-  if (r.stream) c.st = r.stream;
+}  // namespace
+
+// Helmholtz front end (as lap_setup).
+int helm_setup(const HelmReq& r, Scratch& S, Prepared& q) {
+  Ctx& c = q.c;
   if (r.nd < 1 || r.ns < 1 || r.nt < 0 || r.pg < 0 || r.pg > 3 || r.pgt < 0 || r.pgt > 3)
     return IER_ARGS;
-  Scratch S(c.st);
+  int e = IER_OK;
   const int nd = r.nd, ns = r.ns, nt = r.nt;
-  const double *src = r.src, *trg = r.trg, *dv = r.dipvec;
-  const double2 *chg = static_cast<const double2*>(r.chg), *dip = static_cast<const double2*>(r.dip);
+  q.src = r.src;
+  q.trg = r.trg;
+  q.dv = r.dipvec;
+  q.chg = r.chg;
+  q.dip = r.dip;
   if (!r.device) {
-    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &src))) return e;
-    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &trg))) return e;
-    if (r.ifc && (e = upload(S, c.st, chg, (size_t)nd * ns, &chg))) return e;
-    if (r.ifd && (e = upload(S, c.st, dip, (size_t)nd * ns, &dip))) return e;
-    if (r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &dv))) return e;
+    const double2 *cc = static_cast<const double2*>(r.chg), *qq = static_cast<const double2*>(r.dip);
+    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &q.src))) return e;
+    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &q.trg))) return e;
+    if (r.ifc && (e = upload(S, c.st, cc, (size_t)nd * ns, &cc))) return e;
+    if (r.ifd && (e = upload(S, c.st, qq, (size_t)nd * ns, &qq))) return e;
+    if (r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+    q.chg = cc;
+    q.dip = qq;
   }
-  SortedPoints P;
-  Plan plan;
-  e = build_plan(c, S, src, ns, trg, nt, r.idivflag, r.ndiv, P, plan, true, r.pg, r.pgt,
-                 r.forest);
-  if (e) return e;
+  return build_plan(c, S, q.src, ns, q.trg, nt, r.idivflag, r.ndiv, q.P, q.plan, true, r.pg,
+                    r.pgt, r.forest);
+}
+
+// Helmholtz evaluation after helm_setup (as lap_eval).
+int helm_eval(const HelmReq& r, Scratch& S, Prepared& q) {
+  Ctx& c = q.c;
+  SortedPoints& P = q.P;
+  Plan& plan = q.plan;
+  int e = IER_OK;
+  const int nd = r.nd, ns = r.ns, nt = r.nt;
+  const double* dv = q.dv;
+  const double2 *chg = static_cast<const double2*>(q.chg), *dip = static_cast<const double2*>(q.dip);
   HelmArgs a;
   a.nd = nd; a.eps = r.eps; a.zk = r.zk;
   a.ifcharge = r.ifc; a.ifdipole = r.ifd; a.ifpgh = r.pg; a.ifpghtarg = r.pgt; a.ifnear = r.ifnear;
@@ -325,7 +332,7 @@ int run_helm(const HelmReq& r) {
     if (flag < need) return nullptr;
     size_t cnt = (size_t)nd * ncomp * std::max(n, 1);
     double2* p = S.get<double2>(cnt);
-    if (p) (void)hipMemsetAsync(p, 0, sizeof(double2) * cnt, c.st);
+    if (p) fill_async(p, 0u, 4 * cnt, c.st);
     return p;
   };
   a.pot = zeros(r.pg, 1, ns, 1); a.grad = zeros(r.pg, 2, ns, 2); a.hess = zeros(r.pg, 3, ns, 3);
@@ -371,10 +378,24 @@ int run_helm(const HelmReq& r) {
       if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c.st));
     }
   }
+  (void)e;
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(c.st));
+  return IER_OK;
+}
+
+namespace {
+
+int run_helm(const HelmReq& r) {
+  Prepared q;
+  int e = get_ctx(q.c);
+  if (e) return e;
+  if (r.stream) q.c.st = r.stream;
+  Scratch S(q.c.st);
+  if ((e = helm_setup(r, S, q))) return e;
+  if ((e = helm_eval(r, S, q))) return e;
+  HIP_TRY(hipStreamSynchronize(q.c.st));
   finish_timer(r.timeinfo);
-  return sort_status(plan);
+  return sort_status(q.plan);
 }
 
 int guarded_h(const HelmReq& r) {
@@ -400,34 +421,45 @@ void zero8(double* t) {
 // One problem of the batched/served interface on the calling thread, with
 // the problem's precision mode for the duration of the solve; with a
 // forest, the problem is the concatenation of the forest's problems.
+void helm_req_of(const fmm2dhip_problem& p, const ForestIn* forest, HelmReq& r) {
+  r.nd = p.nd; r.eps = p.eps; r.zk = std::complex<double>(p.zk_re, p.zk_im);
+  r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
+  r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
+  r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
+  r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
+  r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
+  r.ndiv = p.ndiv > 0 ? p.ndiv : ndiv_rule(p.eps, p.ns, p.nt);
+  r.device = p.device_ptrs == 1;
+  r.forest = forest;
+}
+
+void lap_req_of(const fmm2dhip_problem& p, const ForestIn* forest, LapReq& r) {
+  r.logk = p.kernel == FMM2DHIP_LOG;
+  r.realk = p.kernel == FMM2DHIP_REAL;
+  r.nd = p.nd; r.eps = p.eps; r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
+  r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
+  r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
+  r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
+  r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
+  r.ndiv = p.ndiv > 0 ? p.ndiv : ndiv_rule(p.eps, p.ns, p.nt);
+  r.device = p.device_ptrs == 1;
+  r.forest = forest;
+}
+
 int solve_problem(fmm2dhip_problem& p, const ForestIn* forest) {
   const double t0 = wall();
   int& mode = thread_precision();
   const int saved = mode;
   mode = p.precision == PREC_MIXED ? PREC_MIXED : PREC_FP64;
-  const int ndiv = p.ndiv > 0 ? p.ndiv : ndiv_rule(p.eps, p.ns, p.nt);
   int e;
   if (p.kernel == FMM2DHIP_HELMHOLTZ) {
     HelmReq r;
-    r.nd = p.nd; r.eps = p.eps; r.zk = std::complex<double>(p.zk_re, p.zk_im);
-    r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
-    r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
-    r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
-    r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
-    r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
-    r.ndiv = ndiv; r.device = p.device_ptrs == 1; r.forest = forest;
+    helm_req_of(p, forest, r);
     e = guarded_h(r);
   } else if (p.kernel == FMM2DHIP_CAUCHY || p.kernel == FMM2DHIP_LOG ||
              p.kernel == FMM2DHIP_REAL) {
     LapReq r;
-    r.logk = p.kernel == FMM2DHIP_LOG;
-    r.realk = p.kernel == FMM2DHIP_REAL;
-    r.nd = p.nd; r.eps = p.eps; r.ns = p.ns; r.nt = p.nt; r.src = p.sources; r.trg = p.targ;
-    r.ifc = p.ifcharge; r.ifd = p.ifdipole; r.chg = p.charge; r.dip = p.dipstr; r.dipvec = p.dipvec;
-    r.pg = p.ifpgh; r.pgt = p.ifpghtarg;
-    r.pot = p.pot; r.grad = p.grad; r.hess = p.hess;
-    r.pott = p.pottarg; r.gradt = p.gradtarg; r.hesst = p.hesstarg;
-    r.ndiv = ndiv; r.device = p.device_ptrs == 1; r.forest = forest;
+    lap_req_of(p, forest, r);
     e = guarded(r);
   } else {
     e = IER_ARGS;

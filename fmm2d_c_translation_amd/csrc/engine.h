@@ -220,6 +220,36 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
 // Pinned host staging reused across the calls of one thread (slot < 8).
 void* pinned(size_t bytes, int slot);
 
+// Stream work of the FMM steps that a graph capture can record and replay:
+// fill_async sets n 32-bit words to v with a kernel (memset nodes in a
+// replayed graph are avoided); upload_small copies host data to the device,
+// through a pinned copy in the thread's HostKeeper while one is installed (a
+// captured copy reads its source again at every replay, and no pinned
+// allocation may happen while a stream captures: the keeper's pool is
+// reserved before).
+void fill_async(void* p, uint32_t v, size_t n, hipStream_t st);
+int upload_small(void* d, const void* h, size_t bytes, hipStream_t st);
+struct HostKeeper {
+  char* pool = nullptr;  // pinned, freed with the keeper
+  size_t cap = 0, used = 0;
+  int reserve(size_t bytes);
+  ~HostKeeper();
+};
+// While a CaptureScope is alive on a thread, its uploads go through keeper
+// and the step timer records nothing (events stay out of the graph).
+class CaptureScope {
+ public:
+  explicit CaptureScope(HostKeeper* keeper);
+  ~CaptureScope();
+  CaptureScope(const CaptureScope&) = delete;
+  CaptureScope& operator=(const CaptureScope&) = delete;
+};
+bool capturing();
+// Moves the calling thread's workspace blocks on the current device to out
+// (their memory then belongs to the caller, to free with hipFree); the
+// workspace starts empty.  Call while the allocations of interest are live.
+void detach_workspace(std::vector<void*>& out);
+
 // Boundaries of the eight FMM steps, recorded on the solve's stream:
 // ev[k] .. ev[k+1] bracket step order[k]; indices follow the reference's
 // timeinfo (0 form mp, 1 form lo, 2 M2M, 3 M2L, 4 L2L, 5 list 3, 6 local
@@ -229,9 +259,7 @@ struct StepTimer {
   int order[8] = {0, 2, 1, 3, 4, 5, 6, 7};
   int n = 0;
   bool ok = false;
-  void mark(hipStream_t st) {
-    if (ok && n < 9) (void)hipEventRecord(ev[n++], st);
-  }
+  void mark(hipStream_t st);
 };
 StepTimer* step_timer();       // per thread, reset
 StepTimer* step_timer_peek();  // per thread, as left by the last solve
@@ -369,6 +397,66 @@ int fft_rows_device(hipStream_t st, Scratch& S, double2* rows, int nrows, int n,
 
 int l2dterms(double eps);
 int ndiv_rule(double eps, int ns, int nt);
+
+// A drop-in solve's arguments: Laplace (cfmm2d, lfmm2d, rfmm2d) and
+// Helmholtz (hfmm2d), as the C-ABI entry points hand them over (abi.hip).
+struct LapReq {
+  bool logk = false;  // lfmm (log kernel, complex densities) vs cfmm (Cauchy)
+  bool realk = false;  // rfmm (log kernel, real densities and outputs)
+  int nd = 1;
+  double eps = 1e-6;
+  int ns = 0, nt = 0;
+  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
+  int ifc = 0, ifd = 0;
+  const void *chg = nullptr, *dip = nullptr;
+  int pg = 0, pgt = 0;
+  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
+  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
+  int ndiv = 0, idivflag = 0, ifnear = 1;
+  bool device = false;
+  hipStream_t stream = nullptr;
+  double* timeinfo = nullptr;
+  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
+};
+
+struct HelmReq {
+  int nd = 1;
+  double eps = 1e-6;
+  std::complex<double> zk = 1.0;
+  int ns = 0, nt = 0;
+  const double *src = nullptr, *trg = nullptr, *dipvec = nullptr;
+  int ifc = 0, ifd = 0;
+  const void *chg = nullptr, *dip = nullptr;
+  int pg = 0, pgt = 0;
+  void *pot = nullptr, *grad = nullptr, *hess = nullptr;
+  void *pott = nullptr, *gradt = nullptr, *hesst = nullptr;
+  int ndiv = 0, idivflag = 0, ifnear = 1;
+  bool device = false;
+  hipStream_t stream = nullptr;
+  double* timeinfo = nullptr;
+  const ForestIn* forest = nullptr;  // the points are a forest's (forest.hip)
+};
+
+// A solve between its front end and its evaluation: the device inputs, the
+// point sort and the tree.  lap_setup / helm_setup run the front end (it
+// reads sizes back from the device); lap_eval / helm_eval queue the rest --
+// strengths in sorted order, the FMM steps, the outputs in the caller's
+// layout -- as stream work alone, which a graph capture can record.
+struct Prepared {
+  Ctx c;
+  SortedPoints P;
+  Plan plan;
+  const double *src = nullptr, *trg = nullptr, *dv = nullptr;
+  const void *chg = nullptr, *dip = nullptr;
+};
+int lap_setup(const LapReq& r, Scratch& S, Prepared& q);
+int lap_eval(const LapReq& r, Scratch& S, Prepared& q);
+int helm_setup(const HelmReq& r, Scratch& S, Prepared& q);
+int helm_eval(const HelmReq& r, Scratch& S, Prepared& q);
+// The drop-in request of a batched problem (leaf size by the eps rule
+// unless the problem sets one).
+void lap_req_of(const fmm2dhip_problem& p, const ForestIn* forest, LapReq& r);
+void helm_req_of(const fmm2dhip_problem& p, const ForestIn* forest, HelmReq& r);
 
 // One problem of the batched/served interface, solved on the calling
 // thread (abi.hip); writes p.ier and p.seconds, returns p.ier.  With a

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <tuple>
 
 #include "engine.h"
@@ -208,7 +209,67 @@ class Copier {
   std::vector<Seg> segs_;
 };
 
-int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<int>& alone) {
+// k_copy_segs over a device table of n segments, in launches of at most
+// 65535 (stream work only: graph captures record it).
+void copy_segs(hipStream_t st, const Seg* d, size_t n) {
+  for (size_t i0 = 0; i0 < n; i0 += 65535)
+    k_copy_segs<<<dim3(64, (unsigned)std::min<size_t>(65535, n - i0)), 256, 0, st>>>(d + i0);
+}
+
+}  // namespace
+
+// One unit of a plan: a forest, or a problem alone, whose front end ran once
+// and whose evaluation -- strengths copied in, the FMM, outputs copied out --
+// is a captured graph, replayed by every execute.  It owns the device memory
+// the graph reads and writes (detached from the thread's workspace) and the
+// pinned pool holding the host data its uploads read (4 MiB).
+struct PlanUnit {
+  std::vector<int> members;
+  bool helm = false;
+  LapReq lr;
+  HelmReq hr;
+  ForestIn fin;
+  Prepared q;
+  HostKeeper keeper;
+  std::vector<void*> blocks;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~PlanUnit() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* b : blocks) (void)hipFree(b);
+  }
+};
+
+namespace {
+
+// Captures pre(); the unit's evaluation; post() into unit->exec.  The
+// capture is always ended, also after an error, so the stream stays usable.
+template <class Pre, class Post>
+int capture_unit(PlanUnit* u, Scratch& S, Pre pre, Post post) {
+  const hipStream_t st = u->q.c.st;
+  // the host data of the captured uploads (level orders, operator jobs)
+  if (int e = u->keeper.reserve(size_t(4) << 20)) return e;
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int e = IER_OK;
+  {
+    CaptureScope cs(&u->keeper);
+    pre();
+    e = u->helm ? helm_eval(u->hr, S, u->q) : lap_eval(u->lr, S, u->q);
+    if (!e) post();
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(st, &g);
+  u->graph = g;
+  if (e) return e;
+  if (ce != hipSuccess || !g) return IER_HIP;
+  HIP_TRY(hipGraphInstantiate(&u->exec, g, nullptr, nullptr, 0));
+  return IER_OK;
+}
+
+int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<int>& alone,
+               PlanUnit* unit) {
   Ctx c;
   int e = get_ctx(c);
   if (e) return e;
@@ -285,7 +346,7 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
     // members unfit for the forest are solved alone; the rest again as one
     std::vector<int> rest;
     for (int i = 0; i < np; ++i) (ok[i] ? rest : alone).push_back(g[i]);
-    if (rest.size() >= 2) return run_forest(probs, rest, alone);
+    if (rest.size() >= 2) return run_forest(probs, rest, alone, unit);
     for (int i : rest) alone.push_back(i);
     return IER_OK;
   }
@@ -323,7 +384,7 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
         std::vector<int> rest;
         for (int j = 0; j < np; ++j)
           if (j != i) rest.push_back(g[j]);
-        if (rest.size() >= 2) return run_forest(probs, rest, alone);
+        if (rest.size() >= 2) return run_forest(probs, rest, alone, unit);
         for (int j : rest) alone.push_back(j);
         return IER_OK;
       }
@@ -406,6 +467,67 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
   q.gradtarg = out(p0.ifpghtarg, 2, W.grad, NT);
   q.hesstarg = out(p0.ifpghtarg, 3, W.hess, NT);
   if (S.err) return S.err;
+  if (unit) {
+    // ---- a plan: front end now, the rest captured (strengths copied in and
+    // outputs copied out by the graph from device tables made here)
+    unit->members = g;
+    unit->helm = helm;
+    unit->fin = fin;
+    unit->q.c = c;
+    std::vector<Seg> sin, sout;
+    for (int i = 0; i < np; ++i) {
+      const fmm2dhip_problem& p = probs[g[i]];
+      const size_t s0 = soff[i], t0 = toff[i];
+      auto seg = [](std::vector<Seg>& v, const void* from, void* to, size_t bytes) {
+        if (bytes) v.push_back({static_cast<const double*>(from), static_cast<double*>(to),
+                                (long long)(bytes / sizeof(double))});
+      };
+      if (chg) seg(sin, p.charge, chg + W.chg * s0, W.chg * p.ns);
+      if (dip) seg(sin, p.dipstr, dip + W.dip * s0, W.dip * p.ns);
+      if (dv) seg(sin, p.dipvec, dv + W.dv * s0, W.dv * p.ns);
+      auto oseg = [&](const void* from, void* to, size_t w, size_t at, int n) {
+        if (from && n) seg(sout, static_cast<const char*>(from) + w * at, to, w * n);
+      };
+      oseg(q.pot, p.pot, W.pot, s0, p.ns);
+      oseg(q.grad, p.grad, W.grad, s0, p.ns);
+      oseg(q.hess, p.hess, W.hess, s0, p.ns);
+      oseg(q.pottarg, p.pottarg, W.pot, t0, p.nt);
+      oseg(q.gradtarg, p.gradtarg, W.grad, t0, p.nt);
+      oseg(q.hesstarg, p.hesstarg, W.hess, t0, p.nt);
+    }
+    Seg* d_in = S.get<Seg>(sin.size());
+    Seg* d_out = S.get<Seg>(sout.size());
+    if (S.err) return S.err;
+    if (!sin.empty())
+      HIP_TRY(hipMemcpyAsync(d_in, sin.data(), sizeof(Seg) * sin.size(), hipMemcpyHostToDevice, c.st));
+    if (!sout.empty())
+      HIP_TRY(hipMemcpyAsync(d_out, sout.data(), sizeof(Seg) * sout.size(), hipMemcpyHostToDevice, c.st));
+    HIP_TRY(hipStreamSynchronize(c.st));  // the host tables die here
+    if (helm) helm_req_of(q, &unit->fin, unit->hr);
+    else lap_req_of(q, &unit->fin, unit->lr);
+    e = helm ? helm_setup(unit->hr, S, unit->q) : lap_setup(unit->lr, S, unit->q);
+    if (e) return e;
+    const int nd = p0.nd;
+    e = capture_unit(
+        unit, S,
+        [&] {
+          copy_segs(c.st, d_in, sin.size());
+          if (helm && dip)
+            k_scale_points<<<nblk((long)NS * nd, 256), 256, 0, c.st>>>(
+                reinterpret_cast<double2*>(dip), NS, nd, d_soff, np, d_fp, 1);
+        },
+        [&] {
+          if (helm) {
+            if (q.grad) k_scale_points<<<nblk((long)NS * 2 * nd, 256), 256, 0, c.st>>>(static_cast<double2*>(q.grad), NS, 2 * nd, d_soff, np, d_fp, 1);
+            if (q.hess) k_scale_points<<<nblk((long)NS * 3 * nd, 256), 256, 0, c.st>>>(static_cast<double2*>(q.hess), NS, 3 * nd, d_soff, np, d_fp, 2);
+            if (NT && q.gradtarg) k_scale_points<<<nblk((long)NT * 2 * nd, 256), 256, 0, c.st>>>(static_cast<double2*>(q.gradtarg), NT, 2 * nd, d_toff, np, d_fp, 1);
+            if (NT && q.hesstarg) k_scale_points<<<nblk((long)NT * 3 * nd, 256), 256, 0, c.st>>>(static_cast<double2*>(q.hesstarg), NT, 3 * nd, d_toff, np, d_fp, 2);
+          }
+          copy_segs(c.st, d_out, sout.size());
+        });
+    if (!e) detach_workspace(unit->blocks);  // before S rewinds the workspace
+    return e;
+  }
   e = solve_problem(q, &fin);
   if (e) return e;
   if (helm) {  // gradients times a, Hessians times a^2 (per point, nd (2|3) values)
@@ -477,7 +599,7 @@ int solve_forest(fmm2dhip_problem* probs, const std::vector<int>& group) {
   std::vector<int> alone;
   int e;
   try {
-    e = run_forest(probs, group, alone);
+    e = run_forest(probs, group, alone, nullptr);
   } catch (const std::bad_alloc&) {
     e = IER_ALLOC;
   } catch (...) {
@@ -497,4 +619,138 @@ int solve_forest(fmm2dhip_problem* probs, const std::vector<int>& group) {
   return failed;
 }
 
+namespace {
+
+// A unit for one problem solved alone (not fused).
+int alone_unit(fmm2dhip_problem& p, int index, PlanUnit* u) {
+  Ctx c;
+  int e = get_ctx(c);
+  if (e) return e;
+  Scratch S(c.st);
+  u->members = {index};
+  u->helm = p.kernel == FMM2DHIP_HELMHOLTZ;
+  u->q.c = c;
+  if (u->helm) helm_req_of(p, nullptr, u->hr);
+  else lap_req_of(p, nullptr, u->lr);
+  e = u->helm ? helm_setup(u->hr, S, u->q) : lap_setup(u->lr, S, u->q);
+  if (e) return e;
+  e = capture_unit(u, S, [] {}, [] {});
+  if (!e) detach_workspace(u->blocks);
+  return e;
+}
+
+// Runs f with the thread's precision mode set to p's.
+template <class F>
+int with_precision(const fmm2dhip_problem& p, F f) {
+  int& mode = thread_precision();
+  const int saved = mode;
+  mode = p.precision == PREC_MIXED ? PREC_MIXED : PREC_FP64;
+  int e;
+  try {
+    e = f();
+  } catch (const std::bad_alloc&) {
+    e = IER_ALLOC;
+  } catch (...) {
+    e = IER_HIP;
+  }
+  mode = saved;
+  return e;
+}
+
+}  // namespace
 }  // namespace fmm
+
+struct fmm2dhip_plan {
+  std::vector<std::unique_ptr<fmm::PlanUnit>> units;
+  fmm2dhip_problem* probs = nullptr;
+  int n = 0;
+};
+
+extern "C" {
+
+fmm2dhip_plan* fmm2dhip_plan_create(int n, fmm2dhip_problem* probs, int* ier) {
+  int e = fmm::IER_OK;
+  fmm2dhip_plan* pl = nullptr;
+  try {
+    if (n <= 0 || !probs) e = fmm::IER_ARGS;
+    for (int i = 0; !e && i < n; ++i)
+      if (probs[i].device_ptrs != 1) e = fmm::IER_ARGS;  // plans take device arrays only
+    if (!e) {
+      pl = new fmm2dhip_plan;
+      pl->probs = probs;
+      pl->n = n;
+      const std::vector<std::vector<int>> groups = fmm::fused_groups(n, probs);
+      std::vector<char> grouped(n, 0);
+      std::vector<int> alone;
+      for (const auto& g : groups) {
+        for (int i : g) grouped[i] = 1;
+        // a first solve sets up what the capture must find ready (cached
+        // operator tables, kernel attributes) and checks the group
+        if (fmm::solve_forest(probs, g) != 0) {
+          e = fmm::IER_HIP;
+          for (int i : g)
+            if (probs[i].ier) e = probs[i].ier;
+          break;
+        }
+        auto u = std::make_unique<fmm::PlanUnit>();
+        std::vector<int> out;
+        e = fmm::with_precision(probs[g[0]], [&] {
+          return fmm::run_forest(probs, g, out, u.get());
+        });
+        if (e) break;
+        alone.insert(alone.end(), out.begin(), out.end());
+        if (!u->members.empty()) pl->units.push_back(std::move(u));
+      }
+      for (int i = 0; !e && i < n; ++i)
+        if (!grouped[i]) alone.push_back(i);
+      for (size_t k = 0; !e && k < alone.size(); ++k) {
+        const int i = alone[k];
+        if (fmm::solve_problem(probs[i]) != 0) {
+          e = probs[i].ier;
+          break;
+        }
+        auto u = std::make_unique<fmm::PlanUnit>();
+        e = fmm::with_precision(probs[i], [&] { return fmm::alone_unit(probs[i], i, u.get()); });
+        if (!e) pl->units.push_back(std::move(u));
+      }
+    }
+  } catch (const std::bad_alloc&) {
+    e = fmm::IER_ALLOC;
+  } catch (...) {
+    e = fmm::IER_HIP;
+  }
+  if (e) {
+    if (pl) {
+      (void)hipDeviceSynchronize();
+      delete pl;
+    }
+    pl = nullptr;
+  }
+  if (ier) *ier = e;
+  return pl;
+}
+
+int fmm2dhip_plan_execute(fmm2dhip_plan* pl) {
+  if (!pl) return fmm::IER_ARGS;
+  fmm::Ctx c;
+  int e = fmm::get_ctx(c);
+  if (e) return e;
+  const double t0 = fmm::wall();
+  for (auto& u : pl->units)
+    if (hipGraphLaunch(u->exec, c.st) != hipSuccess) e = fmm::IER_HIP;
+  if (hipStreamSynchronize(c.st) != hipSuccess) e = fmm::IER_HIP;
+  const double t1 = fmm::wall();
+  for (int i = 0; i < pl->n; ++i) {
+    pl->probs[i].ier = e;
+    pl->probs[i].seconds = t1 - t0;
+  }
+  return e;
+}
+
+void fmm2dhip_plan_destroy(fmm2dhip_plan* pl) {
+  if (!pl) return;
+  (void)hipDeviceSynchronize();
+  delete pl;
+}
+
+}  // extern "C"

@@ -731,8 +731,8 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   double2* mp = S.get<double2>(per * nb);
   double2* lo = S.get<double2>(per * nb);
   if (S.err) return S.err;
-  HIP_TRY(hipMemsetAsync(mp, 0, sizeof(double2) * per * nb, c.st));
-  HIP_TRY(hipMemsetAsync(lo, 0, sizeof(double2) * per * nb, c.st));
+  fill_async(mp, 0u, 4 * per * nb, c.st);
+  fill_async(lo, 0u, 4 * per * nb, c.st);
   const double2* T2 = reinterpret_cast<const double2*>(tab.m2m);
   const double2* TL = reinterpret_cast<const double2*>(tab.l2l);
   const double2* TM = reinterpret_cast<const double2*>(tab.m2l);
@@ -784,7 +784,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
   int* map2 = S.get<int>((size_t)std::max(plan.m2l.n, 1) * 49);
   if (S.err) return S.err;
   if (plan.m2l.n) {
-    HIP_TRY(hipMemsetAsync(map2, 0xff, sizeof(int) * 49 * (size_t)plan.m2l.n, c.st));
+    fill_async(map2, 0xffffffffu, 49 * (size_t)plan.m2l.n, c.st);
     k_map_m2l_all<<<nblk(32L * plan.m2l.n, 256), 256, 0, c.st>>>(W + plan.m2l.off, plan.m2l.n, D,
                                                                 map2);
   }

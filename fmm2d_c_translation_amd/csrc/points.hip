@@ -18,6 +18,38 @@ double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+namespace {
+thread_local HostKeeper* g_keeper = nullptr;
+thread_local bool g_capture = false;
+}  // namespace
+
+bool capturing() { return g_capture; }
+CaptureScope::CaptureScope(HostKeeper* keeper) {
+  g_keeper = keeper;
+  g_capture = true;
+}
+CaptureScope::~CaptureScope() {
+  g_keeper = nullptr;
+  g_capture = false;
+}
+int HostKeeper::reserve(size_t bytes) {
+  if (pool) return IER_OK;
+  if (hipHostMalloc(reinterpret_cast<void**>(&pool), bytes, hipHostMallocDefault) != hipSuccess) {
+    pool = nullptr;
+    return IER_ALLOC;
+  }
+  cap = bytes;
+  used = 0;
+  return IER_OK;
+}
+HostKeeper::~HostKeeper() {
+  if (pool) (void)hipHostFree(pool);
+}
+
+void StepTimer::mark(hipStream_t st) {
+  if (ok && n < 9 && !g_capture) (void)hipEventRecord(ev[n++], st);
+}
+
 StepTimer* step_timer_peek() { return &g_timer; }
 StepTimer* step_timer() {
   StepTimer& t = g_timer;
@@ -205,6 +237,44 @@ void release_thread() {
     (void)hipStreamDestroy(g_stream.st);
   }
   g_stream = ThreadStream{};
+}
+
+namespace {
+__global__ void k_fill32(uint32_t* p, uint32_t v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+}  // namespace
+
+void fill_async(void* p, uint32_t v, size_t n, hipStream_t st) {
+  if (!n) return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  k_fill32<<<(unsigned)blocks, 256, 0, st>>>(static_cast<uint32_t*>(p), v, n);
+}
+
+int upload_small(void* d, const void* h, size_t bytes, hipStream_t st) {
+  if (!bytes) return IER_OK;
+  if (g_keeper) {
+    HostKeeper& K = *g_keeper;
+    const size_t at = (K.used + 255) & ~size_t(255);
+    if (!K.pool || at + bytes > K.cap) return IER_ALLOC;
+    std::memcpy(K.pool + at, h, bytes);
+    K.used = at + bytes;
+    h = K.pool + at;
+  }
+  HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+  return IER_OK;
+}
+
+void detach_workspace(std::vector<void*>& out) {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) return;
+  Arena& A = g_arena[d];
+  for (auto& b : A.blocks) out.push_back(b.first);
+  A.blocks.clear();
+  A.used = 0;
+  A.total = 0;
 }
 
 void* pinned(size_t bytes, int slot) {

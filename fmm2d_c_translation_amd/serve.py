@@ -6,7 +6,8 @@
 struct that points at them; ``Problem.result()`` reads the outputs back in
 the same form as the single-problem calls.  ``batch`` solves a list of
 problems on the library's worker pool; ``batch_fused`` solves compatible
-problems together as one tree (``fmm2dhip_batch_fused``); ``Server`` is the request loop with
+problems together as one tree (``fmm2dhip_batch_fused``); ``Plan`` captures
+repeated solves on fixed points as graphs; ``Server`` is the request loop with
 HBM-budget admission, continuous batching and latency statistics.
 """
 from __future__ import annotations
@@ -118,6 +119,40 @@ def batch_fused(lib, problems, nworkers=8):
     for i, p in enumerate(problems):
         p.c = arr[i]
     return nbad
+
+
+class Plan:
+    """fmm2dhip_plan_* over an array of device-memory problem structs (the
+    structs and the arrays they point at are kept alive by the caller):
+    ``execute()`` re-solves them with the current strengths."""
+
+    def __init__(self, lib, arr):
+        self.lib, self.arr = lib, arr
+        lib.fmm2dhip_plan_create.restype = C.c_void_p
+        lib.fmm2dhip_plan_create.argtypes = [C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        lib.fmm2dhip_plan_execute.restype = C.c_int
+        lib.fmm2dhip_plan_execute.argtypes = [C.c_void_p]
+        lib.fmm2dhip_plan_destroy.argtypes = [C.c_void_p]
+        ier = C.c_int(0)
+        self.h = lib.fmm2dhip_plan_create(len(arr), C.cast(arr, C.c_void_p), C.byref(ier))
+        self.ier = ier.value
+        if not self.h:
+            raise RuntimeError(f"fmm2dhip_plan_create failed: ier {self.ier}")
+
+    def execute(self):
+        return self.lib.fmm2dhip_plan_execute(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.fmm2dhip_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
 
 STAT_KEYS = ("done", "failed", "points", "span_s", "mpts_s", "p50_s", "p99_s", "max_s",

@@ -464,6 +464,25 @@ int fmm2dhip_batch(int n, fmm2dhip_problem *probs, int nworkers);
  * fmm2dhip_batch; returns the number of problems whose ier is not 0. */
 int fmm2dhip_batch_fused(int n, fmm2dhip_problem *probs, int nworkers);
 
+/* Plans: repeated solves on fixed points (iterative solvers, time steps).
+ * fmm2dhip_plan_create takes problems whose arrays are all device memory
+ * (device_ptrs = 1), solves them once as fmm2dhip_batch_fused would, then
+ * builds each group's (or lone problem's) point sort, tree and lists once
+ * and captures the rest of its solve -- strengths copied in, every FMM
+ * step, outputs written -- as a HIP graph.  fmm2dhip_plan_execute replays
+ * the graphs on the calling thread's stream: it reads the current contents
+ * of each problem's charge, dipstr and dipvec arrays and overwrites its
+ * outputs, with the results an fmm2dhip_batch_fused call would give on them;
+ * the sources, targets, flags and array pointers are those of creation and
+ * must stay valid and unchanged.  The plan holds its device memory until
+ * fmm2dhip_plan_destroy.  create returns NULL and sets *ier (102 for host
+ * arrays) on failure; execute returns 0 or an ier and writes each problem's
+ * ier and seconds.  Executes of one plan must not overlap. */
+typedef struct fmm2dhip_plan fmm2dhip_plan;
+fmm2dhip_plan *fmm2dhip_plan_create(int n, fmm2dhip_problem *probs, int *ier);
+int fmm2dhip_plan_execute(fmm2dhip_plan *plan);
+void fmm2dhip_plan_destroy(fmm2dhip_plan *plan);
+
 /* A persistent request loop.  Requests are admitted in arrival order as a
  * worker and device memory free up: a worker keeps the workspace of its
  * largest solve resident, and a request starts only while the workspaces of
