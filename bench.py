@@ -426,10 +426,13 @@ def _device_problems(specs, mode):
 
 
 def batched_c4(lib, nworkers=8):
-    """The 512 problems of workloads.c4_specs solved by fmm2dhip_batch, in
-    FP64 and in mixed precision, from host arrays (PCIe transfers inside the
-    timed region, as a drop-in caller would pay) and from device arrays; and
-    one at a time through the drop-in calls for comparison."""
+    """The 512 problems of workloads.c4_specs, in FP64 and in mixed
+    precision: on the worker pool (fmm2dhip_batch) from host arrays (PCIe
+    transfers inside the timed region, as a drop-in caller would pay) and
+    from device arrays; fused, one launch sequence per group of compatible
+    problems (fmm2dhip_batch_fused), from device and host arrays; as a plan
+    (fmm2dhip_plan_*: trees built once, each solve a replay of captured
+    graphs); and one at a time through the drop-in calls for comparison."""
     import fmm2d_c_translation_amd as F
     from fmm2d_c_translation_amd import abi, serve, workloads
     specs = workloads.c4_specs()
@@ -477,6 +480,25 @@ def batched_c4(lib, nworkers=8):
             "mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4),
             "seconds_all": [round(t, 4) for t in ts],
             "data": "device arrays (resident in HBM), fmm2dhip_batch_fused"}
+        # a plan: the same problems with the trees built once and each solve a
+        # replay of captured graphs (repeated solves on fixed points)
+        t0 = time.perf_counter()
+        plan = serve.Plan(lib, arr)
+        t_create = time.perf_counter() - t0
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            if plan.execute():
+                raise RuntimeError("plan execute failed")
+            ts.append(time.perf_counter() - t0)
+        plan.close()
+        dt = min(ts)
+        out[label + "_plan_replay"] = {
+            "mpts_s": round(npts / dt / 1e6, 3), "seconds": round(dt, 4),
+            "seconds_all": [round(t, 4) for t in ts], "create_seconds": round(t_create, 3),
+            "data": "device arrays, fmm2dhip_plan_execute (graph replay; trees built at create)"}
+        print(f"plan {label}: create {t_create:.3f} s, replay {dt * 1e3:.2f} ms",
+              file=sys.stderr, flush=True)
         del keep
         probs = [serve.Problem(precision=mode, **s) for s in specs]
         t0 = time.perf_counter()
@@ -640,6 +662,9 @@ def main():
                     help="skip the eps = 1e-6 mixed-precision vs FP64 comparison")
     ap.add_argument("--no-configs", action="store_true", help="skip the C1 and C3 sub-lines")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-problem batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in tile choices (no fmm2dhip_autotune on the headline "
+                         "workload before timing)")
     ap.add_argument("--served-seconds", type=float, default=60.0,
                     help="length of each served request stream, FP64 and mixed (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -671,6 +696,18 @@ def main():
     zk, eps = 40.0, 1e-9
     n = nt = args.n
     src, trg, ch, dp, dv = make_inputs(n, nt)
+    tuned = None
+    if not args.no_autotune:
+        # the library's autotuner picks the translation kernels' tile shapes
+        # on the headline problem itself (its choices hold for every solve)
+        tp = serve.Problem(serve.HELMHOLTZ, eps, src, charges=ch, dipstr=dp, dipvec=dv, pg=2,
+                           targets=trg, pgt=2, zk=zk)
+        t0 = time.perf_counter()
+        changed = serve.autotune(lib, tp, nrep=2)
+        tuned = {"changed": changed, "seconds": round(time.perf_counter() - t0, 2),
+                 "table": serve.tune_table(lib)}
+        del tp
+        print(f"autotune: {tuned}", file=sys.stderr, flush=True)
     prob = DeviceProblem(lib, src, trg, ch, dp, dv)
     ndiv = 20  # hndiv2d for eps = 1e-9 (c_translation/src/hndiv2d.c:31-47)
     ti = np.zeros(8)
@@ -738,6 +775,7 @@ def main():
                      "hbm_peak_gbs": HBM_PEAK_GBS,
                      "m2l_tflops": None if m2l_tf is None else round(m2l_tf, 3)},
         "cpu_baseline": None,
+        "autotune": tuned,
     }
     if not args.no_mixed:
         out["mixed_vs_fp64"] = mixed_vs_fp64(lib, prob, src, trg, ch, dp, dv, zk)

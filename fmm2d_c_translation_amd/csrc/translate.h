@@ -576,7 +576,7 @@ inline hipError_t launch_xlate(hipStream_t st, Scratch& S, const int* work, int 
   if (MT > 8 || nslot > 64)
     return launch_translate(st, work, nwork, map, nslot, T, nin, nout, X, xs, Y, ys, nd, lap_log);
   const int ntile = (nwork + 15) / 16, nb = (ntile + 3) / 4;
-  constexpr int kWaves = 4096;
+  const int kWaves = xlate_waves();
   int mw = MT <= 4 ? MT : (MT + 1) / 2, G = 1;
   if (ntile * MT < kWaves) {  // small level: one row tile per wave and split the slots
     mw = 1;
@@ -631,7 +631,7 @@ struct M2LLevel {
 inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
                                     const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
   // row tiles per wave: all of them up to 4, else two groups of ceil(MT/2)
-  auto tiles = [](int MT) { return MT <= 4 ? MT : (MT + 1) / 2; };
+  // (or the autotuner's choice, m2l_tiles)
   for (int mw = 1; mw <= 4; ++mw) {
     XLevels L;
     XLevels32 L32;
@@ -655,7 +655,8 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
     };
     for (const M2LLevel& v : lv) {
       const int MT = (v.P + 15) / 16;
-      if (v.nwork <= 0 || MT > 8 || tiles(MT) != mw) continue;
+      if (v.nwork <= 0 || MT > 8 || m2l_tiles(MT, v.T32 != nullptr) != mw) continue;
+      last_stats().m2l_mt_mask |= 1u << (MT + (v.T32 ? 16 : 0));
       const int nbl = ((v.nwork + 15) / 16 + 3) / 4;
       for (int by = 0; by * mw < MT; ++by) {
         if (L.n == kMaxXLevels || L32.n == kMaxXLevels) flush();

@@ -155,6 +155,23 @@ class Plan:
         return False
 
 
+def autotune(lib, problem, nrep=3):
+    """Run the library's autotuner on one Problem; returns the number of
+    tile choices changed (negative: -ier)."""
+    lib.fmm2dhip_autotune.restype = C.c_int
+    return lib.fmm2dhip_autotune(C.byref(problem.c), int(nrep))
+
+
+def tune_table(lib):
+    """The autotuner's table: {"m2l_fp64": {MT: tiles}, "m2l_fp32": {...},
+    "xlate_waves": n}."""
+    v = (C.c_int * 19)()
+    lib.fmm2dhip_tune_get(v, 19)
+    return {"m2l_fp64": {mt: v[mt] for mt in range(1, 9)},
+            "m2l_fp32": {mt: v[9 + mt] for mt in range(1, 9)},
+            "xlate_waves": v[18]}
+
+
 STAT_KEYS = ("done", "failed", "points", "span_s", "mpts_s", "p50_s", "p99_s", "max_s",
              "peak_workspace_bytes", "budget_bytes", "tracked_tickets")
 
