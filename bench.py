@@ -662,6 +662,9 @@ def main():
                     help="skip the eps = 1e-6 mixed-precision vs FP64 comparison")
     ap.add_argument("--no-configs", action="store_true", help="skip the C1 and C3 sub-lines")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-problem batch")
+    ap.add_argument("--c4-fused-only", action="store_true",
+                    help="only time the 512-problem batch fused from device arrays (FP64, "
+                         "--steps solves after --warmup; for kernel profiles of the forests)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in tile choices (no fmm2dhip_autotune on the headline "
                          "workload before timing)")
@@ -679,6 +682,28 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        return
+    if args.c4_fused_only:
+        import torch
+        import fmm2d_c_translation_amd as F
+        from fmm2d_c_translation_amd import workloads
+        torch.zeros(1, device="cuda:0")
+        lib = F.lib()
+        specs = workloads.c4_specs()
+        npts = sum(s["sources"].shape[1] for s in specs)
+        arr, keep = _device_problems(specs, F.FP64)
+        lib.fmm2dhip_batch_fused.restype = C.c_int
+        ts = []
+        for k in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            if lib.fmm2dhip_batch_fused(len(specs), arr, 8):
+                raise RuntimeError("fused device solve failed")
+            if k >= args.warmup:
+                ts.append(time.perf_counter() - t0)
+        print(json.dumps({"workload": "C4 fused, device arrays, FP64", "problems": len(specs),
+                          "points": npts, "solves": len(ts),
+                          "ms_per_batch": round(1e3 * float(np.mean(ts)), 3),
+                          "mpts_s": round(npts / float(np.mean(ts)) / 1e6, 3)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; the multi-GPU path is a "
