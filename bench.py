@@ -425,7 +425,7 @@ def _device_problems(specs, mode):
     return arr, keep
 
 
-def batched_c4(lib, nworkers=8):
+def batched_c4(lib, nworkers=8, autotune=True):
     """The 512 problems of workloads.c4_specs, in FP64 and in mixed
     precision: on the worker pool (fmm2dhip_batch) from host arrays (PCIe
     transfers inside the timed region, as a drop-in caller would pay) and
@@ -441,6 +441,14 @@ def batched_c4(lib, nworkers=8):
     print(f"batched: {len(specs)} problems, {npts} points", file=sys.stderr, flush=True)
     results = {}
     for mode, label in ((F.FP64, "fp64"), (F.MIXED, "mixed")):
+        if autotune:
+            # tile choices tuned on the batch's own forests (one per group)
+            t0 = time.perf_counter()
+            probs = [serve.Problem(precision=mode, **s) for s in specs]
+            changed = serve.autotune_batch(lib, probs, nrep=2)
+            out[label + "_autotune"] = {"changed": changed,
+                                        "seconds": round(time.perf_counter() - t0, 2)}
+            print(f"autotune {label}: {changed} changed", file=sys.stderr, flush=True)
         probs = [serve.Problem(precision=mode, **s) for s in specs[:nworkers]]
         serve.batch(lib, probs, nworkers)  # workers' streams and workspaces
         probs = [serve.Problem(precision=mode, **s) for s in specs]
@@ -526,6 +534,7 @@ def batched_c4(lib, nworkers=8):
         for k in ("pot", "grad"):
             worst[key] = max(worst.get(key, 0.0), rel(rb[k], ra[k]))
     out["fused_max_rel_l2_diff_vs_alone_fp64"] = {k: e3(v) for k, v in worst.items()}
+    out["tune_table"] = serve.tune_table(lib)
     t0 = time.perf_counter()
     for s in specs:
         if s["kernel"] == serve.HELMHOLTZ:
@@ -816,7 +825,7 @@ def main():
         print("Stokes done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_batch:
-        out["batched_c4"] = batched_c4(lib)
+        out["batched_c4"] = batched_c4(lib, autotune=not args.no_autotune)
     if args.served_seconds > 0:
         out["served_c5"] = served_c5(lib, args.served_seconds, serve.FP64)
         out["served_c5_mixed"] = served_c5(lib, args.served_seconds, serve.MIXED)

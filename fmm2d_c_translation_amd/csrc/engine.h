@@ -290,23 +290,24 @@ struct SolveStats {
   // order+gather, 6 operator setup, 7 whole call
   double host_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double workspace_bytes = 0;  // device workspace the solve took
-  // M2L row-tile counts used, FP64 in bits 1..8, FP32 in bits 17..24
-  unsigned m2l_mt_mask = 0;
+  // M2L row-tile counts MT used: bit (family * 2 + fp32) * 9 + MT
+  uint64_t m2l_mt_mask = 0;
 };
 
 // Tile choices of the translation kernels that the autotuner sets
 // (tune.cpp); 0 leaves the built-in rule.  Process-wide, read at launch.
 struct TuneTable {
-  // [0 FP64 / 1 FP32][row tiles MT of the order]: row tiles per wave of the
-  // all-levels M2L kernels (1..min(4, MT))
-  std::atomic<int> m2l_mw[2][9];
+  // [family: 0 Laplace (and biharmonic), 1 Helmholtz][0 FP64 / 1 FP32]
+  // [row tiles MT of the order]: row tiles per wave of the all-levels M2L
+  // kernels (1..min(4, MT))
+  std::atomic<int> m2l_mw[2][2][9];
   // M2M/L2L: a level with fewer 16-row wave tiles than this takes one row
   // tile per wave and splits the slots over wave groups
   std::atomic<int> xlate_waves;
   TuneTable();
 };
 TuneTable& tune_table();
-int m2l_tiles(int MT, bool fp32);  // row tiles per wave for MT row tiles
+int m2l_tiles(int MT, bool fp32, int family);  // row tiles per wave for MT row tiles
 int xlate_waves();
 double wall();  // seconds, monotonic
 SolveStats& last_stats();

@@ -1275,18 +1275,23 @@ int h2dterms_host(double bs, std::complex<double> zk, double eps, int* ier) {
 }
 
 // Consecutive levels (from lfrom) of a level-sorted work list, grouped by
-// whether each level's bound on k r allows a real H0/H1 series at all
-// (jy_nterms > 0; all one group when rz is false): f(span, xmax) per group,
-// xmax the group's largest bound, which sets its series length.  A tree
-// whose leaves span many sizes -- adaptive trees, forests of problems of
-// many sizes -- keeps the series on its small leaves and the general
-// evaluator on the large ones only.  (Splitting by series length as well
-// cost more in launch tails than the shorter series saved: 1.19 -> 1.26 ms
+// the kind of H0/H1 evaluation each level's bound on k r allows: a short
+// real series (up to 11 terms), the 18-term series, or the general
+// evaluator (all one group when rz is false): f(span, xmax) per group, xmax
+// the group's largest bound, which sets its series length.  A tree whose
+// leaves span many sizes -- adaptive trees, forests of problems of many
+// sizes -- keeps short series on its small leaves and the general
+// evaluator on the large ones only.  (Splitting by every series length cost
+// more in launch tails than 8 instead of 9 terms saved: 1.19 -> 1.26 ms
 // for the near field of 1M + 1M uniform points.)
 template <class XF, class F>
 void by_tier(const WorkList& wl, int lfrom, int L, bool rz, XF xmax, F f) {
   int l = std::max(lfrom, 0);
-  auto tier = [&](int q) { return rz && jy_nterms(xmax(q)) > 0 ? 1 : 0; };
+  auto tier = [&](int q) {
+    if (!rz) return 0;
+    const int nr = jy_nterms(xmax(q));
+    return nr == 0 ? 0 : nr <= JY_NTERMS2 ? 1 : 2;
+  };
   while (l <= L) {
     if (!wl.at(l).n) {
       ++l;
@@ -1567,7 +1572,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
       continue;
     }
     m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0,
-                          dT32 ? dT32 + tab[l].m2l[0] : nullptr});
+                          dT32 ? dT32 + tab[l].m2l[0] : nullptr, nullptr, 1});
   }
   if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
   tm->mark(c.st);

@@ -627,6 +627,7 @@ struct M2LLevel {
   // Laplace: per-box sizes whose log replaces lap_log (a forest's levels
   // hold boxes of different sizes), or null
   const double* lap_bsb = nullptr;
+  int family = 0;  // for the tile choice: 0 Laplace and biharmonic, 1 Helmholtz
 };
 inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
                                     const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
@@ -655,8 +656,9 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
     };
     for (const M2LLevel& v : lv) {
       const int MT = (v.P + 15) / 16;
-      if (v.nwork <= 0 || MT > 8 || m2l_tiles(MT, v.T32 != nullptr) != mw) continue;
-      last_stats().m2l_mt_mask |= 1u << (MT + (v.T32 ? 16 : 0));
+      const bool f32 = v.T32 != nullptr;
+      if (v.nwork <= 0 || MT > 8 || m2l_tiles(MT, f32, v.family) != mw) continue;
+      last_stats().m2l_mt_mask |= uint64_t(1) << ((v.family * 2 + (f32 ? 1 : 0)) * 9 + MT);
       const int nbl = ((v.nwork + 15) / 16 + 3) / 4;
       for (int by = 0; by * mw < MT; ++by) {
         if (L.n == kMaxXLevels || L32.n == kMaxXLevels) flush();

@@ -162,14 +162,27 @@ def autotune(lib, problem, nrep=3):
     return lib.fmm2dhip_autotune(C.byref(problem.c), int(nrep))
 
 
+def autotune_batch(lib, problems, nrep=3):
+    """The autotuner on every fused group of a list of Problems (each group
+    timed as one forest); returns the number of tile choices changed."""
+    arr = (fmm2dhip_problem * len(problems))()
+    for i, p in enumerate(problems):
+        arr[i] = p.c
+    lib.fmm2dhip_autotune_batch.restype = C.c_int
+    return lib.fmm2dhip_autotune_batch(len(problems), arr, int(nrep))
+
+
+TUNE_KEYS = ("m2l_laplace_fp64", "m2l_laplace_fp32", "m2l_helmholtz_fp64", "m2l_helmholtz_fp32")
+
+
 def tune_table(lib):
-    """The autotuner's table: {"m2l_fp64": {MT: tiles}, "m2l_fp32": {...},
+    """The autotuner's table: {"m2l_laplace_fp64": {MT: tiles}, ...,
     "xlate_waves": n}."""
-    v = (C.c_int * 19)()
-    lib.fmm2dhip_tune_get(v, 19)
-    return {"m2l_fp64": {mt: v[mt] for mt in range(1, 9)},
-            "m2l_fp32": {mt: v[9 + mt] for mt in range(1, 9)},
-            "xlate_waves": v[18]}
+    v = (C.c_int * 37)()
+    lib.fmm2dhip_tune_get(v, 37)
+    out = {k: {mt: v[9 * j + mt] for mt in range(1, 9)} for j, k in enumerate(TUNE_KEYS)}
+    out["xlate_waves"] = v[36]
+    return out
 
 
 STAT_KEYS = ("done", "failed", "points", "span_s", "mpts_s", "p50_s", "p99_s", "max_s",

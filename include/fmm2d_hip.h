@@ -485,19 +485,24 @@ void fmm2dhip_plan_destroy(fmm2dhip_plan *plan);
 
 /* Autotuner of the translation kernels' tile shapes (no counterpart in the
  * reference).  The process-wide table holds, for each M2L order's count MT
- * of 16-row tiles (1..8), FP64 and FP32 apart, the row tiles each wave of
- * the M2L kernels computes (1..min(4, MT)), and the level size (in 16-row
- * wave tiles) below which M2M/L2L split their slots over wave groups.
- * fmm2dhip_autotune solves *p (any kernel, host or device arrays; outputs
- * overwritten) 1 + nrep times per candidate, times the M2L (and M2M + L2L)
- * steps with the solve's own step events, and keeps each choice that is at
- * least 2% faster than the current one; it returns the number of entries
- * changed, or -ier.  fmm2dhip_tune_get writes the table as 19 ints (FP64
- * MT 0..8, FP32 MT 0..8, the M2M/L2L threshold; index 0 of each half is
- * unused); save/load write and read it as text (0 on success for save, the
- * entries read for load, -1 when the file cannot be opened).  At start the
- * table is loaded from the file FMM2DHIP_TUNE_FILE names, if any. */
+ * of 16-row tiles (1..8), Laplace (with biharmonic) and Helmholtz, FP64 and
+ * FP32 apart, the row tiles each wave of the M2L kernels computes
+ * (1..min(4, MT)), and the level size (in 16-row wave tiles) below which
+ * M2M/L2L split their slots over wave groups.  fmm2dhip_autotune solves *p
+ * (any kernel, host or device arrays; outputs overwritten) 1 + nrep times
+ * per candidate, times the M2L (and M2M + L2L) steps with the solve's own
+ * step events, and keeps each choice that is at least 2% faster than the
+ * current one; it returns the number of entries changed, or -ier.
+ * fmm2dhip_autotune_batch does the same for every group of probs that
+ * fmm2dhip_batch_fused would solve as one forest, timing the group's forest.
+ * fmm2dhip_tune_get writes the table as 37 ints: 9 per (family, precision)
+ * in the order Laplace FP64, Laplace FP32, Helmholtz FP64, Helmholtz FP32
+ * (MT 0..8, index 0 unused), then the M2M/L2L threshold; save/load write and
+ * read it as text (0 on success for save, the entries read for load, -1
+ * when the file cannot be opened).  At start the table is loaded from the
+ * file FMM2DHIP_TUNE_FILE names, if any. */
 int fmm2dhip_autotune(fmm2dhip_problem *p, int nrep);
+int fmm2dhip_autotune_batch(int n, fmm2dhip_problem *probs, int nrep);
 int fmm2dhip_tune_get(int *out, int n);
 int fmm2dhip_tune_save(const char *path);
 int fmm2dhip_tune_load(const char *path);

@@ -28,8 +28,9 @@ from fmm2d_c_translation_amd import serve
 lib = F.lib()
 t = serve.tune_table(lib)
 print(repr(t))
-open({str(path)!r}, "w").write("m2l_mw fp64 3 1\\nm2l_mw fp32 6 4\\nxlate_waves 2048\\n"
-                            "m2l_mw fp64 5 9\\nbogus line\\n")
+open({str(path)!r}, "w").write("m2l_mw laplace fp64 3 1\\nm2l_mw helmholtz fp32 6 4\\n"
+                            "xlate_waves 2048\\nm2l_mw laplace fp64 5 9\\n"
+                            "m2l_mw poisson fp64 2 1\\nbogus line\\n")
 print(lib.fmm2dhip_tune_load({str(path)!r}.encode()))
 print(repr(serve.tune_table(lib)))
 print(lib.fmm2dhip_tune_save({str(path) + '.out'!r}.encode()))
@@ -41,15 +42,18 @@ print(lib.fmm2dhip_tune_save({str(path) + '.out'!r}.encode()))
     assert out.returncode == 0, out.stderr
     lines = out.stdout.strip().splitlines()
     t0 = eval(lines[0])
-    assert t0["m2l_fp64"] == _defaults() and t0["m2l_fp32"] == _defaults()
+    for k in serve.TUNE_KEYS:
+        assert t0[k] == _defaults(), k
     assert t0["xlate_waves"] == 4096
     assert int(lines[1]) == 3  # the two valid m2l lines and the threshold
     t1 = eval(lines[2])
-    assert t1["m2l_fp64"][3] == 1 and t1["m2l_fp32"][6] == 4 and t1["xlate_waves"] == 2048
-    assert t1["m2l_fp64"][5] == _defaults()[5]  # 9 tiles per wave is refused
+    assert t1["m2l_laplace_fp64"][3] == 1 and t1["m2l_helmholtz_fp32"][6] == 4
+    assert t1["m2l_helmholtz_fp64"][3] == _defaults()[3]  # families apart
+    assert t1["xlate_waves"] == 2048
+    assert t1["m2l_laplace_fp64"][5] == _defaults()[5]  # 9 tiles per wave is refused
     assert int(lines[3]) == 0
     saved = open(str(path) + ".out").read()
-    assert "m2l_mw fp64 3 1" in saved and "m2l_mw fp32 6 4" in saved
+    assert "m2l_mw laplace fp64 3 1" in saved and "m2l_mw helmholtz fp32 6 4" in saved
     assert "xlate_waves 2048" in saved
 
 
@@ -71,7 +75,7 @@ def test_autotune_keeps_results():
         assert changed >= 0, changed
         print(f"autotune {k}: {changed} choices changed -> {serve.tune_table(lib)}", flush=True)
     t = serve.tune_table(lib)
-    for half in ("m2l_fp64", "m2l_fp32"):
+    for half in serve.TUNE_KEYS:
         for mt, v in t[half].items():
             assert 1 <= v <= min(4, mt)
     assert t["xlate_waves"] in (1024, 2048, 4096, 8192)
