@@ -336,6 +336,7 @@ struct Int4Sum {
 
 struct ListOut {
   int *o1, *o2, *o3, *o4, *l1, *l2, *l3, *l4;
+  long cap1, cap2, cap3, cap4;  // entries each list array holds
 };
 
 // Lists of box b (tree_routs2d.c:369-510):
@@ -367,8 +368,10 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
     out.o1[b] = o.x; out.o2[b] = o.y; out.o3[b] = o.z; out.o4[b] = o.w;
   }
   int c1 = 0, c2 = 0, c3 = 0, c4 = 0;
-  auto put = [&](int* lst, int base, int& c, int v) {
-    if (FILL) lst[base + c] = v;
+  // an entry past the array's capacity is counted but not written; the
+  // host then finds the total above the capacity and rejects the tree
+  auto put = [&](int* lst, long cap, int base, int& c, int v) {
+    if (FILL && (long)base + c < cap) lst[base + c] = v;
     ++c;
   };
   const int l = level[b];
@@ -376,7 +379,7 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
   int ix, iy;
   key_cell(key[b], ix, iy);
   if (l == 0) {
-    if (leaf) put(out.l1, o.x, c1, 0);
+    if (leaf) put(out.l1, out.cap1, o.x, c1, 0);
   } else {
     const int p = parent[b];
     for (int s = 0; s < 9; ++s) {
@@ -388,14 +391,14 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
         for (int q = 0; q < 4; ++q) {
           int kx, ky;
           key_cell(key[kk[q]], kx, ky);
-          if (abs(kx - ix) >= 2 || abs(ky - iy) >= 2) put(out.l2, o.y, c2, kk[q]);
+          if (abs(kx - ix) >= 2 || abs(ky - iy) >= 2) put(out.l2, out.cap2, o.y, c2, kk[q]);
         }
       } else if (s != 4) {
         int px, py;
         key_cell(key[P], px, py);
         bool touch = ix >= 2 * px - 1 && ix <= 2 * px + 2 && iy >= 2 * py - 1 && iy <= 2 * py + 2;
-        if (touch) put(out.l1, o.x, c1, P);
-        else put(out.l4, o.w, c4, P);
+        if (touch) put(out.l1, out.cap1, o.x, c1, P);
+        else put(out.l4, out.cap4, o.w, c4, P);
       }
     }
     if (leaf) {
@@ -403,7 +406,7 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
         int J = nbr[9 * b + s];
         if (J < 0) continue;
         if (!nchild[J]) {
-          put(out.l1, o.x, c1, J);
+          put(out.l1, out.cap1, o.x, c1, J);
           continue;
         }
         int4 ch = child[J];
@@ -412,8 +415,8 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
           int kx, ky;
           key_cell(key[kk[q]], kx, ky);
           bool touch = kx >= 2 * ix - 1 && kx <= 2 * ix + 2 && ky >= 2 * iy - 1 && ky <= 2 * iy + 2;
-          if (touch) put(out.l1, o.x, c1, kk[q]);
-          else put(out.l3, o.z, c3, kk[q]);
+          if (touch) put(out.l1, out.cap1, o.x, c1, kk[q]);
+          else put(out.l3, out.cap3, o.z, c3, kk[q]);
         }
       }
     }
@@ -476,8 +479,8 @@ __device__ inline unsigned wl_mask(const WlArgs& a, int b) {
 // point-to-point distance over list 1: per axis |centre difference| + the
 // two half sizes, over leaves with points and list-1 boxes with sources
 __global__ void __launch_bounds__(WLB)
-k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, double* pairs,
-           unsigned long long* m2l_lev) {
+k_wl_count(WlArgs a, const int* o1, const int* l1, long cap1, int* bcnt, int nblocks,
+           double* pairs, unsigned long long* m2l_lev) {
   __shared__ int sc[WL_N];
   __shared__ unsigned long long sm2l[kMaxDepth + 1];
   __shared__ unsigned long long sr2[kMaxDepth + 1];  // per level: squared bound, as bits
@@ -503,7 +506,10 @@ k_wl_count(WlArgs a, const int* o1, const int* l1, int* bcnt, int nblocks, doubl
       const double2 cb = a.ctr[b];
       const double hb = 0.5 * a.bs[a.level[b]];
       double srcs = 0;
-      for (int k = o1[b]; k < o1[b + 1]; ++k) {
+      // entries past the array's capacity were not written (the host
+      // rejects such a tree after this launch)
+      const int kend = (int)min((long)o1[b + 1], cap1);
+      for (int k = o1[b]; k < kend; ++k) {
         const int j = l1[k];
         int2 r = a.srng[j];
         srcs += r.y - r.x;
@@ -912,24 +918,23 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   size_t tb = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveScan(nullptr, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
   void* tmp = S.get<char>(tb);
-  int* tot4 = S.get<int>(4);
   if (S.err) return S.err;
   HIP_TRY(hipcub::DeviceScan::ExclusiveScan(tmp, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
-  int tot[4];
-  HIP_TRY(hipMemcpyAsync(tot4, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
-  e = sync_read(c, tot4, 4, tot, plan.sort_err);
-  if (e) return e;
-  // list lengths per box are bounded (13, 27, 20 and 5 after level restriction;
-  // twice that is a generous cap on any tree this code builds)
-  {
-    const long lim[4] = {26L * nb, 54L * nb, 40L * nb, 10L * nb};
-    for (int k = 0; k < 4; ++k)
-      if (tot[k] < 0 || tot[k] > lim[k]) return IER_HIP;
-  }
-  lo.l1 = S.get<int>(std::max(tot[0], 1));
-  lo.l2 = S.get<int>(std::max(tot[1], 1));
-  lo.l3 = S.get<int>(std::max(tot[2], 1));
-  lo.l4 = S.get<int>(std::max(tot[3], 1));
+  // List lengths per box are bounded (13, 27, 20 and 5 after level
+  // restriction); the arrays hold twice that per box, so their totals need
+  // no read-back here: they come back with the work-list counts below and
+  // a total above its array's capacity rejects the tree there, before any
+  // kernel other than the work-list counter (which stops at the capacity)
+  // reads the lists.
+  const long lim[4] = {26L * nb, 54L * nb, 40L * nb, 10L * nb};
+  lo.cap1 = lim[0];
+  lo.cap2 = lim[1];
+  lo.cap3 = lim[2];
+  lo.cap4 = lim[3];
+  lo.l1 = S.get<int>(lim[0]);
+  lo.l2 = S.get<int>(lim[1]);
+  lo.l3 = S.get<int>(lim[2]);
+  lo.l4 = S.get<int>(lim[3]);
   if (S.err) return S.err;
   k_lists<true><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
                                                     nullptr, off, lo);
@@ -942,7 +947,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   // distance bound, over all levels and per level (5 + kMaxDepth + 1
   // doubles as ints)
   const int nsmall = ninfo + WL_N * W + 2 * (kMaxDepth + 1) + 10 + 2 * (kMaxDepth + 1);
-  int* small = S.get<int>(nsmall + 2);
+  int* small = S.get<int>(nsmall + 6);  // + the four list totals
   int* bcnt = S.get<int>((size_t)WL_N * nbk);
   if (S.err) return S.err;
   int* info = small;
@@ -950,9 +955,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   unsigned long long* m2l_lev = reinterpret_cast<unsigned long long*>(small + ((ninfo + WL_N * W + 1) & ~1));
   double* pairs = reinterpret_cast<double*>(m2l_lev + kMaxDepth + 1);
   HIP_TRY(hipMemsetAsync(small, 0, sizeof(int) * (nsmall + 2), c.st));
+  HIP_TRY(hipMemcpyAsync(small + nsmall + 2, off + nb, sizeof(int4), hipMemcpyDeviceToDevice, c.st));
   WlArgs wa{level, nchild, srng, trng, cnt, nb, ifpgh, ifpghtarg, ctr, dbs, key,
             forest ? *forest : ForestDev{}};
-  k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, bcnt, nbk, pairs, m2l_lev);
+  k_wl_count<<<nbk, WLB, 0, c.st>>>(wa, lo.o1, lo.l1, lo.cap1, bcnt, nbk, pairs, m2l_lev);
   k_wl_scan<<<1, 64 * WL_N, 0, c.st>>>(bcnt, nbk, info);
   // every list holds at most nb entries; the L2L children at most 4 nb
   int* work = S.get<int>((size_t)WL_N * nb + 4 * (size_t)nb + 1);
@@ -961,9 +967,14 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   k_wl_levels<<<nblk(WL_N * W, 64), 64, 0, c.st>>>(lv, info, work, lev);
   k_wl_kids<<<nblk(nb, 256), 256, 0, c.st>>>(info, lev, W, child, work);
   HIP_TRY(hipGetLastError());
-  std::vector<int> hs(nsmall + 2);
-  e = sync_read(c, small, nsmall + 2, hs.data(), plan.sort_err);
+  std::vector<int> hs(nsmall + 6);
+  e = sync_read(c, small, nsmall + 6, hs.data(), plan.sort_err);
   if (e) return e;
+  int tot[4];
+  for (int k = 0; k < 4; ++k) {
+    tot[k] = hs[nsmall + 2 + k];
+    if (tot[k] < 0 || tot[k] > lim[k]) return IER_HIP;
+  }
 
   // ---- host view
   plan.nb = nb;
