@@ -67,26 +67,48 @@ FMM_HD void hank_series(cd z, cd& h0, cd& h1) {
   h1 = j1 + mul_i(y1);
 }
 
+// Tier of a series length (jy_series.h), -1 for any other length
+template <int N>
+constexpr int jy_tier() {
+  return N == JY_NTERMS0 ? 0 : N == JY_NTERMS1 ? 1 : N == JY_NTERMS2 ? 2 : N == JY_NTERMS3 ? 3 : -1;
+}
+// Polynomial c[0..n) at w and its derivative: Horner with the derivative
+// chain beside it, its first step peeled (it starts from the top coefficient)
+template <int n>
+FMM_HD void horner_d(const double* c, double w, double& v, double& dv) {
+  static_assert(n >= 2, "two terms at least");
+  dv = c[n - 1];
+  v = fma(c[n - 1], w, c[n - 2]);
+#pragma unroll
+  for (int k = n - 3; k >= 0; --k) {
+    dv = fma(dv, w, v);
+    v = fma(v, w, c[k]);
+  }
+}
+
 // H0 = J0 + i Y0 and H1 = J1 + i Y1 of a real argument 0 < x <= JY_XMAX
 // of the chosen N (jy_series.h), given xi = 1/x: the ascending series of hank_series in
 // real arithmetic, J0 = P(w) and the Y0 remainder Q(w) in w = x^2/4 by
-// Horner with their derivatives, J1 = -P'(w) x/2 and
+// Horner with their derivatives (for the four tier lengths, the tier's
+// economised P and Q), J1 = -P'(w) x/2 and
 // Y1 = -(2/pi) [J0/x - (ln(x/2) + gamma) J1 + Q'(w) x/2]
 // (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
 // Core = true: x/2 must be a positive normal double (the table log,
 // log_tab, without special cases).
 template <int N, bool Core = false>
 FMM_HD void hank_series_real_w(double x, double xi, double w, cd& h0, cd& h1) {
-  // Horner for P and P' (and Q, Q'), the first step of the derivative
-  // chain peeled (it starts from the top coefficient)
-  double dp = JY_JA[N - 1], p = fma(JY_JA[N - 1], w, JY_JA[N - 2]);
-  double dq = JY_YC[N - 1], q = fma(JY_YC[N - 1], w, JY_YC[N - 2]);
-#pragma unroll
-  for (int k = N - 3; k >= 0; --k) {
-    dp = fma(dp, w, p);
-    p = fma(p, w, JY_JA[k]);
-    dq = fma(dq, w, q);
-    q = fma(q, w, JY_YC[k]);
+  double p, dp, q, dq;
+  constexpr int T = jy_tier<N>();
+  if constexpr (T >= 0) {
+    // the tier's economised P and Q (x <= JY_XMAX<T>)
+    horner_d<JY_TIER[T].n[0]>(JY_TIER[T].c[0], w, p, dp);
+    horner_d<JY_TIER[T].n[1]>(JY_TIER[T].c[1], w, q, dq);
+  } else if constexpr (N >= 2) {
+    // the series to N terms
+    horner_d<N>(JY_JA, w, p, dp);
+    horner_d<N>(JY_YC, w, q, dq);
+  } else {
+    p = dp = q = dq = 0.0;  // no series of fewer than two terms
   }
   const double hx = 0.5 * x, lg = (Core ? log_tab(hx) : log_pos(hx)) + kEuler;
   const double j1 = -dp * hx;
