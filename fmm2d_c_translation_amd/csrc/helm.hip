@@ -315,25 +315,28 @@ __device__ __noinline__ H01 hank_real_point(double x) {
 // for multipoles by Miller's downward recurrence run twice (once for the
 // normalisation, once to emit the values, which equal jvals_real's bit for
 // bit) -- and forms its terms of the coefficients in the order they become
-// available.  A 16 x 16 tile in LDS then hands the 16 terms of each
-// coefficient to one lane, which sums them in point order and adds the sum
-// to the expansion.  The per-point factors (charge; dipole terms of orders
+// available.  A tile in LDS (FG_ROWS coefficients of 16 terms) then hands
+// the 16 terms of each coefficient to one lane, which sums them in point
+// order and adds the sum to the expansion.  The per-point factors (charge; dipole terms of orders
 // m > 0, m < 0 and m = 0) and the term of coefficient m,
 //   t_m = (F0 V_n + FA V_{n-1} + FB V_{n+1}) e^{-i m theta},  n = |m|,
 // are those of k_hform (helmrouts2d.c:294-391, 513-612).  Needs p <= 63.
 // ---------------------------------------------------------------------
 constexpr int FG_PAD = 17;  // tile row length (16 terms + 1: no bank conflicts)
+// tile rows: coefficients summed per flush.  8 rows keep the tile at 8.5 KB
+// per wave, so LDS admits four waves per SIMD (16 rows: 17 KB, two)
+constexpr int FG_ROWS = 8;
 
 template <bool MP, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_hform_g(const int* work, int nwork, DTree D, const HLev* lev, double k, const double2* pts,
           const double2* chg, const double2* dip, const double* dv, int N, int nd, size_t per,
           double2* out) {
-  __shared__ cd tile[4 * 16 * FG_PAD];
+  __shared__ cd tile[4 * FG_ROWS * FG_PAD];
   const int grp = (int)threadIdx.x >> 4, u = (int)threadIdx.x & 15;
   const int w = blockIdx.x * 4 + grp;
   if (w >= nwork) return;  // the whole group leaves; only wave-level syncs below
-  cd* T = tile + grp * 16 * FG_PAD;
+  cd* T = tile + grp * FG_ROWS * FG_PAD;
   const int b = work[w];
   const int L = D.level[b];
   const int p = lev[L].p;
@@ -378,18 +381,24 @@ k_hform_g(const int* work, int nwork, DTree D, const HLev* lev, double k, const 
           F0d = (qf * rsd) * ((zm + zi) * v1 + mul_i(zm - zi) * v2);
         }
         int e = 0;  // emissions so far (uniform over the group)
+        // lane u < FG_ROWS adds coefficient base + u of each group of FG_ROWS
+        // emissions; its current value is loaded one group ahead, so the
+        // load's latency hides behind the group's emissions (each coefficient
+        // has one writer, this lane, and is written once per group)
+        double2 pre = u < FG_ROWS && u < ncoef ? o[m_of(u)] : make_double2(0.0, 0.0);
         auto emit = [&](cd term) {
-          T[(e & 15) * FG_PAD + u] = term;
+          T[(e % FG_ROWS) * FG_PAD + u] = term;
           ++e;
-          if ((e & 15) == 0 || e == ncoef) {
-            const int base = (e - 1) & ~15, nfill = e - base;
+          if (e % FG_ROWS == 0 || e == ncoef) {
+            const int base = (e - 1) / FG_ROWS * FG_ROWS, nfill = e - base;
             wave_sync();
             if (u < nfill) {
               cd sum = mk(0, 0);
+#pragma unroll 4
               for (int q = 0; q < 16; ++q) sum += T[u * FG_PAD + q];
-              const int m = m_of(base + u);
-              o[m] = make_double2(o[m].x + sum.x, o[m].y + sum.y);
+              o[m_of(base + u)] = make_double2(pre.x + sum.x, pre.y + sum.y);
             }
+            if (u < FG_ROWS && base + FG_ROWS + u < ncoef) pre = o[m_of(base + FG_ROWS + u)];
             wave_sync();
           }
         };
@@ -903,6 +912,11 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
   const bool cp_mode = C && !Q && PG == 1;
   const double izk = NR > 0 ? 1.0 / zk.x : 0.0;
   const double kk4 = zk.x * zk.x * 0.25;  // w = (k r)^2 / 4 = kk4 r^2
+  // real k: k/2, the threshold on k r / 2 (a pair skips on k r < thresh,
+  // i.e. k r / 2 < thresh / 2, both sides scaled exactly), and the factors
+  // of H1 / r below
+  const double hk = 0.5 * zk.x, hth = 0.5 * thresh;
+  const double gc1 = -(2.0 / kPi) * izk, gc2 = -zk.x / kPi;
   for (int t0 = tr.x; t0 < tr.y; t0 += WAVE) {
     const int Tc = min(WAVE, tr.y - t0);
     const int K = WAVE / Tc;
@@ -928,22 +942,31 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
         if (act && NR > 0 && PG <= 2) {
           // real k, branch-free: a pair under the threshold (a coincident one
           // included, whose r^2 is raised to the smallest normal double) runs
-          // the series at its own tiny but finite arguments with 1/r and H0
-          // set to 0, which zeroes every term it adds.
-          // With g = H1 / r and k H2 = 2 g - k H0, the charge and dipole
-          // gradients are -a (dx, dy) + b v, a = g c + q k H2 (d.v) / r^2,
-          // b = q g, and the dipole potential is b (d.v).
+          // the series at its own tiny but finite arguments with H0 and
+          // g = H1 / r set to 0, which zeroes every term it adds.
+          // With k H2 = 2 g - k H0, the charge and dipole gradients are
+          // -a (dx, dy) + b v, a = g c + q k H2 (d.v) / r^2, b = q g, and the
+          // dipole potential is b (d.v).
           for (int m = k; m < cnt; m += K) {
             const double dx = x.x - sx[m].x, dy = x.y - sx[m].y;
             const double rr = fmax(dx * dx + dy * dy, 0x1p-1022);
-            const double ri0 = rsqrt_pos(rr);
-            const double az = zk.x * (rr * ri0);
-            const bool skip = cp_mode ? az <= thresh : az < thresh;
-            const double ri = skip ? 0.0 : ri0;
-            cd h0, h1;
-            hank_series_real_w<(NR > 0 ? NR : 1), true>(az, ri * izk, kk4 * rr, h0, h1);
-            if (skip) h0 = cd{0.0, 0.0};
-            const cd g = h1 * ri;
+            const double ri = rsqrt_pos(rr);
+            const double hx = hk * (rr * ri);  // k r / 2
+            const bool skip = cp_mode ? hx <= hth : hx < hth;
+            // H0 = P + i (2/pi) (lg P + Q) and, from J1 = -P' k r / 2 and
+            // Y1 = -(2/pi) [P / (k r) + (k r / 2)(lg P' + Q')] (A&S 9.1.11,
+            // 9.1.13), g = -(k/2) P' - i (2/pi) [P / (k r^2) + (k/2)(lg P' + Q')],
+            // lg = ln(k r / 2) + gamma, P and Q in w = (k r / 2)^2
+            double p, dp, q, dq;
+            jy_pq<(NR > 0 ? NR : 1)>(kk4 * rr, p, dp, q, dq);
+            const double lg = log_tab(hx) + kEuler;
+            const double ri2 = ri * ri;
+            cd h0 = cd{p, (lg * p + q) * (2.0 / kPi)};
+            cd g = cd{dp * -hk, fma(p * ri2, gc1, fma(lg, dp, dq) * gc2)};
+            if (skip) {
+              h0 = cd{0.0, 0.0};
+              g = cd{0.0, 0.0};
+            }
             cd a = mk(0, 0);
             if (C) {
               const double2 cv = sc[m];
@@ -960,7 +983,7 @@ k_hp2p(const int* work, DTree D, cd zk, const double2* src, const double2* chg,
               A.Pd += bq * dv;
               if (PG >= 2) {
                 const cd kh2 = g * 2.0 - h0 * zk.x;
-                a += (q * kh2) * (dv * (ri * ri));
+                a += (q * kh2) * (dv * ri2);
                 A.Gx += bq * v1;
                 A.Gy += bq * v2;
               }

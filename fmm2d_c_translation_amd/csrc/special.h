@@ -95,9 +95,10 @@ FMM_HD void horner_d(const double* c, double w, double& v, double& dv) {
 // (Abramowitz & Stegun 9.1.10, 9.1.13 and Y1 = -Y0').
 // Core = true: x/2 must be a positive normal double (the table log,
 // log_tab, without special cases).
-template <int N, bool Core = false>
-FMM_HD void hank_series_real_w(double x, double xi, double w, cd& h0, cd& h1) {
-  double p, dp, q, dq;
+// P(w), P'(w), Q(w), Q'(w) of series length N (the tier's economised forms
+// for the four tier lengths)
+template <int N>
+FMM_HD void jy_pq(double w, double& p, double& dp, double& q, double& dq) {
   constexpr int T = jy_tier<N>();
   if constexpr (T >= 0) {
     // the tier's economised P and Q (x <= JY_XMAX<T>)
@@ -110,6 +111,12 @@ FMM_HD void hank_series_real_w(double x, double xi, double w, cd& h0, cd& h1) {
   } else {
     p = dp = q = dq = 0.0;  // no series of fewer than two terms
   }
+}
+
+template <int N, bool Core = false>
+FMM_HD void hank_series_real_w(double x, double xi, double w, cd& h0, cd& h1) {
+  double p, dp, q, dq;
+  jy_pq<N>(w, p, dp, q, dq);
   const double hx = 0.5 * x, lg = (Core ? log_tab(hx) : log_pos(hx)) + kEuler;
   const double j1 = -dp * hx;
   const double y0 = (lg * p + q) * (2.0 / kPi);
