@@ -353,9 +353,16 @@ k_hform_g(const int* work, int nwork, DTree D, const HLev* lev, double k, const 
   };
   for (int d = 0; d < nd; ++d) {
     double2* o = out + (size_t)b * per + (size_t)d * ncoef + p;  // o[m], |m| <= p
+    // source ranges of 16 boxes at a time, one box per lane (no chain of
+    // dependent loads per box), handed out by shuffles
+    int2 mrg = make_int2(0, 0);
+    auto load_rng = [&](int sb0) {
+      if (sb0 + u < nsb) mrg = D.srng[MP ? b : D.l4[D.o4[b] + sb0 + u]];
+    };
+    load_rng(0);
     for (int sb = 0; sb < nsb; ++sb) {
-      const int jb = MP ? b : D.l4[D.o4[b] + sb];
-      const int2 rg = D.srng[jb];
+      if (sb > 0 && (sb & 15) == 0) load_rng(sb);
+      const int2 rg = make_int2(__shfl(mrg.x, sb & 15, 16), __shfl(mrg.y, sb & 15, 16));
       for (int c0 = rg.x; c0 < rg.y; c0 += 16) {
         const int j = c0 + u;
         const bool ok = j < rg.y;
@@ -722,36 +729,115 @@ __device__ void eval_lo_sd(const cd* E, int top_n, cd zk, double rs, double r, c
 // SdTerm (the derivative expansions computed once per box instead of once
 // per point and order), then evaluate.  The work list is ordered by
 // decreasing cost within each level, so the four boxes of a wave cost about
-// the same.  PG <= 2.  W >= NQ (pmax + 2) entries per group.
+// the same.  PG <= 2.  Per group, W >= NQ (pmax + 2) entries of staged
+// form and then, for list 3, HE_RAW (>= 2 pmax + 1) raw coefficients.
+//
+// Latency (list 3): the source boxes' ids, levels, orders and centres are loaded
+// once per 16 boxes, one box per lane, and handed out by shuffles; the raw
+// coefficients of box k + 1 are loaded into registers (HE_PRE per lane,
+// orders up to 8 HE_PRE - 1; any beyond load at staging) while box k is
+// staged and evaluated, so no dependent chain of loads stalls the group
+// between boxes.
+constexpr int HE_PRE = 4;
+
 template <bool MPK, int PG, int NR>
 __global__ void __launch_bounds__(WAVE)
 k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int2* rng,
-          const double2* pts, int N, int nd, size_t per, const double2* ex, int W,
+          const double2* pts, int N, int nd, size_t per, const double2* ex, int W, int HE_RAW,
           double2* pot, double2* grad) {
   using T = SdTerm<PG>;
   extern __shared__ double2 shx[];
   const int grp = (int)threadIdx.x >> 4, gl = (int)threadIdx.x & 15;
   const int w = blockIdx.x * 4 + grp;
   if (w >= nwork) return;  // the whole group leaves; no block barriers below
-  cd* E = reinterpret_cast<cd*>(shx) + (size_t)grp * W;
+  cd* E = reinterpret_cast<cd*>(shx) + (size_t)grp * (W + HE_RAW);
+  cd* Raw = E + W;
   const int b = work[w];
   const int2 rg = rng[b];
   const cd i4 = mk(0, 0.25);
   const int nsrc = MPK ? D.o3[b + 1] - D.o3[b] : 1;
+  const int o3b = MPK ? D.o3[b] : 0;
   for (int i0 = rg.x; i0 < rg.y; i0 += 16) {
     const int i = i0 + gl;
     const bool act = i < rg.y;
     const double2 x = act ? pts[i] : make_double2(0.0, 0.0);
     for (int d = 0; d < nd; ++d) {
       EvalAcc A = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+      // box k0 + gl of the current 16: id, order, scale, centre
+      int ms = 0, mp = 0;
+      double mrs = 1.0;
+      double2 mc = make_double2(0.0, 0.0);
+      auto load_meta = [&](int k0) {
+        const int kk = k0 + gl;
+        if (kk < nsrc) {
+          ms = MPK ? D.l3[o3b + kk] : b;
+          const HLev lv = lev[D.level[ms]];
+          mp = lv.p;
+          mrs = lv.rs;
+          mc = D.ctr[ms];
+        }
+      };
+      // raw coefficients -p..p of a box, lane gl holding gl + 16 j
+      double2 pre[HE_PRE];
+      auto load_raw = [&](int s, int p) {
+        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1);
+#pragma unroll
+        for (int j = 0; j < HE_PRE; ++j) {
+          const int q = gl + 16 * j;
+          pre[j] = q < 2 * p + 1 ? e[q] : make_double2(0.0, 0.0);
+        }
+      };
+      if constexpr (MPK) {
+        load_meta(0);
+        load_raw(__shfl(ms, 0, 16), __shfl(mp, 0, 16));
+      }
       for (int k = 0; k < nsrc; ++k) {
-        const int s = MPK ? D.l3[D.o3[b] + k] : b;
-        const HLev lv = lev[D.level[s]];
-        const int p = lv.p, top = p + (PG >= 2 ? 1 : 0);
-        const double rsd = MPK ? lv.rs : 1.0 / lv.rs;
+        // the box's id, order, scale and centre: list 3 from the lanes' loads,
+        // the local expansion (one box, the group's own) directly
+        int s, p;
+        double rs;
+        double2 cc;
+        if constexpr (MPK) {
+          const int kl = k & 15;
+          s = __shfl(ms, kl, 16);
+          p = __shfl(mp, kl, 16);
+          rs = __shfl(mrs, kl, 16);
+          cc = make_double2(__shfl(mc.x, kl, 16), __shfl(mc.y, kl, 16));
+        } else {
+          s = b;
+          const HLev lv = lev[D.level[b]];
+          p = lv.p;
+          rs = lv.rs;
+          cc = D.ctr[b];
+        }
+        const int top = p + (PG >= 2 ? 1 : 0);
+        wave_sync();  // the group's previous readers of Raw and E are done
+        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1);
+        if constexpr (MPK) {
+#pragma unroll
+          for (int j = 0; j < HE_PRE; ++j) {
+            const int q = gl + 16 * j;
+            if (q < 2 * p + 1) Raw[q] = ld(&pre[j]);
+          }
+          for (int q = 16 * HE_PRE + gl; q < 2 * p + 1; q += 16) Raw[q] = ld(e + q);
+          // box k + 1: its loads are in flight while box k is staged and
+          // evaluated
+          if (k + 1 < nsrc) {
+            if (((k + 1) & 15) == 0) load_meta(k + 1);
+            const int k1 = (k + 1) & 15;
+            load_raw(__shfl(ms, k1, 16), __shfl(mp, k1, 16));
+          }
+          wave_sync();
+        }
+        const double rsd = MPK ? rs : 1.0 / rs;
         const cd z1 = zk * (0.5 / rsd), z2 = zk * (0.5 * rsd);
-        const double2* e = ex + (size_t)s * per + (size_t)d * (2 * p + 1) + p;  // order 0
-        auto co = [&](int q) { return (q < -p || q > p) ? mk(0, 0) : ld(e + q); };
+        // coefficient of order q: list 3 from the raw copy in LDS, the local
+        // expansion from memory
+        auto co = [&](int q) {
+          if (q < -p || q > p) return mk(0, 0);
+          if constexpr (MPK) return Raw[p + q];
+          else return ld(e + p + q);
+        };
         // the derivative expansions at order q (from orders q - 1 and q + 1)
         auto grads = [&](int q, cd& gx, cd& gy) {
           const cd a = co(q + 1), bm = co(q - 1);
@@ -759,7 +845,6 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
           else if (q < 0) { gx = z1 * a - z2 * bm; gy = mul_i(z1 * a + z2 * bm); }
           else { gx = z2 * (a - bm); gy = mul_i(z2 * (a + bm)); }
         };
-        wave_sync();  // the group's previous readers are done
         for (int n = gl; n <= top; n += 16) {
           cd* o = E + T::NQ * n;
           if (n == 0) {
@@ -789,17 +874,16 @@ k_heval_s(const int* work, int nwork, DTree D, const HLev* lev, cd zk, const int
         }
         wave_sync();
         if (act) {
-          const double2 cc = D.ctr[s];
           const double ddx = x.x - cc.x, ddy = x.y - cc.y;
           const double r = sqrt(ddx * ddx + ddy * ddy);
           const double ri = r > 0.0 ? 1.0 / r : 0.0;
           const cd e1 = r > 0.0 ? mk(ddx * ri, ddy * ri) : mk(1, 0);  // e^{i theta}
           if (NR > 0) {
-            if (MPK) eval_mp_sd_real<PG, (NR > 0 ? NR : 1)>(E, top, zk.x, lv.rs, r, e1, A);
-            else eval_lo_sd_real<PG>(E, top, zk.x, lv.rs, r, e1, A);
+            if (MPK) eval_mp_sd_real<PG, (NR > 0 ? NR : 1)>(E, top, zk.x, rs, r, e1, A);
+            else eval_lo_sd_real<PG>(E, top, zk.x, rs, r, e1, A);
           } else {
-            if (MPK) eval_mp_sd<PG>(E, top, zk, lv.rs, r, e1, A);
-            else eval_lo_sd<PG>(E, top, zk, lv.rs, r, e1, A);
+            if (MPK) eval_mp_sd<PG>(E, top, zk, rs, r, e1, A);
+            else eval_lo_sd<PG>(E, top, zk, rs, r, e1, A);
           }
         }
       }
@@ -1206,13 +1290,16 @@ void launch_heval(int pg, int nw, hipStream_t st, const int* work, const DTree& 
   // staged expansions (value and gradient, NQ entries per order) while
   // they fit comfortably in LDS
   const int W = (pg >= 2 ? 6 : 2) * (pmax + 2);
+  // raw coefficients of a list-3 box, 2 pmax + 1 rounded up (the local
+  // expansion stages from memory)
+  const int RAW = MPK ? 2 * pmax + 2 : 0;
   const bool staged = pg <= 2 && pmax <= 77;
-  const size_t shs = (size_t)4 * W * sizeof(double2);
+  const size_t shs = (size_t)4 * (W + RAW) * sizeof(double2);
 #define FMM_HEVAL(PGV, NRV)                                                                      \
   do {                                                                                           \
     if (staged && PGV <= 2)                                                                      \
       k_heval_s<MPK, (PGV <= 2 ? PGV : 2), NRV><<<nbx, WAVE, shs, st>>>(                         \
-          work, nw, D, lev, zk, rng, pts, N, nd, per, ex, W, pot, grad);                         \
+          work, nw, D, lev, zk, rng, pts, N, nd, per, ex, W, RAW, pot, grad);                    \
     else                                                                                         \
       k_heval<MPK, PGV, NRV><<<nbx, WAVE, 0, st>>>(work, nw, D, lev, zk, rng, pts, N, nd, per,   \
                                                    ex, pot, grad, hess);                         \
