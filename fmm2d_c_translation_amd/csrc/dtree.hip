@@ -148,17 +148,18 @@ __global__ void k_split_emit(const uint64_t* k, int n, int ndiv, int Ls, int L, 
     out[o++] = ((uint64_t)l << (2 * L)) | (k[i] >> (2 * (kMaxDepth - l)));
 }
 // used = the split keys emitted (the totals of the two scans, capped at
-// cap); pads [used, cap) with v, which sorts after every split key.  Thread
-// 0 records used, and cap in *nsplit when no unique pass follows.
+// cap).  With nsplit null, [used, cap) is padded with v, which sorts after
+// every split key (for a unique pass over all cap keys); else
+// *nsplit = used, and the sort takes that count from the device.
 __global__ void k_pad_tail(uint64_t* a, const int* tot_s, const int* tot_t, long cap, uint64_t v,
                            int* used, int* nsplit) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long u = min(cap, (long)(tot_s ? *tot_s : 0) + (tot_t ? *tot_t : 0));
   if (i == 0) {
     *used = (int)u;
-    if (nsplit) *nsplit = (int)cap;
+    if (nsplit) *nsplit = (int)u;
   }
-  if (i < cap && i >= u) a[i] = v;
+  if (!nsplit && i < cap && i >= u) a[i] = v;
 }
 
 // Level-restriction closure in one pass.  A split box r of level k forces
@@ -792,7 +793,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
                                                       use_s ? os + ns : nullptr, cap, Re);
       }
     }
-    k_pad_tail<<<(unsigned)((cap + 255) / 256), 256, 0, c.st>>>(
+    k_pad_tail<<<idivflag == 2 ? (unsigned)((cap + 255) / 256) : 1u, 256, 0, c.st>>>(
         Re, Ls >= 0 && use_s ? os + ns : nullptr, Ls >= 0 && use_t ? ot + nt : nullptr, cap, kPad,
         d_used, idivflag == 2 ? nullptr : d_nsplit);
     if (idivflag == 2) {
@@ -802,7 +803,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
       if (e) return e;
       HIP_TRY(hipcub::DeviceSelect::Unique(tmp, tbu, Rt, Rs, d_nsplit, (int)cap, c.st));
     } else {
-      e = rsort::radix_sort(c, S, Re, Rs, nullptr, nullptr, (int)cap, 0, kbits, plan.sort_err);
+      // the emitted keys only (their count on the device): the capacity is
+      // a bound several times the usual count
+      e = rsort::radix_sort(c, S, Re, Rs, nullptr, nullptr, (int)cap, 0, kbits, plan.sort_err,
+                            d_nsplit);
       if (e) return e;
     }
     k_split_levels<<<1, 64, 0, c.st>>>(Rs, d_nsplit, L, d_roff);

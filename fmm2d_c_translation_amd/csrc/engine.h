@@ -126,6 +126,9 @@ struct ForestIn {
   bool frames = false;
   double min_bs0 = 0;      // smallest root box, kernel coordinates (the near field's skip threshold)
   int lfirst = 2;          // 2 below the highest root (the smallest m)
+  // levels the first point sort orders (then every level if the tree turns
+  // out deeper): an estimate of the forest's depth, kMaxDepth for all bits
+  int sort_levels = 29;
   ForestDev dev;
 };
 

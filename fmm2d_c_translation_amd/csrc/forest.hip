@@ -436,6 +436,20 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
   fin.frames = true;
   fin.min_bs0 = min_b0;
   fin.lfirst = mmin + 2;
+  // depth estimate for the first point sort: each problem's root level,
+  // plus the levels a uniform tree of its points would have, plus five for
+  // clustering; rounded up to what the same number of 8-bit passes covers
+  // (2 levels + the target flag: 15, 19, 23, 27 levels in 4..7 passes).  A
+  // deeper tree sorts again on every bit.
+  {
+    int est = 0;
+    for (int i = 0; i < np; ++i) {
+      const double n = (double)probs[g[i]].ns + (double)probs[g[i]].nt;
+      const int lu = (int)std::ceil(std::log(std::max(n / std::max(ndiv, 1), 1.0)) / std::log(4.0));
+      est = std::max(est, fp[i].m + lu + 1 + 5);
+    }
+    fin.sort_levels = est <= 15 ? 15 : est <= 19 ? 19 : est <= 23 ? 23 : est <= 27 ? 27 : kMaxDepth;
+  }
   fin.dev.M = M;
   fin.dev.slot = d_slots;
   // dipole strengths of normalised problems times a

@@ -624,8 +624,8 @@ int build_plan(Ctx& c, Scratch& S, const double* d_src, int ns, const double* d_
   P.perm_t = P.perm_s + ns;
   double tt[3] = {0, 0, 0};
   int e = IER_OK;
-  // (a forest's trees sit below its root cells: every key bit, at once)
-  for (int levels : {fin ? kMaxDepth : kSortLevels, kMaxDepth}) {
+  // (a forest's trees sit below its root cells: the forest's estimate)
+  for (int levels : {fin ? std::min(fin->sort_levels, kMaxDepth) : kSortLevels, kMaxDepth}) {
     const int begin_bit = 2 * (kMaxDepth - levels);
     uint64_t *ks = nullptr, *kt = nullptr;
     e = sort_points(c, S, src, ns, trg, nt, d_geo, begin_bit, plan.sort_err, P.perm_s, &ks,

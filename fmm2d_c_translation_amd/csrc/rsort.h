@@ -55,8 +55,10 @@ __global__ void k_rs_init(uint32_t* state, size_t nwords) {
 // 256 for small ones, whose blocks then also fit beside other streams' work
 template <class K, int TH>
 __global__ void __launch_bounds__(TH)
-k_rs_hist(const K* __restrict__ key, int n, int begin, int end, int npass, uint32_t* hist) {
+k_rs_hist(const K* __restrict__ key, int n, const int* n_dev, int begin, int end, int npass,
+          uint32_t* hist) {
   __shared__ uint32_t h[8 * 256];
+  if (n_dev) n = min(n, *n_dev);
   for (int i = threadIdx.x; i < npass * 256; i += TH) h[i] = 0;
   __syncthreads();
   for (long i = (long)blockIdx.x * TH + threadIdx.x; i < n; i += (long)gridDim.x * TH) {
@@ -75,8 +77,8 @@ k_rs_hist(const K* __restrict__ key, int n, int begin, int end, int npass, uint3
 template <class K, bool V>
 __global__ void __launch_bounds__(kThreads)
 k_rs_pass(const K* __restrict__ kin, K* __restrict__ kout, const int* __restrict__ vin,
-          int* __restrict__ vout, int n, int sh, int wd, const uint32_t* hist, uint32_t* look,
-          uint32_t* counter, uint32_t* err) {
+          int* __restrict__ vout, int n, const int* n_dev, int sh, int wd, const uint32_t* hist,
+          uint32_t* look, uint32_t* counter, uint32_t* err) {
   __shared__ uint32_t s_cnt[kWaves][256];  // per wave and digit: keys ranked so far
   __shared__ uint32_t s_gofs[256];         // per digit: output position - tile slot
   __shared__ uint32_t s_wsum[2][kWaves];
@@ -89,6 +91,10 @@ k_rs_pass(const K* __restrict__ kin, K* __restrict__ kout, const int* __restrict
   for (int w = 0; w < kWaves; ++w) s_cnt[w][tid] = 0;
   __syncthreads();
   const int tile = s_tile;
+  // a count on the device: tiles past it leave at once (no tile before them
+  // waits on them)
+  if (n_dev) n = min(n, *n_dev);
+  if ((long)tile * kTile >= n) return;
   const long base = (long)tile * kTile + (long)wv * (kItems * 64) + lane;
   const uint32_t mask = (1u << wd) - 1;
   K k[kItems];
@@ -209,12 +215,15 @@ k_rs_pass(const K* __restrict__ kin, K* __restrict__ kout, const int* __restrict
 
 // Stable sort of n < 2^30 keys (and values, unless vin is null) by key bits
 // [begin, end) of K: kin/vin -> kout/vout, which must not overlap them.
+// n_dev (device memory, optional): the count actually sorted, min(n,
+// *n_dev), read by the kernels; n then only sizes the launch and the
+// buffers, and outputs past the count are left unwritten.
 // *err (device-visible: device or pinned host memory) is set to 1 if a
 // look-back wait gave up; the caller clears it and reads it after the
 // stream synchronises.
 template <class K>
 int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* vout, int n,
-               int begin, int end, uint32_t* err) {
+               int begin, int end, uint32_t* err, const int* n_dev = nullptr) {
   if (n <= 0) return IER_OK;
   if (begin < 0 || end <= begin || end > (int)(8 * sizeof(K)) || n >= (1 << 30) || !err ||
       (vin && !vout))
@@ -233,11 +242,11 @@ int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* v
   uint32_t* e = err;
   k_rs_init<<<(unsigned)std::min<size_t>(1024, (nwords + 255) / 256), 256, 0, c.st>>>(state, nwords);
   if (n >= (1 << 20))
-    k_rs_hist<K, 1024><<<std::min(256, (n + 4095) / 4096), 1024, 0, c.st>>>(kin, n, begin, end,
-                                                                          npass, hist);
+    k_rs_hist<K, 1024><<<std::min(256, (n + 4095) / 4096), 1024, 0, c.st>>>(kin, n, n_dev, begin,
+                                                                          end, npass, hist);
   else
-    k_rs_hist<K, 256><<<std::min(256, (n + 1023) / 1024), 256, 0, c.st>>>(kin, n, begin, end,
-                                                                        npass, hist);
+    k_rs_hist<K, 256><<<std::min(256, (n + 1023) / 1024), 256, 0, c.st>>>(kin, n, n_dev, begin,
+                                                                        end, npass, hist);
   const K* ks = kin;
   const int* vs = vin;
   for (int p = 0; p < npass; ++p) {
@@ -246,11 +255,11 @@ int radix_sort(Ctx& c, Scratch& S, const K* kin, K* kout, const int* vin, int* v
     int* vd = last ? vout : vb;
     const int sh = begin + 8 * p, wd = std::min(8, end - sh);
     if (vin)
-      k_rs_pass<K, true><<<ntiles, kThreads, 0, c.st>>>(ks, kd, vs, vd, n, sh, wd, hist + 256 * p,
+      k_rs_pass<K, true><<<ntiles, kThreads, 0, c.st>>>(ks, kd, vs, vd, n, n_dev, sh, wd, hist + 256 * p,
                                                         look + (size_t)p * ntiles * 256,
                                                         counter + p, e);
     else
-      k_rs_pass<K, false><<<ntiles, kThreads, 0, c.st>>>(ks, kd, nullptr, nullptr, n, sh, wd,
+      k_rs_pass<K, false><<<ntiles, kThreads, 0, c.st>>>(ks, kd, nullptr, nullptr, n, n_dev, sh, wd,
                                                          hist + 256 * p,
                                                          look + (size_t)p * ntiles * 256,
                                                          counter + p, e);
