@@ -338,6 +338,9 @@ struct Int4Sum {
 struct ListOut {
   int *o1, *o2, *o3, *o4, *l1, *l2, *l3, *l4;
   long cap1, cap2, cap3, cap4;  // entries each list array holds
+  // per box: bit (dy + 3) 7 + (dx + 3) for each list-2 box at cell offset
+  // (dx, dy) that holds sources (the M2L grouping key's mask), or null
+  uint64_t* m2l_mask;
 };
 
 // Lists of box b (tree_routs2d.c:369-510):
@@ -348,11 +351,12 @@ struct ListOut {
 //           touch b;
 //   list 3 (leaf b): children of split colleagues not touching b;
 //   list 4: leaf colleagues of the parent not touching b.
-// FILL = false counts into cnt[b]; FILL = true writes at the scanned offsets.
+// FILL = false counts into cnt[b]; FILL = true writes at the scanned offsets
+// (and the list-2 source mask when out.m2l_mask is set; srng: source ranges).
 template <bool FILL>
 __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild,
-                        const int4* child, const int* parent, const int* nbr, int nb,
-                        int4* cnt, const int4* off, ListOut out) {
+                        const int4* child, const int* parent, const int* nbr, const int2* srng,
+                        int nb, int4* cnt, const int4* off, ListOut out) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nb) return;
   if (b == nb) {
@@ -369,6 +373,7 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
     out.o1[b] = o.x; out.o2[b] = o.y; out.o3[b] = o.z; out.o4[b] = o.w;
   }
   int c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  uint64_t mask = 0;
   // an entry past the array's capacity is counted but not written; the
   // host then finds the total above the capacity and rejects the tree
   auto put = [&](int* lst, long cap, int base, int& c, int v) {
@@ -392,7 +397,13 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
         for (int q = 0; q < 4; ++q) {
           int kx, ky;
           key_cell(key[kk[q]], kx, ky);
-          if (abs(kx - ix) >= 2 || abs(ky - iy) >= 2) put(out.l2, out.cap2, o.y, c2, kk[q]);
+          if (abs(kx - ix) >= 2 || abs(ky - iy) >= 2) {
+            put(out.l2, out.cap2, o.y, c2, kk[q]);
+            if (FILL && out.m2l_mask) {
+              const int2 r = srng[kk[q]];
+              if (r.y > r.x) mask |= 1ull << ((ky - iy + 3) * 7 + (kx - ix + 3));
+            }
+          }
         }
       } else if (s != 4) {
         int px, py;
@@ -423,6 +434,7 @@ __global__ void k_lists(const uint64_t* key, const int* level, const int* nchild
     }
   }
   if (!FILL) cnt[b] = make_int4(c1, c2, c3, c4);
+  if (FILL && out.m2l_mask) out.m2l_mask[b] = mask;
 }
 
 // ---- work lists: one ordered multi-list compaction over the boxes
@@ -644,21 +656,15 @@ __global__ void k_wl_levels(Levels lv, const int* info, const int* out, int* lev
 
 // M2L grouping key of a target: its level (5 bits) above a 19-bit hash
 // (Fibonacci hashing) of the 49-bit mask of the list-2 offsets (slots) whose
-// source box holds sources.  Equal masks get equal keys; a rare collision
-// only puts two masks in one group.
+// source box holds sources (k_lists' m2l_mask).  Equal masks get equal keys;
+// a rare collision only puts two masks in one group.
 constexpr int kM2lKeyBits = 5 + 19;
-__global__ void k_m2l_keys(Levels lv, const double2* ctr, const int* o2, const int* l2,
-                           const int2* srng, const int* list, int n, uint32_t* key, int* val) {
+__global__ void k_m2l_keys(Levels lv, const uint64_t* m2l_mask, const int* list, int n,
+                           uint32_t* key, int* val) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int b = list[e], l = level_of(lv, b);
-  uint64_t mask = 0;
-  for (int k = o2[b]; k < o2[b + 1]; ++k) {
-    const int j = l2[k];
-    if (srng[j].y <= srng[j].x) continue;
-    const int sl = m2l_slot(ctr[j], ctr[b], lv.bs[l]);
-    if (sl >= 0) mask |= 1ull << sl;
-  }
+  const uint64_t mask = m2l_mask[b];
   key[e] = ((uint32_t)l << 19) | (uint32_t)((mask * 0x9e3779b97f4a7c15ull) >> (64 - 19));
   val[e] = b;
 }
@@ -916,9 +922,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   k_box_search<<<dim3(nblk(nb, 256), kBoxSearches), 256, 0, c.st>>>(
       lv, key, nb, ks, ns, kt, nt, reinterpret_cast<int*>(srng), reinterpret_cast<int*>(trng), nbr);
   // ---- lists 1-4 (count, scan, fill)
-  ListOut lo{o, o + nb + 1, o + 2 * (nb + 1), o + 3 * (nb + 1), nullptr, nullptr, nullptr, nullptr};
-  k_lists<false><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
-                                                     cnt, nullptr, lo);
+  ListOut lo{o, o + nb + 1, o + 2 * (nb + 1), o + 3 * (nb + 1), nullptr, nullptr, nullptr, nullptr,
+             0, 0, 0, 0, nullptr};
+  k_lists<false><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr,
+                                                     srng, nb, cnt, nullptr, lo);
   size_t tb = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveScan(nullptr, tb, cnt, off, Int4Sum(), make_int4(0, 0, 0, 0), nb + 1, c.st));
   void* tmp = S.get<char>(tb);
@@ -939,9 +946,10 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
   lo.l2 = S.get<int>(lim[1]);
   lo.l3 = S.get<int>(lim[2]);
   lo.l4 = S.get<int>(lim[3]);
+  lo.m2l_mask = S.get<uint64_t>(nb);
   if (S.err) return S.err;
-  k_lists<true><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr, nb,
-                                                    nullptr, off, lo);
+  k_lists<true><<<nblk(nb + 1, 256), 256, 0, c.st>>>(key, level, nchild, child, parent, nbr,
+                                                    srng, nb, nullptr, off, lo);
   // ---- work lists
   const int nbk = nblk(nb, WLB);
   const int W = L + 2;
@@ -1014,8 +1022,7 @@ int build_dtree(Ctx& c, Scratch& S, const uint64_t* ks, int ns, const uint64_t* 
     uint32_t* k1 = S.get<uint32_t>(n);
     int* v0 = S.get<int>(n);
     if (S.err) return S.err;
-    k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, ctr, lo.o2, lo.l2, srng, work + plan.m2l.off,
-                                               n, k0, v0);
+    k_m2l_keys<<<nblk(n, 256), 256, 0, c.st>>>(lv, lo.m2l_mask, work + plan.m2l.off, n, k0, v0);
     const int kb = kM2lKeyBits;
     e = rsort::radix_sort(c, S, k0, k1, v0, work + plan.m2l.off, n, 0, kb, plan.sort_err);
     if (e) return e;

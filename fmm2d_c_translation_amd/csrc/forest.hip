@@ -43,33 +43,60 @@ namespace {
 
 inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
-// Root box of every problem: one block per problem over its sources and
-// targets, the reduction and the result as k_bbox_final's
-// (pts_tree2d.c:493-516): {centre x, centre y, size, unused}.
-__global__ void __launch_bounds__(256) k_bbox_probs(const double2* src, const double2* trg,
+// Root box of every problem (pts_tree2d.c:493-516): {centre x, centre y,
+// size, unused}.  The points of problem p (its sources, then its targets)
+// are cut into chunks of kBBChunk, one block each (blocks pfirst[p] to
+// pfirst[p+1] - 1), so a large problem does not hold the launch up; a
+// second kernel takes the extremes over each problem's chunks.
+constexpr int kBBChunk = 16384;
+
+__device__ inline void bb_add(double4& m, double2 q) {
+  m.x = fmin(m.x, q.x); m.y = fmax(m.y, q.x); m.z = fmin(m.z, q.y); m.w = fmax(m.w, q.y);
+}
+__device__ inline void bb_merge(double4& m, const double4& o) {
+  m.x = fmin(m.x, o.x); m.y = fmax(m.y, o.y); m.z = fmin(m.z, o.z); m.w = fmax(m.w, o.w);
+}
+
+__global__ void __launch_bounds__(256) k_bbox_parts(const double2* src, const double2* trg,
                                                     const int* soff, const int* toff,
-                                                    double4* out) {
-  const int p = blockIdx.x;
-  const int s0 = soff[p], s1 = soff[p + 1], t0 = toff[p], t1 = toff[p + 1];
+                                                    const int* pfirst, int np, double4* part) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = np;  // the problem whose chunks hold block b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pfirst[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const int p = lo;
+  const long s0 = soff[p], ns = soff[p + 1] - s0, t0 = toff[p];
+  const long n = ns + (toff[p + 1] - t0);
+  const long i0 = (long)(b - pfirst[p]) * kBBChunk, i1 = min(n, i0 + kBBChunk);
   double4 m = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-  for (int i = s0 + (int)threadIdx.x; i < s1; i += blockDim.x) {
-    const double2 q = src[i];
-    m.x = fmin(m.x, q.x); m.y = fmax(m.y, q.x); m.z = fmin(m.z, q.y); m.w = fmax(m.w, q.y);
-  }
-  for (int i = t0 + (int)threadIdx.x; i < t1; i += blockDim.x) {
-    const double2 q = trg[i];
-    m.x = fmin(m.x, q.x); m.y = fmax(m.y, q.x); m.z = fmin(m.z, q.y); m.w = fmax(m.w, q.y);
-  }
+  for (long i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    bb_add(m, i < ns ? src[s0 + i] : trg[t0 + (i - ns)]);
   __shared__ double4 sh[256];
   sh[threadIdx.x] = m;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double4 o = sh[threadIdx.x + s];
-      m.x = fmin(m.x, o.x); m.y = fmax(m.y, o.y); m.z = fmin(m.z, o.z); m.w = fmax(m.w, o.w);
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      bb_merge(m, sh[threadIdx.x + st]);
       sh[threadIdx.x] = m;
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) part[b] = m;
+}
+
+__global__ void __launch_bounds__(64) k_bbox_probs(const double4* part, const int* pfirst,
+                                                   double4* out) {
+  const int p = blockIdx.x;
+  double4 m = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+  for (int j = pfirst[p] + (int)threadIdx.x; j < pfirst[p + 1]; j += 64) bb_merge(m, part[j]);
+  for (int o = 32; o > 0; o >>= 1) {
+    double4 y;
+    y.x = __shfl_xor(m.x, o); y.y = __shfl_xor(m.y, o);
+    y.z = __shfl_xor(m.z, o); y.w = __shfl_xor(m.w, o);
+    bb_merge(m, y);
   }
   if (threadIdx.x == 0) {
     double bs0 = m.y - m.x;
@@ -295,8 +322,15 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
   char* chg = p0.ifcharge ? get(W.chg * NS) : nullptr;
   char* dip = p0.ifdipole ? get(W.dip * NS) : nullptr;
   char* dv = p0.ifdipole && W.dv ? get(W.dv * NS) : nullptr;
-  int* d_off = S.get<int>(2 * (size_t)(np + 1));
+  // chunks of the root-box search: blocks pfirst[p] .. pfirst[p + 1] - 1
+  std::vector<int> pfirst(np + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    const long long n = (long long)probs[g[i]].ns + probs[g[i]].nt;
+    pfirst[i + 1] = pfirst[i] + (int)std::max<long long>(1, (n + kBBChunk - 1) / kBBChunk);
+  }
+  int* d_off = S.get<int>(3 * (size_t)(np + 1));
   double4* d_bb = S.get<double4>(np);
+  double4* d_bbp = S.get<double4>(pfirst[np]);
   if (S.err) return S.err;
   Copier in(c.st, dev);
   for (int i = 0; i < np; ++i) {
@@ -310,16 +344,20 @@ int run_forest(fmm2dhip_problem* probs, const std::vector<int>& g, std::vector<i
   }
   if ((e = in.flush(S))) return e;
   {
-    int* h = static_cast<int*>(pinned(sizeof(int) * 2 * (np + 1), 4));
+    int* h = static_cast<int*>(pinned(sizeof(int) * 3 * (np + 1), 4));
     if (!h) return IER_ALLOC;
     std::memcpy(h, soff.data(), sizeof(int) * (np + 1));
     std::memcpy(h + np + 1, toff.data(), sizeof(int) * (np + 1));
-    HIP_TRY(hipMemcpyAsync(d_off, h, sizeof(int) * 2 * (np + 1), hipMemcpyHostToDevice, c.st));
+    std::memcpy(h + 2 * (np + 1), pfirst.data(), sizeof(int) * (np + 1));
+    HIP_TRY(hipMemcpyAsync(d_off, h, sizeof(int) * 3 * (np + 1), hipMemcpyHostToDevice, c.st));
   }
   const int* d_soff = d_off;
   const int* d_toff = d_off + np + 1;
-  k_bbox_probs<<<np, 256, 0, c.st>>>(reinterpret_cast<const double2*>(src),
-                                     reinterpret_cast<const double2*>(trg), d_soff, d_toff, d_bb);
+  const int* d_pfirst = d_off + 2 * (np + 1);
+  k_bbox_parts<<<pfirst[np], 256, 0, c.st>>>(reinterpret_cast<const double2*>(src),
+                                             reinterpret_cast<const double2*>(trg), d_soff, d_toff,
+                                             d_pfirst, np, d_bbp);
+  k_bbox_probs<<<np, 64, 0, c.st>>>(d_bbp, d_pfirst, d_bb);
   std::vector<double4> bb(np);
   {
     double4* h = static_cast<double4*>(pinned(sizeof(double4) * np, 3));
