@@ -170,7 +170,10 @@ FMM_HD void hank_asym(cd z, cd& h0, cd& h1) {
 
 FMM_HD void hank_upper(cd z, cd& h0, cd& h1) {
   double a = cabs_(z);
-  if (a <= 4.0) hank_series(z, h0, h1);
+  // a positive real argument within the series' reach: the real series
+  // (same function, real arithmetic, a fixed number of terms)
+  if (z.y == 0.0 && z.x > 0.0 && z.x <= JY_XMAX3) hank_series_real<JY_NTERMS3>(z.x, h0, h1);
+  else if (a <= 4.0) hank_series(z, h0, h1);
   else if (a < 20.0) hank_laguerre(z, h0, h1);
   else hank_asym(z, h0, h1);
 }
