@@ -11,6 +11,9 @@
 //                     with a 20-point Gauss-Laguerre rule (gl20.h)
 //   |z| >= 20         Hankel's asymptotic expansion (A&S 9.2.7)
 // in the upper half plane; below it H^(1)(z) = 2 J(z) - conj(H^(1)(conj z)).
+// A positive real argument above 4 takes the modulus form instead
+// (hank_modulus_real: P_nu, Q_nu fitted in 4/x, hank_asym.h), in real
+// arithmetic.
 // Scaled sequences as in the reference (c_translation/src/h2dcommon.c:35-90,
 // cdjseval2d.c:11-164): H_n(z) rscale^n by upward recurrence, J_n(z)/rscale^n
 // by Miller's downward recurrence normalised through the generating
@@ -19,6 +22,7 @@
 
 #include "common.h"
 #include "gl20.h"
+#include "hank_asym.h"
 #include "jy_series.h"
 
 namespace fmm {
@@ -168,12 +172,39 @@ FMM_HD void hank_asym(cd z, cd& h0, cd& h1) {
   h1 = pre * cexp_(mul_i(z - mk(3 * kPi / 4, 0))) * s1;
 }
 
+// H0, H1 of a real x > 4 in the modulus form of Hankel's expansion
+// (A&S 9.2.5-9.2.6): H_nu = sqrt(2/(pi x)) (P_nu + i Q_nu) e^{i(x - (2nu+1)pi/4)},
+// P_nu, Q_nu polynomials in s = 8/x - 1 (hank_asym.h).  The phase factor
+// comes from sin x and cos x, e^{i(x - pi/4)} = ((cos x + sin x) + i (sin x -
+// cos x)) / sqrt 2, so no rounded x - pi/4 enters it; e^{i(x - 3pi/4)} is
+// -i times that.
+template <int n>
+FMM_HD double horner_c(const double (&c)[n], double s) {
+  double v = c[n - 1];
+#pragma unroll
+  for (int k = n - 2; k >= 0; --k) v = fma(v, s, c[k]);
+  return v;
+}
+FMM_HD void hank_modulus_real(double x, cd& h0, cd& h1) {
+  const double u = 4.0 / x, s = fma(2.0, u, -1.0);
+  const double p0 = horner_c(HA_P0, s), q0 = horner_c(HA_Q0, s);
+  const double p1 = horner_c(HA_P1, s), q1 = horner_c(HA_Q1, s);
+  double sn, cs;
+  sincos(x, &sn, &cs);
+  const double a = sqrt(u * (0.5 / kPi));  // sqrt(2 / (pi x))
+  const double cr = (cs + sn) * (a * 0.70710678118654752440), ci = (sn - cs) * (a * 0.70710678118654752440);
+  h0 = cd{p0 * cr - q0 * ci, p0 * ci + q0 * cr};
+  h1 = cd{p1 * ci + q1 * cr, q1 * ci - p1 * cr};
+}
+
 FMM_HD void hank_upper(cd z, cd& h0, cd& h1) {
   double a = cabs_(z);
-  // a positive real argument within the series' reach: the real series
-  // (same function, real arithmetic, a fixed number of terms)
-  if (z.y == 0.0 && z.x > 0.0 && z.x <= JY_XMAX3) hank_series_real<JY_NTERMS3>(z.x, h0, h1);
-  else if (a <= 4.0) hank_series(z, h0, h1);
+  // a positive real argument: the real series up to 4 (same function, real
+  // arithmetic, a fixed number of terms), the modulus form beyond
+  if (z.y == 0.0 && z.x > 0.0) {
+    if (z.x <= JY_XMAX3) hank_series_real<JY_NTERMS3>(z.x, h0, h1);
+    else hank_modulus_real(z.x, h0, h1);
+  } else if (a <= 4.0) hank_series(z, h0, h1);
   else if (a < 20.0) hank_laguerre(z, h0, h1);
   else hank_asym(z, h0, h1);
 }

@@ -9,7 +9,10 @@ written by csrc/gen_jy_series.py), checked on the CPU:
   2. H0 and H1 assembled from them as the device assembles them
      (special.h, hank_series_real_w) against scipy's Bessel functions, an
      independent implementation: a dropped term, a wrong sign or a wrong
-     tier range fails here."""
+     tier range fails here;
+  3. the modulus-form tables for real x > 4 (csrc/hank_asym.h, written by
+     csrc/gen_hank_asym.py) assembled into H0 and H1 as the device does,
+     against scipy."""
 import os
 import re
 from fractions import Fraction
@@ -87,3 +90,34 @@ def test_tier_hankel_values_match_scipy(tier):
                            (y1, sp.y1(x), "Y1")):
         err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
         assert err.max() < 4e-15, (tier, name, float(x[np.argmax(err)]), err.max())
+
+
+HA = os.path.join(ROOT, "fmm2d_c_translation_amd", "csrc", "hank_asym.h")
+
+
+def test_modulus_form_tables_match_scipy():
+    """H0 and H1 for real x > 4 from hank_asym.h, assembled as the device
+    assembles them (special.h, hank_modulus_real), against scipy's Hankel
+    functions (an independent implementation), from x = 4 to 1e4."""
+    txt = open(HA).read()
+    tab = {m.group(1): [float(v) for v in m.group(2).split(",")]
+           for m in re.finditer(r"HA_(\w+)\[\d+\] = \{([^}]*)\}", txt)}
+    assert set(tab) == {"P0", "Q0", "P1", "Q1"}
+    x = np.concatenate([np.geomspace(4.0 + 1e-9, 1e4, 3000), [4.0 + 1e-12, 8.0, 20.0]])
+    u = 4.0 / x
+    s = 2.0 * u - 1.0
+
+    def horner(c):
+        v = np.full_like(s, c[-1])
+        for a in reversed(c[:-1]):
+            v = v * s + a
+        return v
+
+    p0, q0, p1, q1 = (horner(tab[k]) for k in ("P0", "Q0", "P1", "Q1"))
+    a = np.sqrt(u * (0.5 / np.pi)) * np.sqrt(0.5)
+    cr, ci = (np.cos(x) + np.sin(x)) * a, (np.sin(x) - np.cos(x)) * a
+    h0 = (p0 * cr - q0 * ci) + 1j * (p0 * ci + q0 * cr)
+    h1 = (p1 * ci + q1 * cr) + 1j * (q1 * ci - p1 * cr)
+    for got, ref, name in ((h0, sp.hankel1(0, x), "H0"), (h1, sp.hankel1(1, x), "H1")):
+        err = np.abs(got - ref) / np.abs(ref)
+        assert err.max() < 3e-15, (name, float(x[np.argmax(err)]), err.max())
