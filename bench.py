@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE of k_hp2p, each scaled
 # by what the same counter reads for a 1 GiB device copy (x2.0 and x1.0):
 # profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md
-P2P_TRAFFIC_BYTES_1M = 784.8e6
+P2P_TRAFFIC_BYTES_1M = 789.1e6
 TRAFFIC_SOURCE = "profiles/hfmm2d_1M_zk40_eps1e-9_hbm_traffic.md"
 STEP_NAMES = ["form_mp", "form_lo", "m2m", "m2l", "l2l", "list3", "eval_lo", "p2p"]
 
