@@ -111,6 +111,29 @@ int upload(Scratch& S, hipStream_t st, const T* h, size_t n, const T** d) {
   return IER_OK;
 }
 
+// A solve's input arrays in host memory, moved together through the
+// thread's pinned staging (h2d_staged); add points *d at the device copy.
+struct Uploads {
+  std::vector<HostXfer> x;
+  template <class T>
+  int add(Scratch& S, const T* h, size_t n, const T** d) {
+    T* p = S.get<T>(n);
+    if (S.err) return S.err;
+    if (n) x.push_back(HostXfer{h, p, sizeof(T) * n});
+    *d = p;
+    return IER_OK;
+  }
+  int flush(hipStream_t st) { return h2d_staged(st, x.data(), (int)x.size()); }
+};
+// Output arrays to the callers' host memory, moved together (d2h_staged).
+struct Downloads {
+  std::vector<HostXfer> x;
+  void add(const void* d, void* h, size_t bytes) {
+    if (bytes) x.push_back(HostXfer{d, h, bytes});
+  }
+  int flush(hipStream_t st) { return d2h_staged(st, x.data(), (int)x.size()); }
+};
+
 }  // namespace
 
 // Laplace front end: arguments, host inputs to the device, point sort, tree
@@ -129,14 +152,16 @@ int lap_setup(const LapReq& r, Scratch& S, Prepared& q) {
   q.dip = r.dip;
   if (!r.device) {
     const size_t esz = r.realk ? sizeof(double) : sizeof(double2);
-    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &q.src))) return e;
-    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &q.trg))) return e;
+    Uploads up;
+    if ((e = up.add(S, r.src, 2 * (size_t)ns, &q.src))) return e;
+    if ((e = up.add(S, r.trg, 2 * (size_t)nt, &q.trg))) return e;
     const char *cc = static_cast<const char*>(r.chg), *qq = static_cast<const char*>(r.dip);
-    if (r.ifc && (e = upload(S, c.st, cc, esz * nd * ns, &cc))) return e;
-    if (r.ifd && (e = upload(S, c.st, qq, esz * nd * ns, &qq))) return e;
+    if (r.ifc && (e = up.add(S, cc, esz * nd * ns, &cc))) return e;
+    if (r.ifd && (e = up.add(S, qq, esz * nd * ns, &qq))) return e;
     q.chg = cc;
     q.dip = qq;
-    if (split && r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+    if (split && r.ifd && (e = up.add(S, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+    if ((e = up.flush(c.st))) return e;
   }
   return build_plan(c, S, q.src, ns, q.trg, nt, r.idivflag, r.ndiv, q.P, q.plan, true, r.pg,
                     r.pgt, r.forest);
@@ -221,6 +246,7 @@ int lap_eval(const LapReq& r, Scratch& S, Prepared& q) {
   e = lap_run(c, S, P, plan, a);
   if (e) return e;
   const bool fused[2] = {a.fused_s, a.fused_t};
+  Downloads down;
   for (int k = 0; k < 2; ++k) {
     const Out& o = outs[k];
     if (o.flag <= 0 || o.n == 0) continue;
@@ -241,12 +267,12 @@ int lap_eval(const LapReq& r, Scratch& S, Prepared& q) {
       if (o.flag >= 3) scatter_channels(c.st, o.sh, static_cast<double2*>(o.dh), o.perm, o.n, nd, 1);
     }
     if (!r.device) {
-      HIP_TRY(hipMemcpyAsync(o.up, o.dp, esz * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, o.dg, esz * ng, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, esz * nh, hipMemcpyDeviceToHost, c.st));
+      down.add(o.dp, o.up, esz * np);
+      if (o.flag >= 2) down.add(o.dg, o.ug, esz * ng);
+      if (o.flag >= 3) down.add(o.dh, o.uh, esz * nh);
     }
   }
-  (void)e;
+  if (!r.device && (e = down.flush(c.st))) return e;
   HIP_TRY(hipGetLastError());
   return IER_OK;
 }
@@ -297,11 +323,13 @@ int helm_setup(const HelmReq& r, Scratch& S, Prepared& q) {
   q.dip = r.dip;
   if (!r.device) {
     const double2 *cc = static_cast<const double2*>(r.chg), *qq = static_cast<const double2*>(r.dip);
-    if ((e = upload(S, c.st, r.src, 2 * (size_t)ns, &q.src))) return e;
-    if ((e = upload(S, c.st, r.trg, 2 * (size_t)nt, &q.trg))) return e;
-    if (r.ifc && (e = upload(S, c.st, cc, (size_t)nd * ns, &cc))) return e;
-    if (r.ifd && (e = upload(S, c.st, qq, (size_t)nd * ns, &qq))) return e;
-    if (r.ifd && (e = upload(S, c.st, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+    Uploads up;
+    if ((e = up.add(S, r.src, 2 * (size_t)ns, &q.src))) return e;
+    if ((e = up.add(S, r.trg, 2 * (size_t)nt, &q.trg))) return e;
+    if (r.ifc && (e = up.add(S, cc, (size_t)nd * ns, &cc))) return e;
+    if (r.ifd && (e = up.add(S, qq, (size_t)nd * ns, &qq))) return e;
+    if (r.ifd && (e = up.add(S, r.dipvec, 2 * (size_t)nd * ns, &q.dv))) return e;
+    if ((e = up.flush(c.st))) return e;
     q.chg = cc;
     q.dip = qq;
   }
@@ -363,6 +391,7 @@ int helm_eval(const HelmReq& r, Scratch& S, Prepared& q) {
   e = helm_run(c, S, P, plan, a);
   if (e) return e;
   const bool fused[2] = {a.fused_s, a.fused_t};
+  Downloads down;
   for (int k = 0; k < 2; ++k) {
     const Out& o = outs[k];
     if (o.flag <= 0 || o.n == 0) continue;
@@ -373,12 +402,12 @@ int helm_eval(const HelmReq& r, Scratch& S, Prepared& q) {
       if (o.flag >= 3) scatter_channels(c.st, o.sh, o.dh, o.perm, o.n, nd, 3);
     }
     if (!r.device) {
-      HIP_TRY(hipMemcpyAsync(o.up, o.dp, sizeof(double2) * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 2) HIP_TRY(hipMemcpyAsync(o.ug, o.dg, sizeof(double2) * 2 * np, hipMemcpyDeviceToHost, c.st));
-      if (o.flag >= 3) HIP_TRY(hipMemcpyAsync(o.uh, o.dh, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c.st));
+      down.add(o.dp, o.up, sizeof(double2) * np);
+      if (o.flag >= 2) down.add(o.dg, o.ug, sizeof(double2) * 2 * np);
+      if (o.flag >= 3) down.add(o.dh, o.uh, sizeof(double2) * 3 * np);
     }
   }
-  (void)e;
+  if (!r.device && (e = down.flush(c.st))) return e;
   HIP_TRY(hipGetLastError());
   return IER_OK;
 }

@@ -199,7 +199,9 @@ constexpr int kMaxRootLevel = 16;
 class Copier {
  public:
   Copier(hipStream_t st, bool device) : st_(st), device_(device) {}
-  // dst (device) <- src (host or device); to_host: dst is the caller's
+  // dst <- src, one of them the caller's array (in device memory, or in
+  // host memory: then staged through pinned memory, h2d_staged and
+  // d2h_staged); to_host: dst is the caller's
   int add(const void* src, void* dst, size_t bytes, bool to_host) {
     if (!bytes) return IER_OK;
     if (device_) {
@@ -207,11 +209,15 @@ class Copier {
                        (long long)(bytes / sizeof(double))});
       return IER_OK;
     }
-    HIP_TRY(hipMemcpyAsync(dst, src, bytes, to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice,
-                           st_));
+    (to_host ? down_ : up_).push_back(HostXfer{src, dst, bytes});
     return IER_OK;
   }
   int flush(Scratch& S) {
+    int e = IER_OK;
+    if (!up_.empty() && (e = h2d_staged(st_, up_.data(), (int)up_.size()))) return e;
+    if (!down_.empty() && (e = d2h_staged(st_, down_.data(), (int)down_.size()))) return e;
+    up_.clear();
+    down_.clear();
     if (segs_.empty()) return IER_OK;
     for (size_t i0 = 0; i0 < segs_.size(); i0 += 65535) {
       const size_t n = std::min<size_t>(65535, segs_.size() - i0);
@@ -234,6 +240,7 @@ class Copier {
   hipStream_t st_;
   bool device_;
   std::vector<Seg> segs_;
+  std::vector<HostXfer> up_, down_;
 };
 
 // k_copy_segs over a device table of n segments, in launches of at most

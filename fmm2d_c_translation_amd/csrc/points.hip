@@ -7,6 +7,9 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <thread>
+#include <vector>
 
 #include "engine.h"
 #include "rsort.h"
@@ -99,6 +102,9 @@ struct Pinned {
   size_t cap[8] = {};
 };
 thread_local Pinned g_pinned;
+// the thread's staged DMAs in flight (the staging, pinned slot 0, is reused)
+thread_local hipEvent_t g_stage_done = nullptr;
+thread_local bool g_stage_pending = false;
 }  // namespace
 
 namespace {
@@ -227,6 +233,10 @@ void release_thread() {
     for (auto& e : g_timer.ev) (void)hipEventDestroy(e);
     g_timer = StepTimer{};
   }
+  if (g_stage_pending) (void)hipEventSynchronize(g_stage_done);
+  g_stage_pending = false;
+  if (g_stage_done) (void)hipEventDestroy(g_stage_done);
+  g_stage_done = nullptr;
   for (int k = 0; k < 8; ++k) {
     if (g_pinned.buf[k]) (void)hipHostFree(g_pinned.buf[k]);
     g_pinned.buf[k] = nullptr;
@@ -275,6 +285,103 @@ void detach_workspace(std::vector<void*>& out) {
   A.blocks.clear();
   A.used = 0;
   A.total = 0;
+}
+
+namespace {
+
+// memcpy of a list of (src, dst, bytes) between the callers' arrays and the
+// staging (array i at stage + at[i], at[n] the staging's extent) on up to
+// eight threads, each taking a contiguous share of the staging's extent
+// (pieces of 4 MiB at least); small totals stay on the calling thread
+void par_copy(const HostXfer* x, int n, bool from_staging, const char* stage,
+              const size_t* at) {
+  const size_t total = at[n];
+  const size_t kPiece = size_t(4) << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  // helper threads across all callers (the batch workers copy at once) stay
+  // within the host's cores
+  static std::atomic<int> busy{0};
+  int want = (int)std::min<size_t>({8, hw, std::max<size_t>(1, total / kPiece)}) - 1;
+  const int cap = std::max(0, (int)std::min(16u, hw) - busy.load());
+  want = std::min(want, cap);
+  busy += want;
+  const int nth = 1 + want;
+  // bytes [b0, b1) of the concatenation
+  auto run = [&](size_t b0, size_t b1) {
+    for (int i = 0; i < n && b0 < b1; ++i) {
+      const size_t lo = at[i], hi = at[i] + x[i].bytes;
+      if (hi <= b0 || lo >= b1) continue;
+      const size_t a = std::max(lo, b0), b = std::min(hi, b1);
+      if (from_staging)
+        std::memcpy(static_cast<char*>(x[i].dst) + (a - lo), stage + a, b - a);
+      else
+        std::memcpy(const_cast<char*>(stage) + a, static_cast<const char*>(x[i].src) + (a - lo),
+                    b - a);
+    }
+  };
+  if (nth <= 1) {
+    run(0, total);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t share = (total + nth - 1) / nth;
+  for (int k = 1; k < nth; ++k)
+    th.emplace_back(run, std::min(total, k * share), std::min(total, (k + 1) * share));
+  run(0, std::min(total, share));
+  for (auto& t : th) t.join();
+  busy -= want;
+}
+
+int stage_wait() {
+  if (g_stage_pending) {
+    HIP_TRY(hipEventSynchronize(g_stage_done));
+    g_stage_pending = false;
+  }
+  return IER_OK;
+}
+
+}  // namespace
+
+int h2d_staged(hipStream_t st, const HostXfer* x, int n) {
+  if (capturing()) {
+    for (int i = 0; i < n; ++i)
+      if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, x[i].src, x[i].bytes, hipMemcpyHostToDevice, st));
+    return IER_OK;
+  }
+  std::vector<size_t> at(n + 1, 0);
+  for (int i = 0; i < n; ++i) at[i + 1] = at[i] + ((x[i].bytes + 255) & ~size_t(255));
+  if (!at[n]) return IER_OK;
+  int e = stage_wait();
+  if (e) return e;
+  char* stage = static_cast<char*>(pinned(at[n], 0));
+  if (!stage) return IER_ALLOC;
+  par_copy(x, n, false, stage, at.data());
+  for (int i = 0; i < n; ++i)
+    if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, stage + at[i], x[i].bytes, hipMemcpyHostToDevice, st));
+  if (!g_stage_done) HIP_TRY(hipEventCreateWithFlags(&g_stage_done, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(g_stage_done, st));
+  g_stage_pending = true;
+  return IER_OK;
+}
+
+int d2h_staged(hipStream_t st, const HostXfer* x, int n) {
+  if (capturing()) {
+    for (int i = 0; i < n; ++i)
+      if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, x[i].src, x[i].bytes, hipMemcpyDeviceToHost, st));
+    return IER_OK;
+  }
+  std::vector<size_t> at(n + 1, 0);
+  for (int i = 0; i < n; ++i) at[i + 1] = at[i] + ((x[i].bytes + 255) & ~size_t(255));
+  if (!at[n]) return IER_OK;
+  int e = stage_wait();
+  if (e) return e;
+  char* stage = static_cast<char*>(pinned(at[n], 0));
+  if (!stage) return IER_ALLOC;
+  for (int i = 0; i < n; ++i)
+    if (x[i].bytes) HIP_TRY(hipMemcpyAsync(stage + at[i], x[i].src, x[i].bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  par_copy(x, n, true, stage, at.data());
+  return IER_OK;
 }
 
 void* pinned(size_t bytes, int slot) {
