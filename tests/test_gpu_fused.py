@@ -142,6 +142,30 @@ def test_unfusable_problems_go_to_the_pool():
     _check_vs_alone(specs[:6], res[:6], "fusable part")
 
 
+def test_fused_deep_clusters_sort_again():
+    """Problems whose trees run deeper than the forest's first point sort
+    covers (a cluster of sigma 1e-4 inside a uniform background: some 20
+    levels below the root, past the estimate from the point counts), so the
+    forest sorts its points again on every bit; the results still equal the
+    solves alone and are within eps of direct sums."""
+    rng = np.random.default_rng(11)
+    specs = []
+    for i in range(6):
+        bg = rng.random((2, 3000))
+        cl = np.array([[0.3], [0.6]]) + 1e-4 * rng.standard_normal((2, 1000))
+        src = np.concatenate([bg, cl], axis=1)
+        n = src.shape[1]
+        kern = serve.CAUCHY if i % 2 == 0 else serve.HELMHOLTZ
+        s = dict(kernel=kern, eps=1e-9, sources=src, pg=2,
+                 charges=rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        if kern == serve.HELMHOLTZ:
+            s["zk"] = 5.0 + i
+        specs.append(s)
+    res = _run_fused(specs)
+    _check_vs_direct(specs, res, "deep clusters")
+    _check_vs_alone(specs, res, "deep clusters")
+
+
 def test_fused_device_arrays_equal_host_arrays():
     """device_ptrs = 1 (arrays in HBM in the drop-in layouts): the same
     result as from host arrays, bit for bit."""
