@@ -1684,7 +1684,7 @@ int helm_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const
     m2l_levels.push_back({W + sp.off, sp.n, ml, dT + tab[l].m2l[0], 2 * lev[l].p + 1, 0.0,
                           dT32 ? dT32 + tab[l].m2l[0] : nullptr, nullptr, 1});
   }
-  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
+  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd, &S));
   tm->mark(c.st);
   // step 5: L2L
   for (int l = lf; l <= L - 1; ++l) {

@@ -797,7 +797,7 @@ int lap_run(Ctx& c, Scratch& S, const SortedPoints& Pt, const Plan& plan, const 
     m2l_levels.push_back({W + sp.off, sp.n, ml, TM, P, std::log(plan.bs[l]),
                           a.mixed ? tab.m2l32 : nullptr, forest ? D.bsb : nullptr});
   }
-  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd));
+  if (!m2l_levels.empty()) HIP_TRY(launch_m2l_levels(c.st, m2l_levels, mp, per, lo, per, nd, &S));
   tm->mark(c.st);
   // step 5: L2L into the children of boxes that hold targets
   for (int l = 1; l <= L - 1; ++l) {

@@ -357,6 +357,8 @@ struct XLevel {
   const double* lap_bsb;  // per-box sizes for log(box size), or null (lap_log)
   int block0;
   int by;
+  int G;           // slot groups: blocks block0 + G b + g, partial sums in part
+  double2* part;   // G > 1: the level's partial sums (k_xlate_reduce adds them)
 };
 constexpr int kMaxXLevels = 30;
 struct XLevels {
@@ -370,8 +372,9 @@ k_xlate_mfma_levels(XLevels L, int nslot, const double2* __restrict__ X, size_t 
   int j = 0;
   while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
   const XLevel& v = L.lv[j];
-  xlate_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
-                  v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr, v.lap_bsb);
+  const int local = blockIdx.x - v.block0;
+  xlate_block<MW>(local / v.G, v.by, local % v.G, v.work, v.nwork, v.map, nslot, v.T, v.nin,
+                  v.nout, X, xstride, Y, ystride, nd, v.lap_log, v.G, v.part, v.lap_bsb);
 }
 
 // Y[work[w]] += sum over slot groups of the partials, group 0 first
@@ -546,6 +549,8 @@ struct XLevel32 {
   const double* lap_bsb;
   int block0;
   int by;
+  int G;
+  double2* part;
 };
 struct XLevels32 {
   XLevel32 lv[kMaxXLevels];
@@ -558,8 +563,9 @@ k_xlate_mfma32_levels(XLevels32 L, int nslot, const double2* __restrict__ X, siz
   int j = 0;
   while (j + 1 < L.n && (int)blockIdx.x >= L.lv[j + 1].block0) ++j;
   const XLevel32& v = L.lv[j];
-  xlate32_block<MW>(blockIdx.x - v.block0, v.by, 0, v.work, v.nwork, v.map, nslot, v.T, v.nin,
-                    v.nout, X, xstride, Y, ystride, nd, v.lap_log, 1, nullptr, v.lap_bsb);
+  const int local = blockIdx.x - v.block0;
+  xlate32_block<MW>(local / v.G, v.by, local % v.G, v.work, v.nwork, v.map, nslot, v.T, v.nin,
+                    v.nout, X, xstride, Y, ystride, nd, v.lap_log, v.G, v.part, v.lap_bsb);
 }
 
 // Translation through k_xlate_mfma when the output order fits eight row
@@ -629,8 +635,14 @@ struct M2LLevel {
   const double* lap_bsb = nullptr;
   int family = 0;  // for the tile choice: 0 Laplace and biharmonic, 1 Helmholtz
 };
+// With S, a launch group whose waves would not fill the GPU once (small
+// trees: a few long waves would set the time) splits every level's slots
+// over G groups of blocks, whose partial sums k_xlate_reduce then adds,
+// group 0 first.
 inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>& lv,
-                                    const double2* X, size_t xs, double2* Y, size_t ys, int nd) {
+                                    const double2* X, size_t xs, double2* Y, size_t ys, int nd,
+                                    Scratch* S = nullptr) {
+  constexpr int kFill = 2048;  // waves that keep every SIMD busy twice over
   // row tiles per wave: all of them up to 4, else two groups of ceil(MT/2)
   // (or the autotuner's choice, m2l_tiles)
   for (int mw = 1; mw <= 4; ++mw) {
@@ -638,6 +650,21 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
     XLevels32 L32;
     L.n = L32.n = 0;
     int nb = 0, nb32 = 0;
+    // waves of this launch group without a slot split
+    long waves = 0;
+    for (const M2LLevel& v : lv) {
+      const int MT = (v.P + 15) / 16;
+      if (v.nwork <= 0 || MT > 8 || m2l_tiles(MT, v.T32 != nullptr, v.family) != mw) continue;
+      waves += 4L * ((v.nwork + 15) / 16 + 3) / 4 * ((MT + mw - 1) / mw);
+    }
+    const int G = S && waves > 0 && waves < kFill ? (int)std::min<long>(8, (kFill + waves - 1) / waves)
+                                                  : 1;
+    struct Red {
+      const int* work;
+      int nwork, nout;
+      double2* part;
+    };
+    std::vector<Red> reds;
     auto flush = [&]() {
       if (L.n) switch (mw) {
         case 1: k_xlate_mfma_levels<1><<<nb, 256, 0, st>>>(L, 49, X, xs, Y, ys, nd); break;
@@ -659,20 +686,32 @@ inline hipError_t launch_m2l_levels(hipStream_t st, const std::vector<M2LLevel>&
       const bool f32 = v.T32 != nullptr;
       if (v.nwork <= 0 || MT > 8 || m2l_tiles(MT, f32, v.family) != mw) continue;
       last_stats().m2l_mt_mask |= uint64_t(1) << ((v.family * 2 + (f32 ? 1 : 0)) * 9 + MT);
-      const int nbl = ((v.nwork + 15) / 16 + 3) / 4;
+      const int nbl = ((v.nwork + 15) / 16 + 3) / 4 * G;
+      double2* part = nullptr;
+      if (G > 1) {  // shared by the level's row groups (disjoint rows)
+        part = S->get<double2>((size_t)G * nd * v.nwork * v.P);
+        if (!part) return hipErrorOutOfMemory;
+        reds.push_back(Red{v.work, v.nwork, v.P, part});
+      }
       for (int by = 0; by * mw < MT; ++by) {
         if (L.n == kMaxXLevels || L32.n == kMaxXLevels) flush();
         if (v.T32) {
-          L32.lv[L32.n++] =
-              XLevel32{v.work, v.nwork, v.map, v.T32, v.P, v.P, v.lap_log, v.lap_bsb, nb32, by};
+          L32.lv[L32.n++] = XLevel32{v.work, v.nwork, v.map, v.T32, v.P, v.P, v.lap_log,
+                                     v.lap_bsb, nb32, by, G, part};
           nb32 += nbl;
         } else {
-          L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, v.lap_bsb, nb, by};
+          L.lv[L.n++] = XLevel{v.work, v.nwork, v.map, v.T, v.P, v.P, v.lap_log, v.lap_bsb, nb, by,
+                               G, part};
           nb += nbl;
         }
       }
     }
     flush();
+    for (const Red& r : reds) {
+      const size_t ne = (size_t)nd * r.nwork * r.nout;
+      k_xlate_reduce<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(r.work, r.nwork, r.nout, nd, G,
+                                                                  r.part, Y, ys);
+    }
   }
   for (const M2LLevel& v : lv) {
     if (v.nwork <= 0 || (v.P + 15) / 16 <= 8) continue;
