@@ -232,7 +232,8 @@ void* pinned(size_t bytes, int slot);
 // caller's arrays may change after it); the DMAs stay on the stream, and
 // the next staged transfer on the thread waits for them.  d2h_staged
 // synchronises the stream and returns with the data in the caller's arrays.
-// Under a stream capture both fall back on plain copies.
+// Under a stream capture, above 1 GiB of staging, or without pinned memory
+// to be had, both fall back on plain copies.
 struct HostXfer {
   const void* src;
   void* dst;

@@ -342,19 +342,25 @@ int stage_wait() {
 
 }  // namespace
 
+// Transfers whose staging would exceed this (or that a stream capture
+// records) go as plain copies from and to the callers' memory.
+constexpr size_t kStageMax = size_t(1) << 30;
+
 int h2d_staged(hipStream_t st, const HostXfer* x, int n) {
-  if (capturing()) {
+  std::vector<size_t> at(n + 1, 0);
+  for (int i = 0; i < n; ++i) at[i + 1] = at[i] + ((x[i].bytes + 255) & ~size_t(255));
+  if (!at[n]) return IER_OK;
+  char* stage = nullptr;
+  if (!capturing() && at[n] <= kStageMax) {
+    int e = stage_wait();
+    if (e) return e;
+    stage = static_cast<char*>(pinned(at[n], 0));
+  }
+  if (!stage) {  // plain copies (no staging, or none to be had)
     for (int i = 0; i < n; ++i)
       if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, x[i].src, x[i].bytes, hipMemcpyHostToDevice, st));
     return IER_OK;
   }
-  std::vector<size_t> at(n + 1, 0);
-  for (int i = 0; i < n; ++i) at[i + 1] = at[i] + ((x[i].bytes + 255) & ~size_t(255));
-  if (!at[n]) return IER_OK;
-  int e = stage_wait();
-  if (e) return e;
-  char* stage = static_cast<char*>(pinned(at[n], 0));
-  if (!stage) return IER_ALLOC;
   par_copy(x, n, false, stage, at.data());
   for (int i = 0; i < n; ++i)
     if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, stage + at[i], x[i].bytes, hipMemcpyHostToDevice, st));
@@ -365,18 +371,21 @@ int h2d_staged(hipStream_t st, const HostXfer* x, int n) {
 }
 
 int d2h_staged(hipStream_t st, const HostXfer* x, int n) {
-  if (capturing()) {
-    for (int i = 0; i < n; ++i)
-      if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, x[i].src, x[i].bytes, hipMemcpyDeviceToHost, st));
-    return IER_OK;
-  }
   std::vector<size_t> at(n + 1, 0);
   for (int i = 0; i < n; ++i) at[i + 1] = at[i] + ((x[i].bytes + 255) & ~size_t(255));
   if (!at[n]) return IER_OK;
-  int e = stage_wait();
-  if (e) return e;
-  char* stage = static_cast<char*>(pinned(at[n], 0));
-  if (!stage) return IER_ALLOC;
+  char* stage = nullptr;
+  if (!capturing() && at[n] <= kStageMax) {
+    int e = stage_wait();
+    if (e) return e;
+    stage = static_cast<char*>(pinned(at[n], 0));
+  }
+  if (!stage) {  // plain copies (no staging, or none to be had)
+    for (int i = 0; i < n; ++i)
+      if (x[i].bytes) HIP_TRY(hipMemcpyAsync(x[i].dst, x[i].src, x[i].bytes, hipMemcpyDeviceToHost, st));
+    if (!capturing()) HIP_TRY(hipStreamSynchronize(st));
+    return IER_OK;
+  }
   for (int i = 0; i < n; ++i)
     if (x[i].bytes) HIP_TRY(hipMemcpyAsync(stage + at[i], x[i].src, x[i].bytes, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
